@@ -1,0 +1,167 @@
+/*
+ * trspmm — three-way routed SpMM for AMD Instinct MI355X (gfx950).
+ *
+ * C ABI of libtrspmm.so.  Plain pointers and sizes only; device pointers are
+ * HIP device memory, `stream` is a hipStream_t passed as void* (NULL = the
+ * null stream).  Every function returns 0 on success or a TRSPMM_E* code and
+ * leaves a message for trspmm_last_error() (per calling thread).  Calls are
+ * stream-ordered; only functions documented as synchronous block the host.
+ *
+ * Reference interfaces replaced (paths relative to the TR-SPMM sources):
+ *   trspmm_route          <- Libra5BlockOnline.block_sptc_2to4_online
+ *                            TR-source/Block/block_online.cpp:545-549, :873-876
+ *   trspmm_plan_create    <- the device upload + argument contract of
+ *                            spmm_forward_fp16_tcu_cuda_sptc_mma_parallel_online
+ *                            TR-source/SpMM/mGCN_online.cpp:760-1042
+ *   trspmm_plan_create_packed
+ *                         <- spmm_forward_fp16_sptc_mma_packed_online inputs
+ *                            TR-source/SpMM/mGCN_online.cpp:654-706
+ *   trspmm_spmm           <- spmm_forward_fp16_tcu_cuda_sptc_mma_parallel_online_kernel
+ *                            TR-source/SpMM/mGCNkernel_online.cu:1529-1727 (one launch,
+ *                            single-owner output rows instead of 4 streams + add)
+ *   trspmm_spmm_timed     <- the warm-up / timed-epoch loop of the same launcher
+ *                            (mGCNkernel_online.cu:1604-1704) and its ms outputs
+ * Entry points with no reference counterpart (north-star additions):
+ *   trspmm_sddmm, trspmm_plan_transpose, trspmm_batch_*, trspmm_host_sort_order.
+ */
+#ifndef TRSPMM_H
+#define TRSPMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    TRSPMM_OK = 0,
+    TRSPMM_EINVAL = 1,       /* bad argument (shape, range, dtype, ordering) */
+    TRSPMM_EHIP = 2,         /* a HIP runtime call failed */
+    TRSPMM_ENOMEM = 3,       /* device allocation failed */
+    TRSPMM_EUNSUPPORTED = 4  /* valid request this build does not implement */
+};
+
+enum { TRSPMM_F16 = 0, TRSPMM_BF16 = 1 };
+
+/* route masks for trspmm_spmm: which routes of the plan contribute */
+enum { TRSPMM_ROUTE_TC = 1, TRSPMM_ROUTE_SPTC = 2, TRSPMM_ROUTE_RESIDUAL = 4, TRSPMM_ROUTE_ALL = 7 };
+
+/* router output modes */
+enum {
+    TRSPMM_MODE_COMPAT = 0,  /* bit-exact with the reference (3rd/4th hits of a 2:4 row dropped) */
+    TRSPMM_MODE_EXACT = 1,   /* t_* / s_* bit-exact; dropped hits appended to the residual route */
+    TRSPMM_MODE_TUNED = 2    /* wave64 thresholds: low-occupancy 2:4 groups go to the residual route */
+};
+
+const char* trspmm_last_error(void);
+int trspmm_version(void);
+int trspmm_device_arch(char* buf, int len);   /* gcnArchName of the current device */
+
+/* ------------------------------------------------------------------ router */
+
+typedef struct trspmm_route_cfg {
+    int32_t part_t;         /* partSize_t: TC sub-blocks per part (32) */
+    int32_t part_c;         /* partSize_c: residual nonzeros per long part (32) */
+    int32_t short_c;        /* shortSize: residual parts of <= short_c go to the short list (3) */
+    int32_t density;        /* TC column threshold within a window (8) */
+    int32_t window;         /* rows per window, 1..16 (16) */
+    int32_t wide;           /* range-checked only, 1..32, as the reference (16) */
+    float tc_min_util;      /* TC chunk utilisation floor (0.0) */
+    int32_t skip_tc;        /* LIBRA_SKIP_TC */
+    int32_t unit_min_nnz;   /* LIBRA_SPTC_UNIT_MIN_NNZ (2) */
+    int32_t mode;           /* TRSPMM_MODE_* */
+    int32_t sptc_min_nnz;   /* TUNED: a 2:4 group with fewer nonzeros goes to the residual route */
+    int32_t reserved;
+} trspmm_route_cfg;
+
+typedef struct trspmm_meta trspmm_meta;
+
+/* Routes a canonical CSR (each row's columns strictly ascending) that lives
+ * in device memory.  `rows` is the (window-padded) row count, `ncols` the
+ * column count (rows of B).  Produces the reference's 29 metadata arrays on
+ * the device (index order of block_online.cpp:844-868).  Synchronous: the
+ * array sizes are known on the host when it returns. */
+int trspmm_route(const int32_t* d_rowptr, const int32_t* d_col, const float* d_val, int64_t rows,
+                 int64_t nnz, int64_t ncols, const trspmm_route_cfg* cfg, void* stream,
+                 trspmm_meta** out);
+int64_t trspmm_meta_count(const trspmm_meta* meta, int index);  /* elements of array `index` */
+void* trspmm_meta_data(const trspmm_meta* meta, int index);       /* device pointer */
+double trspmm_meta_seconds(const trspmm_meta* meta);             /* router wall time */
+void trspmm_meta_free(trspmm_meta* meta);
+
+/* ------------------------------------------------------------------ plans */
+
+/* Device views of the reference-format metadata (the op's inputs,
+ * mGCN_online.cpp:760-800).  Value arrays may be fp32 (router output) or
+ * fp16 (what the reference's caller passes, mdataset2_online.py:151-153);
+ * `*_value_is_f32` says which. */
+typedef struct trspmm_meta_view {
+    const int32_t* t_row_offset;   int64_t n_tparts;      /* [n_tparts + 1] */
+    const int32_t* t_block_offset;                         /* [n_tsub + 1]   */
+    const int32_t* t_column;       int64_t n_tsub;         /* [8 n_tsub]     */
+    const void* t_value;           int64_t n_tval;
+    const int32_t* t_window_row;                           /* [n_tparts]     */
+    const int64_t* t_binary;                               /* [2 n_tsub]     */
+    const int32_t* c_row_offset;   int64_t n_cparts;
+    const int32_t* c_row;
+    const int32_t* c_column;       int64_t n_cval;
+    const void* c_value;
+    const int32_t* cs_row_offset;  int64_t n_csparts;
+    const int32_t* cs_row;
+    const int32_t* cs_column;      int64_t n_csval;
+    const void* cs_value;
+    const int32_t* s_column;       int64_t n_groups;       /* [n_groups, 4]  */
+    const float* s_value;                                  /* [n_groups, window, 2] */
+    const int8_t* s_pos;                                   /* [n_groups, window, 2] */
+    const int32_t* s_offsets;      int64_t n_windows;      /* [n_windows + 1] */
+    int32_t t_value_is_f32, c_value_is_f32;
+    int32_t window;
+    int32_t reserved;
+} trspmm_meta_view;
+
+typedef struct trspmm_plan trspmm_plan;
+
+/* Builds the device plan (AMD lane layouts) from reference-format metadata.
+ * m_out: output rows (mOri); ncols: rows of B (kOri); dtype: TRSPMM_F16/BF16. */
+int trspmm_plan_create(const trspmm_meta_view* view, int64_t m_out, int64_t ncols, int dtype,
+                       void* stream, trspmm_plan** out);
+/* Same, from the router's own output. */
+int trspmm_plan_from_meta(const trspmm_meta* meta, int64_t m_out, int64_t ncols, int dtype,
+                          void* stream, trspmm_plan** out);
+/* 2:4 route only, from the mma.sp packed tiles (fp16 values). */
+int trspmm_plan_create_packed(const int32_t* s_tile_column, const int32_t* s_packed_a,
+                              const int32_t* s_packed_meta, const int32_t* s_window_tile_offset,
+                              int64_t n_tiles, int64_t n_windows, int32_t window, int64_t m_out,
+                              int64_t ncols, void* stream, trspmm_plan** out);
+void trspmm_plan_free(trspmm_plan* plan);
+/* counts: [windows, tc sub-blocks, tc nnz, 2:4 tiles, residual parts, residual nnz] */
+int trspmm_plan_stats(const trspmm_plan* plan, int64_t* counts6);
+
+/* ------------------------------------------------------------------ multiply */
+
+/* C[m_out, K] (ldc) = A_routed · B[ncols, K] (ldb), FP32 accumulate, fp32
+ * output, every output row written once.  B dtype = plan dtype.  K % 8 == 0,
+ * ldb % 8 == 0, ldc % 4 == 0, B and C 16-byte aligned. */
+int trspmm_spmm(const trspmm_plan* plan, const void* B, int64_t ldb, int64_t K, float* C,
+                int64_t ldc, uint32_t route_mask, void* stream);
+
+/* The reference's timing loop: `warmup` untimed launches, then `iters` timed
+ * ones with hipEvents; *ms_avg = mean ms per launch.  Synchronous. */
+int trspmm_spmm_timed(const trspmm_plan* plan, const void* B, int64_t ldb, int64_t K, float* C,
+                      int64_t ldc, uint32_t route_mask, int warmup, int iters, void* stream,
+                      float* ms_avg);
+
+/* Copies `bytes` between any host/device addresses, ordered on `stream`;
+ * returns after the copy completed (used to hand router output to callers). */
+int trspmm_memcpy(void* dst, const void* src, int64_t bytes, void* stream);
+
+/* ------------------------------------------------------------------ diagnostics */
+
+/* libstdc++ std::sort order of iota(n) by key descending, computed by the
+ * device router's emulation, run on the host (used by tests). */
+int trspmm_host_sort_order(const int32_t* key, int32_t n, int32_t* order);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TRSPMM_H */

@@ -1,0 +1,208 @@
+"""TEST INFRASTRUCTURE ONLY — the oracle.
+
+CPU restatement of the reference router (TR-source/Block/block_online.cpp:545-871)
+plus FP64 products over the routed metadata.  Imported only by tests/, the
+smoke check in __graft_entry__.py and bench.py's CPU-baseline leg, always as
+the checker; the product package under tr-spmm_amd/ never imports it.
+
+Functions
+---------
+route(...)        -> RoutedMeta   (29 arrays in the reference's return order)
+decode(meta)      -> (rows, cols, vals) COO triplets the metadata encodes
+spmm64(meta, B)   -> FP64 C = A_routed @ B
+csr_spmm64(...)   -> FP64 C = A @ B for a plain CSR (torch.sparse / scipy)
+sort_order(keys)  -> libstdc++ std::sort order of iota(n) by key desc
+f32_to_f16_bits   -> IEEE RNE binary16 bits (the router's packing conversion)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+NAMES = [
+    "t_row_offset", "t_block_offset", "t_column", "t_value", "t_window_row", "t_atomic", "t_binary",
+    "c_row_offset", "c_row", "c_atomic", "c_column", "c_value",
+    "c_row_offset_short", "c_row_short", "c_atomic_short", "c_column_short", "c_value_short",
+    "s_column", "s_mask", "s_value", "s_pos", "s_window", "s_offsets",
+    "s_tile_window", "s_tile_group", "s_tile_column", "s_packed_a", "s_packed_meta",
+    "s_window_tile_offset",
+]
+DTYPES = [
+    np.int32, np.int32, np.int32, np.float32, np.int32, np.int32, np.int64,
+    np.int32, np.int32, np.int32, np.int32, np.float32,
+    np.int32, np.int32, np.int32, np.int32, np.float32,
+    np.int32, np.uint8, np.float32, np.int8, np.int32, np.int32,
+    np.int32, np.int32, np.int32, np.int32, np.int32,
+    np.int32,
+]
+
+
+def _shape(name: str, n: int, window: int):
+    return {
+        "s_column": (-1, 4), "s_mask": (-1, window), "s_value": (-1, window, 2), "s_pos": (-1, window, 2),
+        "s_tile_column": (-1, 16), "s_packed_a": (-1, 8, 4, 2), "s_packed_meta": (-1, 8),
+    }.get(name, (n,))
+
+
+def lib():
+    """Loads oracle/liboracle.so, building it with make if it is absent."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = os.path.join(_HERE, "liboracle.so")
+    if not os.path.exists(path):
+        subprocess.run(["make", "-C", _HERE, "liboracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+    L = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    L.oracle_route.restype = P
+    L.oracle_route.argtypes = [P, P, P, ctypes.c_int64] + [ctypes.c_int] * 6 + \
+        [ctypes.c_float] + [ctypes.c_int] * 4
+    L.oracle_error.restype = ctypes.c_char_p
+    L.oracle_error.argtypes = [P]
+    L.oracle_get.restype = ctypes.c_int
+    L.oracle_get.argtypes = [P, ctypes.c_int, ctypes.POINTER(P), ctypes.POINTER(ctypes.c_int64)]
+    L.oracle_duration.restype = ctypes.c_double
+    L.oracle_duration.argtypes = [P]
+    L.oracle_free.argtypes = [P]
+    L.oracle_decode.restype = ctypes.c_int64
+    L.oracle_decode.argtypes = [P, ctypes.c_int64, P, P, P, P, P,
+                                P, P, ctypes.c_int64, P, P,
+                                P, P, ctypes.c_int64, P, P,
+                                P, P, P, P, ctypes.c_int64,
+                                ctypes.c_int, P, P, P, ctypes.c_int64]
+    L.oracle_sort_order.argtypes = [P, ctypes.c_int32, P]
+    L.oracle_f32_to_f16.argtypes = [P, P, ctypes.c_int64]
+    _LIB = L
+    return L
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p) if a.size else None
+
+
+@dataclass
+class RoutedMeta:
+    arrays: dict = field(default_factory=dict)
+    window: int = 16
+    duration: float = 0.0
+
+    def __getitem__(self, k):
+        return self.arrays[k]
+
+    def as_list(self):
+        return [self.arrays[n] for n in NAMES]
+
+
+def env_knobs():
+    """LIBRA_SKIP_TC / LIBRA_SPTC_UNIT_MIN_NNZ parsed as block_online.cpp:118-128 does."""
+    v = os.environ.get("LIBRA_SKIP_TC")
+    skip = v in ("1", "true", "TRUE") if v is not None else False
+    u = os.environ.get("LIBRA_SPTC_UNIT_MIN_NNZ")
+    unit = int(u) if u is not None and u.strip().lstrip("-").isdigit() else (2 if u is None else 0)
+    return skip, unit
+
+
+def route(rowptr, col, val, partsize_t=32, partsize_c=32, shortsize=3, density=8, window=16, wide=16,
+          tc_min_util=0.0, skip_tc=None, unit_min_nnz=None, exact=False, faithful=False) -> RoutedMeta:
+    L = lib()
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    val = np.ascontiguousarray(val, dtype=np.float32)
+    es, eu = env_knobs()
+    skip_tc = es if skip_tc is None else bool(skip_tc)
+    unit_min_nnz = eu if unit_min_nnz is None else int(unit_min_nnz)
+    h = L.oracle_route(_ptr(rowptr), _ptr(col), _ptr(val), rowptr.size - 1,
+                       partsize_t, partsize_c, shortsize, density, window, wide, float(tc_min_util),
+                       int(skip_tc), unit_min_nnz, int(bool(exact)), int(bool(faithful)))
+    try:
+        err = L.oracle_error(h)
+        if err:
+            raise RuntimeError(err.decode())
+        out = RoutedMeta(window=window, duration=L.oracle_duration(h))
+        for i, (name, dt) in enumerate(zip(NAMES, DTYPES)):
+            p = ctypes.c_void_p()
+            n = ctypes.c_int64()
+            L.oracle_get(h, i, ctypes.byref(p), ctypes.byref(n))
+            if n.value:
+                buf = (ctypes.c_char * (n.value * np.dtype(dt).itemsize)).from_address(p.value)
+                a = np.frombuffer(bytes(buf), dtype=dt).copy()
+            else:
+                a = np.zeros(0, dtype=dt)
+            out.arrays[name] = a.reshape(_shape(name, n.value, window))
+        return out
+    finally:
+        L.oracle_free(h)
+
+
+def decode(meta, window=None):
+    """COO triplets encoded by the routed metadata (dict or RoutedMeta).
+
+    Values are taken as stored (float32 arrays; pass f16-rounded copies to see
+    what a half-precision multiply sees)."""
+    L = lib()
+    m = meta.arrays if isinstance(meta, RoutedMeta) else meta
+    window = window or (meta.window if isinstance(meta, RoutedMeta) else 16)
+    a = {k: np.ascontiguousarray(m[k]) for k in NAMES}
+    f = {k: np.ascontiguousarray(a[k], dtype=np.float32) for k in ("t_value", "c_value", "c_value_short", "s_value")}
+    args = [_ptr(a["t_row_offset"]), a["t_window_row"].size, _ptr(a["t_block_offset"]), _ptr(a["t_column"]),
+            _ptr(f["t_value"]), _ptr(a["t_window_row"]), _ptr(a["t_binary"]),
+            _ptr(a["c_row_offset"]), _ptr(a["c_row"]), a["c_row"].size, _ptr(a["c_column"]), _ptr(f["c_value"]),
+            _ptr(a["c_row_offset_short"]), _ptr(a["c_row_short"]), a["c_row_short"].size,
+            _ptr(a["c_column_short"]), _ptr(f["c_value_short"]),
+            _ptr(a["s_column"]), _ptr(f["s_value"]), _ptr(a["s_pos"]), _ptr(a["s_window"]), a["s_window"].size,
+            window]
+    n = L.oracle_decode(*args, None, None, None, 0)
+    rows = np.empty(n, np.int32)
+    cols = np.empty(n, np.int32)
+    vals = np.empty(n, np.float32)
+    L.oracle_decode(*args, _ptr(rows), _ptr(cols), _ptr(vals), n)
+    return rows, cols, vals
+
+
+def coo_spmm64(rows, cols, vals, B, m):
+    """FP64 C[m, K] = sum over triplets of vals * B[cols]  (scipy CSR product)."""
+    import scipy.sparse as sp
+    A = sp.csr_matrix((np.asarray(vals, np.float64), (rows, cols)), shape=(m, B.shape[0]))
+    return np.asarray(A @ np.asarray(B, np.float64))
+
+
+def spmm64(meta, B, m, window=None, half_values=True):
+    """FP64 product of the routed matrix with B; values rounded to fp16 first
+    when `half_values` (what the half-precision multiply consumes)."""
+    rows, cols, vals = decode(meta, window)
+    if half_values:
+        vals = vals.astype(np.float16).astype(np.float32)
+    keep = rows < m
+    return coo_spmm64(rows[keep], cols[keep], vals[keep], B, m)
+
+
+def csr_spmm64(rowptr, col, val, B, m=None):
+    import scipy.sparse as sp
+    m = m if m is not None else len(rowptr) - 1
+    A = sp.csr_matrix((np.asarray(val, np.float64), np.asarray(col), np.asarray(rowptr)),
+                      shape=(len(rowptr) - 1, B.shape[0]))
+    return np.asarray(A @ np.asarray(B, np.float64))[:m]
+
+
+def sort_order(keys):
+    L = lib()
+    keys = np.ascontiguousarray(keys, dtype=np.int32)
+    out = np.empty(keys.size, np.int32)
+    L.oracle_sort_order(_ptr(keys), keys.size, _ptr(out))
+    return out
+
+
+def f32_to_f16_bits(x):
+    L = lib()
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty(x.size, np.uint16)
+    L.oracle_f32_to_f16(_ptr(x), _ptr(out), x.size)
+    return out

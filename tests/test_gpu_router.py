@@ -1,0 +1,57 @@
+"""Device router vs the oracle: all 29 metadata arrays bit-exact."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from graphs_small import CASES, VARIANTS
+from trspmm.core import META_NAMES, route
+
+pytestmark = pytest.mark.gpu
+
+
+def _to_trspmm_kwargs(kw):
+    m = {"partsize_t": "part_t", "partsize_c": "part_c", "shortsize": "short_c"}
+    return {m.get(k, k): v for k, v in kw.items()}
+
+
+def _compare(csr, kw, mode):
+    print(f"[router] nnz={csr.nnz} rows={csr.m_pad} kw={kw} mode={mode}", flush=True)
+    meta = route(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val), csr.n,
+                 mode=mode, **_to_trspmm_kwargs(kw))
+    got = meta.tensors("cpu")
+    ref = oracle.route(csr.rowptr, csr.col, csr.val, exact=(mode == "exact"), **kw)
+    bad = [n for n, gt in zip(META_NAMES, got)
+           if not (gt.numpy().dtype == ref[n].dtype and gt.numpy().shape == ref[n].shape
+                   and np.array_equal(gt.numpy(), ref[n]))]
+    assert not bad, f"arrays differ: {bad}"
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("mode", ["compat", "exact"])
+def test_router_bit_exact(cuda, case, mode):
+    _compare(CASES[case](), {}, mode)
+
+
+@pytest.mark.parametrize("vi", range(len(VARIANTS)))
+@pytest.mark.parametrize("case", ["rmat10", "dense", "ragged"])
+def test_router_variants(cuda, case, vi):
+    _compare(CASES[case](), VARIANTS[vi], "compat")
+
+
+def test_router_window8(cuda):
+    csr = CASES["er512"]()
+    _compare(csr, {"window": 8}, "compat")
+
+
+def test_router_empty_rows(cuda):
+    from trspmm import graphs as g
+    csr = g.csr_from_pairs(np.array([0, 40]), np.array([3, 5]), np.array([1.0, 2.0]), 64, 64)
+    _compare(csr, {}, "compat")
+
+
+def test_router_rejects_unsorted(cuda):
+    rp = torch.tensor([0, 2], dtype=torch.int32)
+    col = torch.tensor([5, 3], dtype=torch.int32)
+    with pytest.raises(RuntimeError, match="ascending"):
+        route(rp, col, torch.ones(2), 8)

@@ -1,0 +1,232 @@
+// extern "C" entry points of libtrspmm.so (declared in include/trspmm.h).
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "lsort.h"
+#include "plan.h"
+#include "router.h"
+
+namespace trspmm {
+
+static thread_local std::string g_error;
+void set_error(const std::string& msg) { g_error = msg; }
+const char* last_error() { return g_error.c_str(); }
+
+}  // namespace trspmm
+
+using namespace trspmm;
+
+#define TRSPMM_NOTNULL(p, name) TRSPMM_CHECK((p) != nullptr, std::string(name) + " must not be null")
+
+extern "C" {
+
+const char* trspmm_last_error(void) { return trspmm::last_error(); }
+
+int trspmm_version(void) { return 1; }
+
+int trspmm_device_arch(char* buf, int len) {
+    int dev = 0;
+    TRSPMM_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    TRSPMM_HIP(hipGetDeviceProperties(&prop, dev));
+    if (buf && len > 0) {
+        std::strncpy(buf, prop.gcnArchName, (size_t)len - 1);
+        buf[len - 1] = 0;
+    }
+    return TRSPMM_OK;
+}
+
+// ---------------------------------------------------------------- router
+int trspmm_route(const int32_t* d_rowptr, const int32_t* d_col, const float* d_val, int64_t rows, int64_t nnz,
+                 int64_t ncols, const trspmm_route_cfg* cfg, void* stream, trspmm_meta** out) {
+    TRSPMM_NOTNULL(cfg, "cfg");
+    TRSPMM_NOTNULL(out, "out");
+    TRSPMM_NOTNULL(d_rowptr, "rowptr");
+    TRSPMM_CHECK(nnz == 0 || (d_col && d_val), "col and val must not be null when nnz > 0");
+    *out = nullptr;
+    trspmm_meta* m = new (std::nothrow) trspmm_meta();
+    if (!m) {
+        set_error("out of host memory");
+        return TRSPMM_ENOMEM;
+    }
+    const int rc = route_device(d_rowptr, d_col, d_val, rows, nnz, ncols, *cfg, (hipStream_t)stream, m->meta);
+    if (rc != TRSPMM_OK) {
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return TRSPMM_OK;
+}
+
+int64_t trspmm_meta_count(const trspmm_meta* meta, int index) {
+    if (!meta || index < 0 || index >= 29) return -1;
+    return meta->meta.count[index];
+}
+
+void* trspmm_meta_data(const trspmm_meta* meta, int index) {
+    if (!meta || index < 0 || index >= 29) return nullptr;
+    return meta->meta.data[index];
+}
+
+double trspmm_meta_seconds(const trspmm_meta* meta) { return meta ? meta->meta.seconds : 0.0; }
+
+void trspmm_meta_free(trspmm_meta* meta) { delete meta; }
+
+// ---------------------------------------------------------------- plans
+int trspmm_plan_create(const trspmm_meta_view* view, int64_t m_out, int64_t ncols, int dtype, void* stream,
+                       trspmm_plan** out) {
+    TRSPMM_NOTNULL(view, "view");
+    TRSPMM_NOTNULL(out, "out");
+    *out = nullptr;
+    trspmm_plan* p = new (std::nothrow) trspmm_plan();
+    if (!p) {
+        set_error("out of host memory");
+        return TRSPMM_ENOMEM;
+    }
+    const int rc = build_plan(*view, m_out, ncols, dtype, (hipStream_t)stream, p->plan);
+    if (rc != TRSPMM_OK) {
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return TRSPMM_OK;
+}
+
+int trspmm_plan_from_meta(const trspmm_meta* meta, int64_t m_out, int64_t ncols, int dtype, void* stream,
+                          trspmm_plan** out) {
+    TRSPMM_NOTNULL(meta, "meta");
+    const Meta& M = meta->meta;
+    trspmm_meta_view v{};
+    v.t_row_offset = (const int32_t*)M.data[0];
+    v.n_tparts = M.count[4];
+    v.t_block_offset = (const int32_t*)M.data[1];
+    v.t_column = (const int32_t*)M.data[2];
+    v.n_tsub = M.count[2] / 8;
+    v.t_value = M.data[3];
+    v.n_tval = M.count[3];
+    v.t_window_row = (const int32_t*)M.data[4];
+    v.t_binary = (const int64_t*)M.data[6];
+    v.c_row_offset = (const int32_t*)M.data[7];
+    v.n_cparts = M.count[8];
+    v.c_row = (const int32_t*)M.data[8];
+    v.c_column = (const int32_t*)M.data[10];
+    v.n_cval = M.count[10];
+    v.c_value = M.data[11];
+    v.cs_row_offset = (const int32_t*)M.data[12];
+    v.n_csparts = M.count[13];
+    v.cs_row = (const int32_t*)M.data[13];
+    v.cs_column = (const int32_t*)M.data[15];
+    v.n_csval = M.count[15];
+    v.cs_value = M.data[16];
+    v.s_column = (const int32_t*)M.data[17];
+    v.n_groups = M.count[21];
+    v.s_value = (const float*)M.data[19];
+    v.s_pos = (const int8_t*)M.data[20];
+    v.s_offsets = (const int32_t*)M.data[22];
+    v.n_windows = M.count[22] - 1;
+    v.t_value_is_f32 = 1;
+    v.c_value_is_f32 = 1;
+    v.window = M.window;
+    return trspmm_plan_create(&v, m_out, ncols, dtype, stream, out);
+}
+
+int trspmm_plan_create_packed(const int32_t* s_tile_column, const int32_t* s_packed_a, const int32_t* s_packed_meta,
+                              const int32_t* s_window_tile_offset, int64_t n_tiles, int64_t n_windows, int32_t window,
+                              int64_t m_out, int64_t ncols, void* stream, trspmm_plan** out) {
+    TRSPMM_NOTNULL(out, "out");
+    TRSPMM_NOTNULL(s_window_tile_offset, "s_window_tile_offset");
+    *out = nullptr;
+    trspmm_plan* p = new (std::nothrow) trspmm_plan();
+    if (!p) {
+        set_error("out of host memory");
+        return TRSPMM_ENOMEM;
+    }
+    const int rc = build_plan_packed(s_tile_column, s_packed_a, s_packed_meta, s_window_tile_offset, n_tiles,
+                                     n_windows, window, m_out, ncols, (hipStream_t)stream, p->plan);
+    if (rc != TRSPMM_OK) {
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return TRSPMM_OK;
+}
+
+void trspmm_plan_free(trspmm_plan* plan) { delete plan; }
+
+int trspmm_plan_stats(const trspmm_plan* plan, int64_t* c) {
+    TRSPMM_NOTNULL(plan, "plan");
+    TRSPMM_NOTNULL(c, "counts");
+    const PlanDev& d = plan->plan.dev;
+    c[0] = d.windows;
+    c[1] = d.n_tsub;
+    c[2] = d.n_tval;
+    c[3] = d.n_tiles;
+    c[4] = d.n_rparts;
+    c[5] = d.n_rval;
+    return TRSPMM_OK;
+}
+
+// ---------------------------------------------------------------- multiply
+int trspmm_spmm(const trspmm_plan* plan, const void* B, int64_t ldb, int64_t K, float* C, int64_t ldc,
+                uint32_t route_mask, void* stream) {
+    TRSPMM_NOTNULL(plan, "plan");
+    TRSPMM_NOTNULL(B, "B");
+    TRSPMM_NOTNULL(C, "C");
+    return launch_spmm(plan->plan.dev, B, ldb, K, C, ldc, route_mask, (hipStream_t)stream);
+}
+
+int trspmm_spmm_timed(const trspmm_plan* plan, const void* B, int64_t ldb, int64_t K, float* C, int64_t ldc,
+                      uint32_t route_mask, int warmup, int iters, void* stream, float* ms_avg) {
+    TRSPMM_NOTNULL(ms_avg, "ms_avg");
+    TRSPMM_CHECK(iters > 0 && warmup >= 0, "iters must be positive and warmup non-negative");
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    for (int i = 0; i < warmup; ++i)
+        if ((rc = trspmm_spmm(plan, B, ldb, K, C, ldc, route_mask, stream))) return rc;
+    hipEvent_t e0, e1;
+    TRSPMM_HIP(hipEventCreate(&e0));
+    TRSPMM_HIP(hipEventCreate(&e1));
+    TRSPMM_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < iters; ++i) {
+        if ((rc = trspmm_spmm(plan, B, ldb, K, C, ldc, route_mask, stream))) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return rc;
+        }
+    }
+    TRSPMM_HIP(hipEventRecord(e1, s));
+    TRSPMM_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    TRSPMM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_avg = ms / (float)iters;
+    return TRSPMM_OK;
+}
+
+int trspmm_memcpy(void* dst, const void* src, int64_t bytes, void* stream) {
+    TRSPMM_CHECK(bytes >= 0, "bytes must be non-negative");
+    if (bytes == 0) return TRSPMM_OK;
+    TRSPMM_CHECK(dst && src, "dst and src must not be null");
+    TRSPMM_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDefault, (hipStream_t)stream));
+    TRSPMM_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return TRSPMM_OK;
+}
+
+// ---------------------------------------------------------------- diagnostics
+int trspmm_host_sort_order(const int32_t* key, int32_t n, int32_t* order) {
+    TRSPMM_CHECK(n >= 0 && n < (1 << 24), "n must be in [0, 2^24)");
+    TRSPMM_CHECK(n == 0 || (key && order), "key and order must not be null");
+    std::vector<uint32_t> a((size_t)n);
+    for (int32_t i = 0; i < n; ++i) {
+        TRSPMM_CHECK(key[i] >= 0 && key[i] < 256, "keys must be in [0, 256)");
+        a[(size_t)i] = ((uint32_t)key[i] << 24) | (uint32_t)i;
+    }
+    int32_t stack[3 * 66];
+    lsort::sort(a.data(), n, lsort::KeyDesc{}, stack);
+    for (int32_t i = 0; i < n; ++i) order[i] = (int32_t)(a[(size_t)i] & 0xffffffu);
+    return TRSPMM_OK;
+}
+
+}  // extern "C"

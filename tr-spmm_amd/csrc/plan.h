@@ -1,0 +1,29 @@
+// Device plan: owned buffers behind a PlanDev view (layout in common.h).
+#pragma once
+
+#include "common.h"
+
+namespace trspmm {
+
+struct Plan {
+    PlanDev dev;
+    DevBuf tc_sub_off, tc_cols, tc_bits, tc_valoff, tc_vals;
+    DevBuf sp_tile_off, sp_cols, sp_a, sp_idx;
+    DevBuf res_off, res_row, res_beg, res_cnt, res_cols, res_vals;
+    int finalize();
+};
+
+int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtype, hipStream_t s, Plan& P);
+int build_plan_packed(const int32_t* s_tile_column, const int32_t* s_packed_a, const int32_t* s_packed_meta,
+                      const int32_t* s_window_tile_offset, int64_t T, int64_t W, int window, int64_t m_out,
+                      int64_t ncols, hipStream_t s, Plan& P);
+
+// routed multiply (spmm.hip)
+int launch_spmm(const PlanDev& p, const void* B, int64_t ldb, int64_t K, float* C, int64_t ldc, uint32_t mask,
+                hipStream_t s);
+
+}  // namespace trspmm
+
+struct trspmm_plan {
+    trspmm::Plan plan;
+};

@@ -1,0 +1,39 @@
+// Router output: the reference's 29 metadata arrays, owned on the device.
+#pragma once
+
+#include "common.h"
+
+namespace trspmm {
+
+// element size of array `i` in the order of block_online.cpp:844-868
+inline size_t meta_elem_size(int i) {
+    switch (i) {
+        case 6: return 8;                       // t_binary int64
+        case 18: case 20: return 1;             // s_mask uint8, s_pos int8
+        default: return 4;                      // int32 / float32
+    }
+}
+
+struct Meta {
+    DevBuf buf[29];
+    void* data[29] = {};
+    int64_t count[29] = {};
+    int window = 16;
+    int64_t rows = 0;
+    double seconds = 0.0;
+    int alloc(int i, int64_t n) {
+        count[i] = n;
+        int rc = buf[i].alloc(meta_elem_size(i) * (size_t)(n > 0 ? n : 0));
+        data[i] = buf[i].p;
+        return rc;
+    }
+};
+
+int route_device(const int32_t* rowptr, const int32_t* col, const float* val, int64_t rows, int64_t nnz,
+                 int64_t ncols, const trspmm_route_cfg& cfg, hipStream_t s, Meta& M);
+
+}  // namespace trspmm
+
+struct trspmm_meta {
+    trspmm::Meta meta;
+};

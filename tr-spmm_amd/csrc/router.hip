@@ -1,0 +1,883 @@
+// Device router: canonical CSR -> the reference's 29 routed-metadata arrays,
+// bit-exact with Libra5BlockOnline.block_sptc_2to4_online
+// (TR-source/Block/block_online.cpp:545-871) for window <= 16.
+//
+// For window <= 16 every per-row field of a column's 2-bit row vector is 0
+// or 1 (block_online.cpp:592-600), so the 2:4 compatibility test
+// (block_online.cpp:85-88) always passes and the reference's unit building
+// and max-degree matching reduce to (per window):
+//   units  = consecutive pairs of the non-TC columns ordered by
+//            (count desc, column asc)                  (:267-328)
+//   order  = libstdc++ std::sort of iota(units) by unit nnz desc (:356-363)
+//   pairs  = each unmatched unit in that order takes the lowest-index
+//            unmatched unit (:365-386: all degrees equal and unit nnz is
+//            non-increasing in the index)
+// The oracle checks this closed form against the step-by-step restatement
+// and against the reference router itself.
+//
+// Pipeline (all on the caller's stream; one host sync to size the outputs):
+//   1. validate the CSR; build keys (window << colbits | column) per edge
+//   2. stable radix sort by key (rocPRIM): each (window, column) becomes a run
+//      of its edges in row order; run-length encode
+//   3. one wave per window: stable counting sort of its runs by count desc,
+//      TC chunk selection, unit list, libstdc++ order emulation on one lane
+//      (in LDS when it fits), pairing
+//   4. prefix sums over windows -> output sizes; allocate outputs
+//   5. emission kernels: TC sub-blocks/parts, 2:4 groups and per-row slots,
+//      residual flags -> per-row residual parts, mma.sp tile packing.
+#include <chrono>
+#include <cstring>   // rocPRIM's texture iterator uses memset without including it
+#include <vector>
+
+#include <rocprim/rocprim.hpp>
+
+#include "lsort.h"
+#include "router.h"
+#include "scan.h"
+
+namespace trspmm {
+
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ inline uint64_t lanemask_lt() {
+    const int l = threadIdx.x & 63;
+    return l ? (~0ull >> (64 - l)) : 0ull;
+}
+
+__device__ inline uint64_t ballot(bool p) { return __ballot(p); }
+
+// window w with off[w] <= x < off[w + 1] (off non-decreasing, off[0] = 0)
+__device__ inline int64_t upper_bound_window(const int32_t* off, int64_t W, int32_t x) {
+    int64_t lo = 0, hi = W + 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (off[mid] <= x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo - 1;
+}
+
+// ---------------------------------------------------------------- 1. validation + keys
+__global__ void k_validate(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, int64_t rows,
+                           int64_t nnz, int64_t ncols, int* __restrict__ err) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r == 0 && (rowptr[0] != 0 || (int64_t)rowptr[rows] != nnz)) atomicOr(err, 1);
+    if (r >= rows) return;
+    const int32_t b = rowptr[r], e = rowptr[r + 1];
+    if (b > e || b < 0 || (int64_t)e > nnz) {
+        atomicOr(err, 1);
+        return;
+    }
+    int32_t prev = -1;
+    for (int32_t i = b; i < e; ++i) {
+        const int32_t c = col[i];
+        if (c < 0 || (int64_t)c >= ncols) {
+            atomicOr(err, 2);
+            return;
+        }
+        if (c <= prev) {
+            atomicOr(err, 4);
+            return;
+        }
+        prev = c;
+    }
+}
+
+__global__ void k_keys(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, int64_t rows, int64_t nnz,
+                       int window, int colbits, uint64_t* __restrict__ keys, int32_t* __restrict__ vals,
+                       int32_t* __restrict__ erow) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+        // row = last r with rowptr[r] <= e
+        int64_t lo = 0, hi = rows;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (rowptr[mid] <= e) lo = mid;
+            else hi = mid - 1;
+        }
+        const int64_t w = lo / window;
+        keys[e] = ((uint64_t)w << colbits) | (uint64_t)col[e];
+        vals[e] = (int32_t)e;
+        erow[e] = (int32_t)lo;
+    }
+}
+
+// ---------------------------------------------------------------- 2. runs
+__global__ void k_run_flags(const uint64_t* __restrict__ keys, int64_t n, int32_t* __restrict__ flag) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
+}
+
+__global__ void k_runs(const uint64_t* __restrict__ keys, const int32_t* __restrict__ flag,
+                       const int32_t* __restrict__ run_of, int64_t n, int colbits, int32_t* __restrict__ run_start,
+                       int32_t* __restrict__ run_col, int32_t* __restrict__ run_win) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (!flag[i]) continue;
+        const int32_t r = run_of[i];
+        run_start[r] = (int32_t)i;
+        run_col[r] = (int32_t)(keys[i] & ((1ull << colbits) - 1ull));
+        run_win[r] = (int32_t)(keys[i] >> colbits);
+    }
+}
+
+__global__ void k_win_runs(const int32_t* __restrict__ run_win, int64_t nruns, int64_t W, int32_t* __restrict__ win_rb) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w <= W) win_rb[w] = (int32_t)lower_bound(run_win, nruns, (int32_t)w);
+}
+
+// ---------------------------------------------------------------- 3. per-window decisions
+struct WinOut {
+    int32_t* n_tcsub;     // TC sub-blocks (kept chunks)
+    int32_t* n_tcval;     // TC nonzeros
+    int32_t* n_pairs;     // matched unit pairs
+    int32_t* n_groups;    // 2:4 groups (pairs + kept leftover)
+    int32_t* leftover;    // unmatched unit index or -1
+    int32_t* fallback;    // some column is in neither TC nor 2:4 (block_online.cpp:706-713)
+    int32_t* n_rest;      // non-TC columns
+};
+
+struct RunArrays {
+    const int32_t* start;   // [nruns + 1] first sorted-edge position
+    const int32_t* col;
+    const int32_t* win_rb;  // [W + 1]
+    int32_t* order;         // [nruns] runs of a window by (count desc, col asc)
+    int32_t* rest;          // [nruns] non-TC runs in that order
+    int32_t* tc_chunk;      // [nruns] kept TC chunk indices
+    uint8_t* in_tc;         // [nruns]
+    uint32_t* units_g;      // [nruns] unit keys for windows whose units do not fit in LDS
+    uint8_t* used_g;        // [nruns]
+    int32_t* pair_a;        // [nruns]
+    int32_t* pair_b;        // [nruns]
+};
+
+__device__ inline int run_count(const RunArrays& R, int32_t run) { return R.start[run + 1] - R.start[run]; }
+
+// One wave per window.  A launch serves one size class: windows whose unit
+// bound ceil(columns/2) lies in [cls_lo, cls_hi]; the others return at once.
+// `lds_cap` = unit capacity of the dynamic LDS (0: units live in global memory).
+__global__ __launch_bounds__(64) void k_window_decide(RunArrays R, WinOut O, int64_t W, int window, int density,
+                                                      float tc_min_util, int skip_tc, int unit_min_nnz,
+                                                      int cls_lo, int cls_hi, int lds_cap) {
+    extern __shared__ uint32_t dyn[];
+    __shared__ int32_t hist[17], base[17], fill[17];
+    __shared__ int32_t s_ncand;
+    __shared__ int32_t sort_stack[3 * 66];
+    const int64_t w = blockIdx.x;
+    if (w >= W) return;
+    const int lane = threadIdx.x;
+    const int32_t rb = R.win_rb[w], re = R.win_rb[w + 1];
+    const int32_t nc = re - rb;
+
+    // size class: units <= ceil(nc / 2) (TC columns only reduce it)
+    const int32_t umax = (nc + 1) / 2;
+    if (umax < cls_lo || umax > cls_hi) return;
+
+    // a. histogram of run counts (1..16)
+    if (lane < 17) {
+        hist[lane] = 0;
+        fill[lane] = 0;
+    }
+    __syncthreads();
+    for (int32_t i = rb + lane; i < re; i += kWave) atomicAdd(&hist[run_count(R, i)], 1);
+    __syncthreads();
+    if (lane == 0) {
+        int32_t acc = 0;
+        for (int c = 16; c >= 1; --c) {
+            base[c] = acc;
+            acc += hist[c];
+        }
+        int32_t ncand = 0;
+        if (!skip_tc)
+            for (int c = 16; c >= 1 && c >= density; --c) ncand += hist[c];
+        s_ncand = ncand;
+    }
+    __syncthreads();
+
+    // b. stable counting sort by count desc (runs arrive in column order)
+    for (int32_t i0 = rb; i0 < re; i0 += kWave) {
+        const int32_t i = i0 + lane;
+        const bool on = i < re;
+        const int c = on ? run_count(R, i) : 0;
+        const uint64_t lt = lanemask_lt();
+        int32_t pos = 0;
+        for (int v = 1; v <= 16; ++v) {
+            const uint64_t m = ballot(on && c == v);
+            if (on && c == v) pos = base[v] + fill[v] + __popcll(m & lt);
+            __syncthreads();
+            if (lane == 0) fill[v] += __popcll(m);
+            __syncthreads();
+        }
+        if (on) R.order[rb + pos] = i;
+    }
+    __syncthreads();
+
+    // c. TC chunks of 8 over the candidates (block_online.cpp:215-253)
+    const int32_t ncand = s_ncand;
+    const int32_t nchunk = (ncand + 7) / 8;
+    int32_t kept_total = 0, tcval = 0;
+    for (int32_t j0 = 0; j0 < nchunk; j0 += kWave) {
+        const int32_t j = j0 + lane;
+        bool keep = false;
+        int32_t nnz = 0;
+        if (j < nchunk) {
+            const int32_t e = (8 * j + 8 < ncand) ? 8 * j + 8 : ncand;
+            for (int32_t k = 8 * j; k < e; ++k) nnz += run_count(R, R.order[rb + k]);
+            const float util = (float)nnz / (float)(window * 8);
+            keep = !(util < tc_min_util);
+            for (int32_t k = 8 * j; k < e; ++k) R.in_tc[R.order[rb + k]] = keep ? 1 : 0;
+        }
+        const uint64_t m = ballot(keep);
+        if (keep) R.tc_chunk[rb + kept_total + __popcll(m & lanemask_lt())] = j;
+        kept_total += __popcll(m);
+        int32_t v = keep ? nnz : 0;
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        tcval += v;
+    }
+    // candidates never reach the TC set when skipped; everything past ncand is not a candidate
+    for (int32_t k = ncand + lane; k < nc; k += kWave) R.in_tc[R.order[rb + k]] = 0;
+    __syncthreads();
+
+    // d. non-TC runs in (count desc, col asc) order
+    int32_t nrest = 0;
+    for (int32_t k0 = 0; k0 < nc; k0 += kWave) {
+        const int32_t k = k0 + lane;
+        const int32_t run = k < nc ? R.order[rb + k] : 0;
+        const bool on = k < nc && !R.in_tc[run];
+        const uint64_t m = ballot(on);
+        if (on) R.rest[rb + nrest + __popcll(m & lanemask_lt())] = run;
+        nrest += __popcll(m);
+    }
+    __syncthreads();
+
+    // e. units and their order key
+    const int32_t nu = (nrest + 1) / 2;
+    const bool in_lds = lds_cap > 0 && nu <= lds_cap;
+    uint32_t* units = in_lds ? dyn : R.units_g + rb;
+    for (int32_t k = lane; k < nu; k += kWave) {
+        int32_t nnz = run_count(R, R.rest[rb + 2 * k]);
+        if (2 * k + 1 < nrest) nnz += run_count(R, R.rest[rb + 2 * k + 1]);
+        units[k] = ((uint32_t)nnz << 24) | (uint32_t)k;
+    }
+    uint8_t* used = in_lds ? (uint8_t*)(dyn + lds_cap) : R.used_g + rb;
+    for (int32_t k = lane; k < nu; k += kWave) used[k] = 0;
+    __syncthreads();
+
+    // f. libstdc++ order + pairing, one lane
+    if (lane == 0) {
+        lsort::sort(units, nu, lsort::KeyDesc{}, sort_stack);
+        int32_t low = 0, np = 0;
+        for (int32_t i = 0; i < nu; ++i) {
+            const int32_t v = (int32_t)(units[i] & 0xffffffu);
+            if (used[v]) continue;
+            while (low < nu && used[low]) ++low;
+            int32_t p = low;
+            if (p == v) {
+                p = v + 1;
+                while (p < nu && used[p]) ++p;
+            }
+            if (p >= nu) continue;
+            used[v] = used[p] = 1;
+            R.pair_a[rb + np] = v;
+            R.pair_b[rb + np] = p;
+            ++np;
+        }
+        int32_t left = -1;
+        if (2 * np < nu)
+            for (int32_t k = 0; k < nu; ++k)
+                if (!used[k]) {
+                    left = k;
+                    break;
+                }
+        int32_t left_nnz = 0;
+        if (left >= 0) {
+            left_nnz = run_count(R, R.rest[rb + 2 * left]);
+            if (2 * left + 1 < nrest) left_nnz += run_count(R, R.rest[rb + 2 * left + 1]);
+        }
+        const bool keep_left = left >= 0 && left_nnz >= unit_min_nnz;
+        O.n_tcsub[w] = kept_total;
+        O.n_tcval[w] = tcval;
+        O.n_pairs[w] = np;
+        O.n_groups[w] = np + (keep_left ? 1 : 0);
+        O.leftover[w] = left;
+        O.fallback[w] = (left >= 0 && !keep_left) ? 1 : 0;
+        O.n_rest[w] = nrest;
+    }
+}
+
+__global__ void k_part_counts(const int32_t* __restrict__ n_tcsub, const int32_t* __restrict__ n_groups, int64_t W,
+                              int part_t, int32_t* __restrict__ n_tparts, int32_t* __restrict__ n_tiles) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W) return;
+    const int32_t s = n_tcsub[w];
+    n_tparts[w] = s == 0 ? 0 : (s <= part_t ? 1 : s / part_t + (s % part_t ? 1 : 0));
+    n_tiles[w] = (n_groups[w] + 3) / 4;
+}
+
+// ---------------------------------------------------------------- 5. emission
+struct Emit {
+    // sorted edges
+    const int32_t* sval;     // edge id at each sorted position
+    const int32_t* erow;     // row of each edge
+    const float* val;
+    // window prefix sums
+    const int32_t* tc_sub_off;
+    const int32_t* tc_val_off;
+    const int32_t* tc_part_off;
+    const int32_t* grp_off;
+    const int32_t* tile_off;
+    // outputs (block_online.cpp:844-868 order)
+    int32_t *t_row_offset, *t_block_offset, *t_column;
+    float* t_value;
+    int32_t *t_window_row, *t_atomic;
+    int64_t* t_binary;
+    int32_t* s_column;
+    uint8_t* s_mask;
+    float* s_value;
+    int8_t* s_pos;
+    int32_t *s_window, *s_offsets;
+    int32_t *s_tile_window, *s_tile_group, *s_tile_column, *s_packed_a, *s_packed_meta, *s_window_tile_offset;
+    uint8_t* res_flag;       // per edge: goes to the residual route
+};
+
+// TC: one thread per (window, kept chunk) = sub-block
+__global__ void k_emit_tc(RunArrays R, WinOut O, Emit E, int64_t W, int window, int part_t, int64_t S) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const int64_t w = upper_bound_window(E.tc_sub_off, W, (int32_t)s);
+    const int32_t rb = R.win_rb[w];
+    const int32_t k = (int32_t)(s - E.tc_sub_off[w]);
+    const int32_t j = R.tc_chunk[rb + k];
+    // the chunk's runs, sorted by column (<= 8)
+    int32_t runs[8], cols[8];
+    int n = 0;
+    // chunk j = order[8j, 8j+8) cut at the first non-candidate (in_tc == 0)
+    for (int q = 0; q < 8; ++q) {
+        const int32_t pos = 8 * j + q;
+        if (pos >= R.win_rb[w + 1] - rb) break;
+        const int32_t run = R.order[rb + pos];
+        if (!R.in_tc[run]) break;
+        runs[n] = run;
+        cols[n] = R.col[run];
+        ++n;
+    }
+    for (int a = 1; a < n; ++a) {   // insertion sort by column
+        const int32_t rr = runs[a], cc = cols[a];
+        int b = a - 1;
+        while (b >= 0 && cols[b] > cc) {
+            runs[b + 1] = runs[b];
+            cols[b + 1] = cols[b];
+            --b;
+        }
+        runs[b + 1] = rr;
+        cols[b + 1] = cc;
+    }
+    for (int q = 0; q < 8; ++q) E.t_column[8 * s + q] = q < n ? cols[q] : -1;
+    // value offset of this sub-block: window offset + values of earlier kept chunks
+    int32_t voff = E.tc_val_off[w];
+    for (int32_t kk = 0; kk < k; ++kk) {
+        const int32_t jj = R.tc_chunk[rb + kk];
+        for (int q = 0; q < 8; ++q) {
+            const int32_t pos = 8 * jj + q;
+            if (pos >= R.win_rb[w + 1] - rb) break;
+            const int32_t run = R.order[rb + pos];
+            if (!R.in_tc[run]) break;
+            voff += run_count(R, run);
+        }
+    }
+    E.t_block_offset[s] = voff;
+    // bits and values in (row, slot) order
+    int32_t ptr[8];
+    for (int q = 0; q < n; ++q) ptr[q] = R.start[runs[q]];
+    const int64_t row0 = w * window;
+    uint64_t lo = 0, hi = 0;
+    for (int lr = 0; lr < window; ++lr) {
+        for (int q = 0; q < n; ++q) {
+            if (ptr[q] < R.start[runs[q] + 1]) {
+                const int32_t e = E.sval[ptr[q]];
+                if (E.erow[e] - row0 == lr) {
+                    if (lr >= 8) hi |= 1ull << ((lr - 8) * 8 + q);
+                    else lo |= 1ull << (lr * 8 + q);
+                    E.t_value[voff++] = E.val[e];
+                    ++ptr[q];
+                }
+            }
+        }
+    }
+    E.t_binary[2 * s] = (int64_t)lo;
+    E.t_binary[2 * s + 1] = (int64_t)hi;
+    // parts: the thread of the first sub-block of each part writes it
+    const int32_t nsub = O.n_tcsub[w];
+    if (k % part_t == 0) {
+        const int32_t p = E.tc_part_off[w] + k / part_t;
+        E.t_row_offset[p] = (int32_t)s;
+        E.t_window_row[p] = (int32_t)(w * 16);
+        E.t_atomic[p] = nsub <= part_t ? O.fallback[w] : 1;
+    }
+}
+
+// 2:4 groups: one thread per group
+__global__ void k_emit_groups(RunArrays R, WinOut O, Emit E, int64_t W, int window, int exact, int64_t G) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    const int64_t w = upper_bound_window(E.grp_off, W, (int32_t)g);
+    const int32_t rb = R.win_rb[w];
+    const int32_t k = (int32_t)(g - E.grp_off[w]);
+    int32_t ua, ub = -1;
+    if (k < O.n_pairs[w]) {
+        ua = R.pair_a[rb + k];
+        ub = R.pair_b[rb + k];
+    } else {
+        ua = O.leftover[w];
+    }
+    // columns of the unit(s): rest[2u], rest[2u+1] if that unit is a pair
+    const int32_t nr = O.n_rest[w];
+    int32_t runs[4], cols[4];
+    int n = 0;
+    const int32_t us[2] = {ua, ub};
+    for (int t = 0; t < 2; ++t) {
+        const int32_t u = us[t];
+        if (u < 0) continue;
+        for (int h = 0; h < 2; ++h) {
+            const int32_t pos = 2 * u + h;
+            if (pos >= nr) break;
+            runs[n] = R.rest[rb + pos];
+            cols[n] = R.col[runs[n]];
+            ++n;
+        }
+    }
+    for (int a = 1; a < n; ++a) {
+        const int32_t rr = runs[a], cc = cols[a];
+        int b = a - 1;
+        while (b >= 0 && cols[b] > cc) {
+            runs[b + 1] = runs[b];
+            cols[b + 1] = cols[b];
+            --b;
+        }
+        runs[b + 1] = rr;
+        cols[b + 1] = cc;
+    }
+    for (int q = 0; q < 4; ++q) E.s_column[4 * g + q] = q < n ? cols[q] : -1;
+    E.s_window[g] = (int32_t)w;
+    int32_t ptr[4];
+    for (int q = 0; q < n; ++q) ptr[q] = R.start[runs[q]];
+    const int64_t row0 = w * window;
+    for (int lr = 0; lr < window; ++lr) {
+        uint8_t mask = 0;
+        int used = 0;
+        float v[2] = {0.f, 0.f};
+        int8_t ps[2] = {-1, -1};
+        for (int q = 0; q < n; ++q) {
+            if (ptr[q] >= R.start[runs[q] + 1]) continue;
+            const int32_t e = E.sval[ptr[q]];
+            if (E.erow[e] - row0 != lr) continue;
+            ++ptr[q];
+            mask |= (uint8_t)(1u << q);
+            if (used < 2) {
+                v[used] = E.val[e];
+                ps[used] = (int8_t)q;
+            } else if (exact) {
+                E.res_flag[e] = 1;
+            }
+            ++used;
+        }
+        const int64_t b = g * window + lr;
+        E.s_mask[b] = mask;
+        E.s_value[2 * b] = v[0];
+        E.s_value[2 * b + 1] = v[1];
+        E.s_pos[2 * b] = ps[0];
+        E.s_pos[2 * b + 1] = ps[1];
+    }
+}
+
+// columns in neither set: the dropped leftover unit
+__global__ void k_flag_dropped(RunArrays R, WinOut O, Emit E, int64_t W) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W || !O.fallback[w]) return;
+    const int32_t rb = R.win_rb[w];
+    const int32_t u = O.leftover[w];
+    const int32_t nr = O.n_rest[w];
+    for (int h = 0; h < 2; ++h) {
+        const int32_t pos = 2 * u + h;
+        if (pos >= nr) break;
+        const int32_t run = R.rest[rb + pos];
+        for (int32_t i = R.start[run]; i < R.start[run + 1]; ++i) E.res_flag[E.sval[i]] = 1;
+    }
+}
+
+// residual: per row counts of parts (long/short) and entries (block_online.cpp:415-439)
+struct ResCounts {
+    int32_t *nl, *ns, *el, *es;   // per row: long parts, short parts, long entries, short entries
+};
+
+__global__ void k_res_count(const int32_t* __restrict__ rowptr, const uint8_t* __restrict__ flag, int64_t rows,
+                            int part_c, int short_c, ResCounts RC) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    int32_t n = 0;
+    for (int32_t e = rowptr[r]; e < rowptr[r + 1]; ++e) n += flag[e];
+    int32_t nl = 0, ns = 0, el = 0, es = 0;
+    if (n > 0) {
+        if (n <= short_c) { ns = 1; es = n; }
+        else if (n <= part_c) { nl = 1; el = n; }
+        else {
+            nl = n / part_c;
+            el = nl * part_c;
+            const int32_t rem = n % part_c;
+            if (rem) {
+                if (rem <= short_c) { ns = 1; es = rem; }
+                else { nl += 1; el += rem; }
+            }
+        }
+    }
+    RC.nl[r] = nl;
+    RC.ns[r] = ns;
+    RC.el[r] = el;
+    RC.es[r] = es;
+}
+
+struct ResOut {
+    const int32_t *pl, *ps, *ql, *qs;     // prefix: long parts, short parts, long entries, short entries
+    int32_t *c_row_offset, *c_row, *c_atomic, *c_column;
+    float* c_value;
+    int32_t *cs_row_offset, *cs_row, *cs_atomic, *cs_column;
+    float* cs_value;
+};
+
+__global__ void k_res_emit(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                           const float* __restrict__ val, const uint8_t* __restrict__ flag,
+                           const int32_t* __restrict__ n_tcsub, int64_t rows, int window, int part_c, int short_c,
+                           ResOut O) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const int32_t nl = O.pl[r + 1] - O.pl[r], ns = O.ps[r + 1] - O.ps[r];
+    if (nl == 0 && ns == 0) return;
+    const int32_t el = O.ql[r + 1] - O.ql[r];
+    const int32_t base_atomic = n_tcsub[r / window] > 0 ? 1 : 0;
+    const int32_t n = el + (O.qs[r + 1] - O.qs[r]);
+    const bool split = n > part_c;
+    // long parts
+    for (int32_t j = 0; j < nl; ++j) {
+        const int32_t p = O.pl[r] + j;
+        O.c_row_offset[p] = O.ql[r] + j * part_c;
+        O.c_row[p] = (int32_t)r;
+        O.c_atomic[p] = split ? 1 : base_atomic;
+    }
+    if (ns) {
+        const int32_t p = O.ps[r];
+        O.cs_row_offset[p] = O.qs[r];
+        O.cs_row[p] = (int32_t)r;
+        O.cs_atomic[p] = split ? 1 : base_atomic;
+    }
+    int32_t k = 0;
+    for (int32_t e = rowptr[r]; e < rowptr[r + 1]; ++e) {
+        if (!flag[e]) continue;
+        if (k < el) {
+            O.c_column[O.ql[r] + k] = col[e];
+            O.c_value[O.ql[r] + k] = val[e];
+        } else {
+            O.cs_column[O.qs[r] + k - el] = col[e];
+            O.cs_value[O.qs[r] + k - el] = val[e];
+        }
+        ++k;
+    }
+}
+
+__global__ void k_res_tail(ResOut O, int64_t rows, int64_t PL, int64_t PS) {
+    O.c_row_offset[PL] = O.ql[rows];
+    O.cs_row_offset[PS] = O.qs[rows];
+}
+
+__global__ void k_tail(Emit E, int64_t S, int64_t PT, int64_t TV) {
+    E.t_row_offset[PT] = (int32_t)S;
+    E.t_block_offset[S] = (int32_t)TV;
+}
+
+// mma.sp tile packing (block_online.cpp:796-841): one thread per (tile, row group)
+__global__ void k_pack_tiles(Emit E, int64_t W, int window, int64_t T) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T * 8) return;
+    const int64_t t = i >> 3;
+    const int rg = (int)(i & 7);
+    const int64_t w = upper_bound_window(E.tile_off, W, (int32_t)t);
+    const int32_t ge = E.grp_off[w + 1];
+    const int32_t gb = E.grp_off[w] + 4 * (int32_t)(t - E.tile_off[w]);
+    if (rg == 0) {
+        E.s_tile_window[t] = (int32_t)w;
+        E.s_tile_group[t] = gb;
+        for (int k = 0; k < 16; ++k) {
+            const int32_t g = gb + (k >> 2);
+            E.s_tile_column[t * 16 + k] = g < ge ? E.s_column[4 * g + (k & 3)] : -1;
+        }
+    }
+    uint32_t meta = 0;
+    for (int ch = 0; ch < 4; ++ch) {
+        for (int hf = 0; hf < 2; ++hf) {
+            const int row = rg + 8 * hf;
+            const int32_t g = gb + ch;
+            float v0 = 0.f, v1 = 0.f;
+            int p0 = 0, p1 = 1;
+            if (g < ge && row < window) {
+                const int64_t b = ((int64_t)g * window + row) * 2;
+                const int q0 = E.s_pos[b], q1 = E.s_pos[b + 1];
+                const float x0 = E.s_value[b], x1 = E.s_value[b + 1];
+                if (q0 >= 0 && q1 >= 0) {
+                    if (q0 <= q1) { p0 = q0; p1 = q1; v0 = x0; v1 = x1; }
+                    else          { p0 = q1; p1 = q0; v0 = x1; v1 = x0; }
+                } else if (q0 >= 0 || q1 >= 0) {
+                    const int q = q0 >= 0 ? q0 : q1;
+                    const float x = q0 >= 0 ? x0 : x1;
+                    const int dummy = q == 0 ? 1 : 0;
+                    if (dummy < q) { p0 = dummy; p1 = q; v1 = x; }
+                    else           { p0 = q; p1 = dummy; v0 = x; }
+                }
+            }
+            meta |= ((uint32_t)(p0 & 3) | ((uint32_t)(p1 & 3) << 2)) << (16 * hf + 4 * ch);
+            const uint32_t packed = (uint32_t)f32_to_f16_rne(v0) | ((uint32_t)f32_to_f16_rne(v1) << 16);
+            E.s_packed_a[((t * 8 + rg) * 4 + ch) * 2 + hf] = (int32_t)packed;
+        }
+    }
+    E.s_packed_meta[t * 8 + rg] = (int32_t)meta;
+}
+
+inline unsigned nblk(int64_t n, int t = 256) {
+    int64_t b = (n + t - 1) / t;
+    return (unsigned)(b < 1 ? 1 : (b > (int64_t)1 << 30 ? (int64_t)1 << 30 : b));
+}
+inline unsigned nblk_gs(int64_t n, int t = 256) {   // grid-stride launches
+    int64_t b = (n + t - 1) / t;
+    if (b < 1) b = 1;
+    if (b > 65536) b = 65536;
+    return (unsigned)b;
+}
+
+int bit_width(int64_t x) {
+    int b = 0;
+    while (x > 0) {
+        ++b;
+        x >>= 1;
+    }
+    return b < 1 ? 1 : b;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host driver
+int route_device(const int32_t* rowptr, const int32_t* col, const float* val, int64_t rows, int64_t nnz,
+                 int64_t ncols, const trspmm_route_cfg& cfg, hipStream_t s, Meta& M) {
+    const auto t_begin = std::chrono::steady_clock::now();
+    TRSPMM_CHECK(cfg.window > 0 && cfg.window <= 16, "window must be in [1, 16]");
+    TRSPMM_CHECK(cfg.wide > 0 && cfg.wide <= 32, "wide must be in [1, 32]");
+    TRSPMM_CHECK(cfg.part_t > 0 && cfg.part_c > 0, "partSize_t and partSize_c must be positive");
+    TRSPMM_CHECK(cfg.mode == TRSPMM_MODE_COMPAT || cfg.mode == TRSPMM_MODE_EXACT,
+                 "router mode must be COMPAT or EXACT");
+    TRSPMM_CHECK(rows >= 0 && nnz >= 0 && nnz < ((int64_t)1 << 31), "bad rows / nnz");
+    TRSPMM_CHECK(ncols > 0 && ncols < ((int64_t)1 << 31), "ncols must be in [1, 2^31)");
+    const int window = cfg.window;
+    const int64_t W = (rows + window - 1) / window;
+    const int colbits = bit_width(ncols - 1);
+    const int winbits = bit_width(W > 0 ? W - 1 : 0);
+    TRSPMM_CHECK(colbits + winbits <= 64, "index range too large");
+    M.window = window;
+    M.rows = rows;
+    const bool exact = cfg.mode == TRSPMM_MODE_EXACT;
+    int rc;
+
+    // 1. validate + keys
+    DevBuf err;
+    if ((rc = err.alloc(4))) return rc;
+    TRSPMM_HIP(hipMemsetAsync(err.p, 0, 4, s));
+    if (rows > 0)
+        hipLaunchKernelGGL(k_validate, dim3(nblk(rows)), dim3(256), 0, s, rowptr, col, rows, nnz, ncols,
+                           err.as<int>());
+    int herr = 0;
+    TRSPMM_HIP(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipStreamSynchronize(s));
+    TRSPMM_CHECK(!(herr & 1), "row pointers must start at 0, be non-decreasing and end at nnz");
+    TRSPMM_CHECK(!(herr & 2), "column index out of [0, ncols)");
+    TRSPMM_CHECK(!(herr & 4), "columns must be strictly ascending within each row (canonical CSR)");
+
+    DevBuf keys, keys2, vals, svals, erow, flag, run_of, run_start, run_col, run_win, win_rb;
+    const int64_t n1 = nnz > 0 ? nnz : 1;
+    if ((rc = keys.alloc(8 * n1)) || (rc = keys2.alloc(8 * n1)) || (rc = vals.alloc(4 * n1)) ||
+        (rc = svals.alloc(4 * n1)) || (rc = erow.alloc(4 * n1)) || (rc = flag.alloc(4 * n1)) ||
+        (rc = run_of.alloc(4 * (n1 + 1))) || (rc = win_rb.alloc(4 * (W + 1))))
+        return rc;
+    if (nnz > 0) {
+        hipLaunchKernelGGL(k_keys, dim3(nblk_gs(nnz)), dim3(256), 0, s, rowptr, col, rows, nnz, window, colbits,
+                           keys.as<uint64_t>(), vals.as<int32_t>(), erow.as<int32_t>());
+        TRSPMM_HIP(hipGetLastError());
+        // 2. stable radix sort by (window, column)
+        size_t tmp_bytes = 0;
+        TRSPMM_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys.as<uint64_t>(), keys2.as<uint64_t>(),
+                                             vals.as<int32_t>(), svals.as<int32_t>(), (size_t)nnz, 0,
+                                             colbits + winbits, s));
+        DevBuf tmp;
+        if ((rc = tmp.alloc(tmp_bytes ? tmp_bytes : 1))) return rc;
+        TRSPMM_HIP(rocprim::radix_sort_pairs(tmp.p, tmp_bytes, keys.as<uint64_t>(), keys2.as<uint64_t>(),
+                                             vals.as<int32_t>(), svals.as<int32_t>(), (size_t)nnz, 0,
+                                             colbits + winbits, s));
+        hipLaunchKernelGGL(k_run_flags, dim3(nblk_gs(nnz)), dim3(256), 0, s, keys2.as<uint64_t>(), nnz,
+                           flag.as<int32_t>());
+        TRSPMM_HIP(hipGetLastError());
+    }
+    if ((rc = exclusive_scan<int32_t>(nnz > 0 ? flag.as<int32_t>() : nullptr, run_of.as<int32_t>(), nnz, s)))
+        return rc;
+    int32_t nruns = 0;
+    TRSPMM_HIP(hipMemcpyAsync(&nruns, run_of.as<int32_t>() + nnz, 4, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipStreamSynchronize(s));
+    const int64_t nr1 = nruns > 0 ? nruns : 1;
+    if ((rc = run_start.alloc(4 * (nr1 + 1))) || (rc = run_col.alloc(4 * nr1)) || (rc = run_win.alloc(4 * nr1)))
+        return rc;
+    if (nnz > 0) {
+        hipLaunchKernelGGL(k_runs, dim3(nblk_gs(nnz)), dim3(256), 0, s, keys2.as<uint64_t>(), flag.as<int32_t>(),
+                           run_of.as<int32_t>(), nnz, colbits, run_start.as<int32_t>(), run_col.as<int32_t>(),
+                           run_win.as<int32_t>());
+        TRSPMM_HIP(hipGetLastError());
+    }
+    {
+        const int32_t tail = (int32_t)nnz;
+        TRSPMM_HIP(hipMemcpyAsync(run_start.as<int32_t>() + nruns, &tail, 4, hipMemcpyHostToDevice, s));
+    }
+    hipLaunchKernelGGL(k_win_runs, dim3(nblk(W + 1)), dim3(256), 0, s, run_win.as<int32_t>(), (int64_t)nruns, W,
+                       win_rb.as<int32_t>());
+    TRSPMM_HIP(hipGetLastError());
+    keys.release();
+
+    // 3. per-window decisions
+    DevBuf order, rest, tc_chunk, in_tc, units_g, used_g, pair_a, pair_b;
+    DevBuf n_tcsub, n_tcval, n_pairs, n_groups, leftover, fallback, n_rest;
+    if ((rc = order.alloc(4 * nr1)) || (rc = rest.alloc(4 * nr1)) || (rc = tc_chunk.alloc(4 * nr1)) ||
+        (rc = in_tc.alloc(nr1)) || (rc = units_g.alloc(4 * nr1)) || (rc = used_g.alloc(nr1)) ||
+        (rc = pair_a.alloc(4 * nr1)) || (rc = pair_b.alloc(4 * nr1)) || (rc = n_tcsub.alloc(4 * (W + 1))) ||
+        (rc = n_tcval.alloc(4 * (W + 1))) || (rc = n_pairs.alloc(4 * (W + 1))) || (rc = n_groups.alloc(4 * (W + 1))) ||
+        (rc = leftover.alloc(4 * (W + 1))) || (rc = fallback.alloc(4 * (W + 1))) || (rc = n_rest.alloc(4 * (W + 1))))
+        return rc;
+    RunArrays R{run_start.as<int32_t>(), run_col.as<int32_t>(), win_rb.as<int32_t>(), order.as<int32_t>(),
+                rest.as<int32_t>(), tc_chunk.as<int32_t>(), in_tc.as<uint8_t>(), units_g.as<uint32_t>(),
+                used_g.as<uint8_t>(), pair_a.as<int32_t>(), pair_b.as<int32_t>()};
+    WinOut O{n_tcsub.as<int32_t>(), n_tcval.as<int32_t>(), n_pairs.as<int32_t>(), n_groups.as<int32_t>(),
+             leftover.as<int32_t>(), fallback.as<int32_t>(), n_rest.as<int32_t>()};
+    if (W > 0) {
+        // size classes by units-per-window: small windows share a CU, large ones get a big LDS
+        // slice, the rare giant window sorts in global memory.
+        struct Cls { int lo, hi, cap; };
+        const Cls classes[3] = {{0, 1024, 1024}, {1025, 12288, 12288}, {12289, 0x7fffffff, 0}};
+        for (const Cls& c : classes) {
+            const size_t lds = c.cap > 0 ? (size_t)c.cap * 5 + 16 : 0;
+            hipLaunchKernelGGL(k_window_decide, dim3((unsigned)W), dim3(64), lds, s, R, O, W, window, cfg.density,
+                               cfg.tc_min_util, cfg.skip_tc, cfg.unit_min_nnz, c.lo, c.hi, c.cap);
+            TRSPMM_HIP(hipGetLastError());
+        }
+    }
+
+    // 4. window prefix sums -> output sizes
+    DevBuf n_tparts, n_tiles, tc_sub_off, tc_val_off, tc_part_off;
+    if ((rc = n_tparts.alloc(4 * (W + 1))) || (rc = n_tiles.alloc(4 * (W + 1))) ||
+        (rc = tc_sub_off.alloc(4 * (W + 1))) || (rc = tc_val_off.alloc(4 * (W + 1))) ||
+        (rc = tc_part_off.alloc(4 * (W + 1))))
+        return rc;
+    if ((rc = M.alloc(22, W + 1)) || (rc = M.alloc(28, W + 1))) return rc;   // s_offsets, s_window_tile_offset
+    if (W > 0) {
+        hipLaunchKernelGGL(k_part_counts, dim3(nblk(W)), dim3(256), 0, s, n_tcsub.as<int32_t>(),
+                           n_groups.as<int32_t>(), W, cfg.part_t, n_tparts.as<int32_t>(), n_tiles.as<int32_t>());
+        TRSPMM_HIP(hipGetLastError());
+    }
+    const int32_t* nz = nullptr;
+    if ((rc = exclusive_scan<int32_t>(W ? n_tcsub.as<int32_t>() : nz, tc_sub_off.as<int32_t>(), W, s)) ||
+        (rc = exclusive_scan<int32_t>(W ? n_tcval.as<int32_t>() : nz, tc_val_off.as<int32_t>(), W, s)) ||
+        (rc = exclusive_scan<int32_t>(W ? n_tparts.as<int32_t>() : nz, tc_part_off.as<int32_t>(), W, s)) ||
+        (rc = exclusive_scan<int32_t>(W ? n_groups.as<int32_t>() : nz, (int32_t*)M.data[22], W, s)) ||
+        (rc = exclusive_scan<int32_t>(W ? n_tiles.as<int32_t>() : nz, (int32_t*)M.data[28], W, s)))
+        return rc;
+    int32_t tot[5];
+    TRSPMM_HIP(hipMemcpyAsync(&tot[0], tc_sub_off.as<int32_t>() + W, 4, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipMemcpyAsync(&tot[1], tc_val_off.as<int32_t>() + W, 4, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipMemcpyAsync(&tot[2], tc_part_off.as<int32_t>() + W, 4, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipMemcpyAsync(&tot[3], (int32_t*)M.data[22] + W, 4, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipMemcpyAsync(&tot[4], (int32_t*)M.data[28] + W, 4, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipStreamSynchronize(s));
+    const int64_t S = tot[0], TV = tot[1], PT = tot[2], G = tot[3], T = tot[4];
+
+    // 5. outputs
+    if ((rc = M.alloc(0, PT + 1)) || (rc = M.alloc(1, S + 1)) || (rc = M.alloc(2, 8 * S)) || (rc = M.alloc(3, TV)) ||
+        (rc = M.alloc(4, PT)) || (rc = M.alloc(5, PT)) || (rc = M.alloc(6, 2 * S)) || (rc = M.alloc(17, 4 * G)) ||
+        (rc = M.alloc(18, G * window)) || (rc = M.alloc(19, G * window * 2)) || (rc = M.alloc(20, G * window * 2)) ||
+        (rc = M.alloc(21, G)) || (rc = M.alloc(23, T)) || (rc = M.alloc(24, T)) || (rc = M.alloc(25, 16 * T)) ||
+        (rc = M.alloc(26, 64 * T)) || (rc = M.alloc(27, 8 * T)))
+        return rc;
+    DevBuf res_flag;
+    if ((rc = res_flag.alloc(n1))) return rc;
+    TRSPMM_HIP(hipMemsetAsync(res_flag.p, 0, n1, s));
+    Emit E{svals.as<int32_t>(), erow.as<int32_t>(), val, tc_sub_off.as<int32_t>(), tc_val_off.as<int32_t>(),
+           tc_part_off.as<int32_t>(), (const int32_t*)M.data[22], (const int32_t*)M.data[28],
+           (int32_t*)M.data[0], (int32_t*)M.data[1], (int32_t*)M.data[2], (float*)M.data[3], (int32_t*)M.data[4],
+           (int32_t*)M.data[5], (int64_t*)M.data[6], (int32_t*)M.data[17], (uint8_t*)M.data[18],
+           (float*)M.data[19], (int8_t*)M.data[20], (int32_t*)M.data[21], (int32_t*)M.data[22],
+           (int32_t*)M.data[23], (int32_t*)M.data[24], (int32_t*)M.data[25], (int32_t*)M.data[26],
+           (int32_t*)M.data[27], (int32_t*)M.data[28], res_flag.as<uint8_t>()};
+    if (S > 0) {
+        hipLaunchKernelGGL(k_emit_tc, dim3(nblk(S)), dim3(256), 0, s, R, O, E, W, window, cfg.part_t, S);
+        TRSPMM_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_tail, dim3(1), dim3(1), 0, s, E, S, PT, TV);
+    if (G > 0) {
+        hipLaunchKernelGGL(k_emit_groups, dim3(nblk(G)), dim3(256), 0, s, R, O, E, W, window, exact ? 1 : 0, G);
+        TRSPMM_HIP(hipGetLastError());
+    }
+    if (W > 0) {
+        hipLaunchKernelGGL(k_flag_dropped, dim3(nblk(W)), dim3(256), 0, s, R, O, E, W);
+        TRSPMM_HIP(hipGetLastError());
+    }
+    if (T > 0) {
+        hipLaunchKernelGGL(k_pack_tiles, dim3(nblk(T * 8)), dim3(256), 0, s, E, W, window, T);
+        TRSPMM_HIP(hipGetLastError());
+    }
+
+    // residual parts per row
+    DevBuf nl, ns, el, es, pl, ps, ql, qs;
+    const int64_t rw1 = rows + 1;
+    if ((rc = nl.alloc(4 * rw1)) || (rc = ns.alloc(4 * rw1)) || (rc = el.alloc(4 * rw1)) || (rc = es.alloc(4 * rw1)) ||
+        (rc = pl.alloc(4 * rw1)) || (rc = ps.alloc(4 * rw1)) || (rc = ql.alloc(4 * rw1)) || (rc = qs.alloc(4 * rw1)))
+        return rc;
+    ResCounts RCn{nl.as<int32_t>(), ns.as<int32_t>(), el.as<int32_t>(), es.as<int32_t>()};
+    if (rows > 0) {
+        hipLaunchKernelGGL(k_res_count, dim3(nblk(rows)), dim3(256), 0, s, rowptr, res_flag.as<uint8_t>(), rows,
+                           cfg.part_c, cfg.short_c, RCn);
+        TRSPMM_HIP(hipGetLastError());
+    }
+    const int32_t* z = nullptr;
+    if ((rc = exclusive_scan<int32_t>(rows ? nl.as<int32_t>() : z, pl.as<int32_t>(), rows, s)) ||
+        (rc = exclusive_scan<int32_t>(rows ? ns.as<int32_t>() : z, ps.as<int32_t>(), rows, s)) ||
+        (rc = exclusive_scan<int32_t>(rows ? el.as<int32_t>() : z, ql.as<int32_t>(), rows, s)) ||
+        (rc = exclusive_scan<int32_t>(rows ? es.as<int32_t>() : z, qs.as<int32_t>(), rows, s)))
+        return rc;
+    int32_t rt[4];
+    TRSPMM_HIP(hipMemcpyAsync(&rt[0], pl.as<int32_t>() + rows, 4, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipMemcpyAsync(&rt[1], ps.as<int32_t>() + rows, 4, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipMemcpyAsync(&rt[2], ql.as<int32_t>() + rows, 4, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipMemcpyAsync(&rt[3], qs.as<int32_t>() + rows, 4, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipStreamSynchronize(s));
+    const int64_t PL = rt[0], PS = rt[1], EL = rt[2], ES = rt[3];
+    if ((rc = M.alloc(7, PL + 1)) || (rc = M.alloc(8, PL)) || (rc = M.alloc(9, PL)) || (rc = M.alloc(10, EL)) ||
+        (rc = M.alloc(11, EL)) || (rc = M.alloc(12, PS + 1)) || (rc = M.alloc(13, PS)) || (rc = M.alloc(14, PS)) ||
+        (rc = M.alloc(15, ES)) || (rc = M.alloc(16, ES)))
+        return rc;
+    ResOut RO{pl.as<int32_t>(), ps.as<int32_t>(), ql.as<int32_t>(), qs.as<int32_t>(),
+              (int32_t*)M.data[7], (int32_t*)M.data[8], (int32_t*)M.data[9], (int32_t*)M.data[10],
+              (float*)M.data[11], (int32_t*)M.data[12], (int32_t*)M.data[13], (int32_t*)M.data[14],
+              (int32_t*)M.data[15], (float*)M.data[16]};
+    if (rows > 0) {
+        hipLaunchKernelGGL(k_res_emit, dim3(nblk(rows)), dim3(256), 0, s, rowptr, col, val, res_flag.as<uint8_t>(),
+                           n_tcsub.as<int32_t>(), rows, window, cfg.part_c, cfg.short_c, RO);
+        TRSPMM_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_res_tail, dim3(1), dim3(1), 0, s, RO, rows, PL, PS);
+    TRSPMM_HIP(hipGetLastError());
+    TRSPMM_HIP(hipStreamSynchronize(s));
+    const auto t_end = std::chrono::steady_clock::now();
+    M.seconds = std::chrono::duration<double>(t_end - t_begin).count();
+    return TRSPMM_OK;
+}
+
+}  // namespace trspmm

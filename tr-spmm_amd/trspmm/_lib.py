@@ -1,0 +1,123 @@
+"""ctypes binding of libtrspmm.so (C ABI declared in include/trspmm.h).
+
+The library is required: there is no CPU fallback.  Loading fails loudly with
+an ImportError naming the missing file when it has not been built
+(`make -C tr-spmm_amd` or `python -c "import __graft_entry__ as g; g.build()"`).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_DIR, "libtrspmm.so")
+
+OK, EINVAL, EHIP, ENOMEM, EUNSUPPORTED = 0, 1, 2, 3, 4
+F16, BF16 = 0, 1
+ROUTE_TC, ROUTE_SPTC, ROUTE_RESIDUAL, ROUTE_ALL = 1, 2, 4, 7
+MODE_COMPAT, MODE_EXACT, MODE_TUNED = 0, 1, 2
+
+# every symbol include/trspmm.h declares (checked by tests/test_abi.py)
+EXPORTS = [
+    "trspmm_last_error", "trspmm_version", "trspmm_device_arch",
+    "trspmm_route", "trspmm_meta_count", "trspmm_meta_data", "trspmm_meta_seconds", "trspmm_meta_free",
+    "trspmm_plan_create", "trspmm_plan_from_meta", "trspmm_plan_create_packed", "trspmm_plan_free",
+    "trspmm_plan_stats", "trspmm_spmm", "trspmm_spmm_timed", "trspmm_memcpy", "trspmm_host_sort_order",
+]
+
+
+class RouteCfg(ctypes.Structure):
+    _fields_ = [
+        ("part_t", ctypes.c_int32), ("part_c", ctypes.c_int32), ("short_c", ctypes.c_int32),
+        ("density", ctypes.c_int32), ("window", ctypes.c_int32), ("wide", ctypes.c_int32),
+        ("tc_min_util", ctypes.c_float), ("skip_tc", ctypes.c_int32), ("unit_min_nnz", ctypes.c_int32),
+        ("mode", ctypes.c_int32), ("sptc_min_nnz", ctypes.c_int32), ("reserved", ctypes.c_int32),
+    ]
+
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+
+
+class MetaView(ctypes.Structure):
+    _fields_ = [
+        ("t_row_offset", _P), ("n_tparts", _I64),
+        ("t_block_offset", _P),
+        ("t_column", _P), ("n_tsub", _I64),
+        ("t_value", _P), ("n_tval", _I64),
+        ("t_window_row", _P),
+        ("t_binary", _P),
+        ("c_row_offset", _P), ("n_cparts", _I64),
+        ("c_row", _P),
+        ("c_column", _P), ("n_cval", _I64),
+        ("c_value", _P),
+        ("cs_row_offset", _P), ("n_csparts", _I64),
+        ("cs_row", _P),
+        ("cs_column", _P), ("n_csval", _I64),
+        ("cs_value", _P),
+        ("s_column", _P), ("n_groups", _I64),
+        ("s_value", _P),
+        ("s_pos", _P),
+        ("s_offsets", _P), ("n_windows", _I64),
+        ("t_value_is_f32", ctypes.c_int32), ("c_value_is_f32", ctypes.c_int32),
+        ("window", ctypes.c_int32), ("reserved", ctypes.c_int32),
+    ]
+
+
+class TrspmmError(RuntimeError):
+    """A libtrspmm call returned an error status."""
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libtrspmm.so is not built ({LIB_PATH}); run `make -C tr-spmm_amd`")
+    L = ctypes.CDLL(LIB_PATH)
+    L.trspmm_last_error.restype = ctypes.c_char_p
+    L.trspmm_version.restype = ctypes.c_int
+    L.trspmm_device_arch.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    L.trspmm_route.argtypes = [_P, _P, _P, _I64, _I64, _I64, ctypes.POINTER(RouteCfg), _P, ctypes.POINTER(_P)]
+    L.trspmm_meta_count.restype = _I64
+    L.trspmm_meta_count.argtypes = [_P, ctypes.c_int]
+    L.trspmm_meta_data.restype = _P
+    L.trspmm_meta_data.argtypes = [_P, ctypes.c_int]
+    L.trspmm_meta_seconds.restype = ctypes.c_double
+    L.trspmm_meta_seconds.argtypes = [_P]
+    L.trspmm_meta_free.argtypes = [_P]
+    L.trspmm_plan_create.argtypes = [ctypes.POINTER(MetaView), _I64, _I64, ctypes.c_int, _P, ctypes.POINTER(_P)]
+    L.trspmm_plan_from_meta.argtypes = [_P, _I64, _I64, ctypes.c_int, _P, ctypes.POINTER(_P)]
+    L.trspmm_plan_create_packed.argtypes = [_P, _P, _P, _P, _I64, _I64, ctypes.c_int32, _I64, _I64, _P,
+                                            ctypes.POINTER(_P)]
+    L.trspmm_plan_free.argtypes = [_P]
+    L.trspmm_plan_stats.argtypes = [_P, ctypes.POINTER(_I64)]
+    L.trspmm_spmm.argtypes = [_P, _P, _I64, _I64, _P, _I64, ctypes.c_uint32, _P]
+    L.trspmm_spmm_timed.argtypes = [_P, _P, _I64, _I64, _P, _I64, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, _P,
+                                    ctypes.POINTER(ctypes.c_float)]
+    L.trspmm_memcpy.argtypes = [_P, _P, _I64, _P]
+    L.trspmm_host_sort_order.argtypes = [_P, ctypes.c_int32, _P]
+    _LIB = L
+    return L
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != OK:
+        msg = lib().trspmm_last_error()
+        msg = msg.decode() if msg else "unknown error"
+        raise TrspmmError(f"{what + ': ' if what else ''}{msg} (status {rc})")
+
+
+def current_stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t) -> ctypes.c_void_p:
+    """Device (or host) address of a tensor; None for an empty tensor."""
+    if t is None or t.numel() == 0:
+        return ctypes.c_void_p(0)
+    return ctypes.c_void_p(t.data_ptr())

@@ -1,0 +1,193 @@
+"""Device-side router, plans and the routed multiply (thin wrappers over the C ABI)."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch
+
+from . import _lib
+from ._lib import check, lib, ptr
+
+# names and dtypes of the 29 routed-metadata arrays, in the reference's
+# return order (TR-source/Block/block_online.cpp:844-868)
+META_NAMES = [
+    "t_row_offset", "t_block_offset", "t_column", "t_value", "t_window_row", "t_atomic", "t_binary",
+    "c_row_offset", "c_row", "c_atomic", "c_column", "c_value",
+    "c_row_offset_short", "c_row_short", "c_atomic_short", "c_column_short", "c_value_short",
+    "s_column", "s_mask", "s_value", "s_pos", "s_window", "s_offsets",
+    "s_tile_window", "s_tile_group", "s_tile_column", "s_packed_a", "s_packed_meta",
+    "s_window_tile_offset",
+]
+META_DTYPES = [
+    torch.int32, torch.int32, torch.int32, torch.float32, torch.int32, torch.int32, torch.int64,
+    torch.int32, torch.int32, torch.int32, torch.int32, torch.float32,
+    torch.int32, torch.int32, torch.int32, torch.int32, torch.float32,
+    torch.int32, torch.uint8, torch.float32, torch.int8, torch.int32, torch.int32,
+    torch.int32, torch.int32, torch.int32, torch.int32, torch.int32,
+    torch.int32,
+]
+
+
+def meta_shape(name: str, n: int, window: int):
+    return {
+        "s_column": (-1, 4), "s_mask": (-1, window), "s_value": (-1, window, 2), "s_pos": (-1, window, 2),
+        "s_tile_column": (-1, 16), "s_packed_a": (-1, 8, 4, 2), "s_packed_meta": (-1, 8),
+    }.get(name, (n,))
+
+
+def _require_cuda():
+    if not torch.cuda.is_available():
+        raise RuntimeError("trspmm needs a ROCm GPU (gfx950); no device is visible")
+
+
+def _atoi(s: str) -> int:
+    """C atoi: optional whitespace and sign, then leading digits (0 if none)."""
+    s = s.lstrip()
+    sign, i = 1, 0
+    if i < len(s) and s[i] in "+-":
+        sign = -1 if s[i] == "-" else 1
+        i += 1
+    j = i
+    while j < len(s) and s[j].isdigit():
+        j += 1
+    return sign * int(s[i:j]) if j > i else 0
+
+
+def env_router_knobs():
+    """LIBRA_SKIP_TC and LIBRA_SPTC_UNIT_MIN_NNZ, parsed as the reference does
+    (TR-source/Block/block_online.cpp:118-128, :564-567)."""
+    v = os.environ.get("LIBRA_SKIP_TC")
+    skip = v in ("1", "true", "TRUE") if v is not None else False
+    u = os.environ.get("LIBRA_SPTC_UNIT_MIN_NNZ")
+    unit = 2 if u is None else _atoi(u)
+    return skip, unit
+
+
+MODES = {"compat": _lib.MODE_COMPAT, "exact": _lib.MODE_EXACT, "tuned": _lib.MODE_TUNED}
+
+
+class RoutedMeta:
+    """The router's 29 arrays, resident on the GPU (owned by the library)."""
+
+    def __init__(self, handle, window: int):
+        self._h = handle
+        self.window = window
+        self.seconds = lib().trspmm_meta_seconds(handle)
+        self._views = None
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            try:
+                lib().trspmm_meta_free(h)
+            except Exception:
+                pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def count(self, i: int) -> int:
+        return int(lib().trspmm_meta_count(self._h, i))
+
+    def tensors(self, device="cpu"):
+        """Copies the 29 arrays into fresh tensors (reference shapes) on `device`."""
+        out = []
+        stream = _lib.current_stream()
+        for i, (name, dt) in enumerate(zip(META_NAMES, META_DTYPES)):
+            n = self.count(i)
+            t = torch.empty(n, dtype=dt, device=device)
+            if n:
+                src = lib().trspmm_meta_data(self._h, i)
+                check(lib().trspmm_memcpy(ptr(t), ctypes.c_void_p(src), n * t.element_size(), stream),
+                      "trspmm_memcpy")
+            out.append(t.reshape(meta_shape(name, n, self.window)))
+        return out
+
+
+def route(rowptr: torch.Tensor, col: torch.Tensor, val: torch.Tensor, ncols: int, *, part_t=32, part_c=32,
+          short_c=3, density=8, window=16, wide=16, tc_min_util=0.0, skip_tc=None, unit_min_nnz=None,
+          mode="exact") -> RoutedMeta:
+    """Routes a canonical CSR on the GPU (see trspmm_route)."""
+    _require_cuda()
+    es, eu = env_router_knobs()
+    cfg = _lib.RouteCfg(part_t, part_c, short_c, density, window, wide, float(tc_min_util),
+                        int(es if skip_tc is None else bool(skip_tc)), int(eu if unit_min_nnz is None else unit_min_nnz),
+                        MODES[mode] if isinstance(mode, str) else int(mode), 0, 0)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    rp = rowptr.to(dev, torch.int32).contiguous()
+    ci = col.to(dev, torch.int32).contiguous()
+    vv = val.to(dev, torch.float32).contiguous()
+    h = ctypes.c_void_p()
+    check(lib().trspmm_route(ptr(rp), ptr(ci), ptr(vv), rp.numel() - 1, ci.numel(), int(ncols), ctypes.byref(cfg),
+                             _lib.current_stream(), ctypes.byref(h)), "trspmm_route")
+    return RoutedMeta(h, window)
+
+
+class Plan:
+    """Device plan of a routed matrix (gfx950 lane layouts); owns its memory."""
+
+    def __init__(self, handle, m_out: int, ncols: int, dtype: torch.dtype, keep=()):
+        self._h = handle
+        self.m_out = m_out
+        self.ncols = ncols
+        self.dtype = dtype
+        self._keep = keep          # device tensors the plan was built from (kept alive by the caller's choice)
+        c = (ctypes.c_int64 * 6)()
+        check(lib().trspmm_plan_stats(handle, c), "trspmm_plan_stats")
+        self.stats = dict(zip(["windows", "tc_subblocks", "tc_nnz", "sptc_tiles", "residual_parts",
+                               "residual_nnz"], list(c)))
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            try:
+                lib().trspmm_plan_free(h)
+            except Exception:
+                pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    @staticmethod
+    def from_meta(meta: RoutedMeta, m_out: int, ncols: int, dtype=torch.float16) -> "Plan":
+        h = ctypes.c_void_p()
+        check(lib().trspmm_plan_from_meta(meta.handle, int(m_out), int(ncols), dtype_code(dtype),
+                                          _lib.current_stream(), ctypes.byref(h)), "trspmm_plan_from_meta")
+        return Plan(h, m_out, ncols, dtype)
+
+    def matmul(self, B: torch.Tensor, out: torch.Tensor | None = None, route_mask: int = _lib.ROUTE_ALL):
+        check_operand(B, self.ncols, self.dtype)
+        K = B.shape[1]
+        if out is None:
+            out = torch.empty((self.m_out, K), device=B.device, dtype=torch.float32)
+        check(lib().trspmm_spmm(self._h, ptr(B), B.stride(0), K, ptr(out), out.stride(0), route_mask,
+                                _lib.current_stream()), "trspmm_spmm")
+        return out
+
+    def timed(self, B, out, route_mask=_lib.ROUTE_ALL, warmup=10, iters=10) -> float:
+        ms = ctypes.c_float()
+        check(lib().trspmm_spmm_timed(self._h, ptr(B), B.stride(0), B.shape[1], ptr(out), out.stride(0), route_mask,
+                                      warmup, iters, _lib.current_stream(), ctypes.byref(ms)), "trspmm_spmm_timed")
+        return float(ms.value)
+
+
+def dtype_code(dtype: torch.dtype) -> int:
+    if dtype == torch.float16:
+        return _lib.F16
+    if dtype == torch.bfloat16:
+        return _lib.BF16
+    raise ValueError(f"unsupported operand dtype {dtype} (float16 or bfloat16)")
+
+
+def check_operand(B: torch.Tensor, ncols: int, dtype: torch.dtype):
+    if not B.is_cuda:
+        raise ValueError("B must be a device tensor")
+    if B.dtype != dtype:
+        raise ValueError(f"B dtype {B.dtype} does not match the plan dtype {dtype}")
+    if B.dim() != 2 or B.shape[0] != ncols:
+        raise ValueError(f"B must be [{ncols}, K], got {tuple(B.shape)}")
+    if B.stride(1) != 1 or B.shape[1] % 8 or B.stride(0) % 8 or B.data_ptr() % 16:
+        raise ValueError("B must be row-major with K and the row stride multiples of 8, 16-byte aligned")
