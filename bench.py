@@ -1,0 +1,159 @@
+"""Routed SpMM benchmark (one MI355X).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rmat18_k128] [--mode exact]
+
+A step is one routed multiply C = A·B over the whole matrix (one fused
+kernel launch), with A's routed plan and B already resident in HBM.
+Preprocessing (device router + plan) runs once before the timed region.
+Prints ONE JSON line (rank 0):
+  value      GFLOP/s = 2·nnz·K / (ms_per_step)
+  roofline   HBM-bound: algorithmic bytes per launch
+             nnz·(2+4) + 4·(M+1) + N·K·2 + M·K·4   (fp16 values/B, int32 cols, fp32 C)
+             over the kernel's mean duration (HIP events on the launch stream),
+             against the 8 TB/s MI355X peak; traffic = measured HBM bytes per launch
+             (rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/) when recorded, else null
+  cpu_baseline  the oracle's routed multiply (decode + FP64 CSR product) timed on
+             the host over a bounded row sample, 1 core
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "tr-spmm_amd"), ROOT]
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--workload", default="rmat18_k128")
+    p.add_argument("--mode", default="exact", choices=["exact", "compat", "tuned"])
+    p.add_argument("--dtype", default="fp16", choices=["fp16", "bf16"])
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-check", action="store_true")
+    return p.parse_args()
+
+
+def algorithmic_bytes(nnz, m, n, k, vb=2, cb=4):
+    return nnz * (vb + 4) + 4 * (m + 1) + n * k * vb + m * k * cb
+
+
+def cpu_baseline(csr, K, B16, budget_s=12.0):
+    """Oracle routed multiply on a leading row sample, grown until ~budget_s of work."""
+    import oracle
+    rows = 16 * 256
+    best = None
+    while True:
+        rows = min(rows, csr.m_pad)
+        rp = csr.rowptr[: rows + 1]
+        nnz = int(rp[-1])
+        meta = oracle.route(rp, csr.col[:nnz], csr.val[:nnz], exact=True)
+        Bf = B16.astype(np.float32)
+        t0 = time.perf_counter()
+        oracle.spmm64(meta, Bf, rows)
+        dt = time.perf_counter() - t0
+        best = (rows, nnz, dt)
+        if dt * 4 > budget_s or rows >= csr.m_pad:
+            break
+        rows *= 4
+    rows, nnz, dt = best
+    return {"value": round(2.0 * nnz * K / dt / 1e9, 4), "unit": "GFLOP/s", "cores": 1, "kind": "port",
+            "sample": f"rows [0, {rows}) of the workload ({nnz} nnz, K={K}): oracle decode of the exact-mode "
+                      f"routed metadata + FP64 CSR product, {dt:.2f} s"}
+
+
+def traffic_record(workload):
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("traffic_bytes")
+    return None
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(the multi-GPU sharding is described in DESIGN.md)"}))
+        sys.exit(2)
+    import torch
+
+    from trspmm import graphs
+    from trspmm.sparse import RoutedSparse
+
+    builder, K, bseed = graphs.WORKLOADS[a.workload]
+    t0 = time.time()
+    csr = builder()
+    gen_s = time.time() - t0
+    dtype = torch.float16 if a.dtype == "fp16" else torch.bfloat16
+    B16 = graphs.dense_features(csr.n, K, seed=bseed)                    # fp16 N(0,1)
+    dev = torch.device("cuda", 0)
+    Bt = torch.from_numpy(B16).to(dev)
+    if dtype == torch.bfloat16:
+        Bt = torch.from_numpy(B16.astype(np.float32)).to(dev).to(torch.bfloat16)
+    print(f"[bench] workload={a.workload} m={csr.m} nnz={csr.nnz} K={K} gen={gen_s:.1f}s", file=sys.stderr,
+          flush=True)
+    t0 = time.time()
+    A = RoutedSparse(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val), csr.n,
+                     m=csr.m, mode=a.mode, dtype=dtype)
+    torch.cuda.synchronize()
+    prep_s = time.time() - t0
+    print(f"[bench] routed on device in {A.meta.seconds:.3f}s (plan total {prep_s:.2f}s) stats={A.plan.stats}",
+          file=sys.stderr, flush=True)
+    C = torch.empty((csr.m, K), device=dev, dtype=torch.float32)
+    stream = torch.cuda.current_stream()
+    for _ in range(a.warmup):
+        A.matmul(Bt, C)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter()
+    e0.record(stream)
+    for _ in range(a.steps):
+        A.matmul(Bt, C)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall
+    ms = e0.elapsed_time(e1) / a.steps
+    flops = 2.0 * csr.nnz * K
+    gflops = flops / (ms * 1e-3) / 1e9
+    byts = algorithmic_bytes(csr.nnz, csr.m, csr.n, K)
+    achieved = byts / (ms * 1e-3) / 1e9
+
+    err = None
+    if not a.no_check:
+        import scipy.sparse as sp
+        Af = sp.csr_matrix((csr.val.astype(np.float64), csr.col, csr.rowptr), shape=(csr.m_pad, csr.n))
+        ref = np.asarray(Af @ B16.astype(np.float64))[: csr.m]
+        got = C.cpu().numpy().astype(np.float64)
+        err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    traffic = traffic_record(a.workload)
+    line = {
+        "metric": "SpMM GFLOP/s (2·nnz·K) + %HBM roofline",
+        "value": round(gflops, 2), "unit": "GFLOP/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": a.dtype, "data": "synthetic (seeded generator, trspmm.graphs)",
+        "config": {"workload": a.workload, "rows": csr.m, "nnz": csr.nnz, "K": K, "mode": a.mode,
+                   "routes": A.plan.stats, "rel_fro_err_vs_fp64": err, "router_s": round(A.meta.seconds, 4),
+                   "wall_s_timed": round(wall, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": int(byts)},
+    }
+    if not a.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(csr, K, B16)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
