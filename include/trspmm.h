@@ -134,7 +134,7 @@ int trspmm_plan_create_packed(const int32_t* s_tile_column, const int32_t* s_pac
                               int64_t n_tiles, int64_t n_windows, int32_t window, int64_t m_out,
                               int64_t ncols, void* stream, trspmm_plan** out);
 void trspmm_plan_free(trspmm_plan* plan);
-/* counts: [windows, tc sub-blocks, tc nnz, 2:4 tiles, residual parts, residual nnz] */
+/* counts: [windows, tc sub-blocks, tc nnz, 2:4 tiles, residual work items, residual nnz] */
 int trspmm_plan_stats(const trspmm_plan* plan, int64_t* counts6);
 
 /* ------------------------------------------------------------------ multiply */

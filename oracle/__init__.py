@@ -64,7 +64,7 @@ def lib():
     P = ctypes.c_void_p
     L.oracle_route.restype = P
     L.oracle_route.argtypes = [P, P, P, ctypes.c_int64] + [ctypes.c_int] * 6 + \
-        [ctypes.c_float] + [ctypes.c_int] * 4
+        [ctypes.c_float] + [ctypes.c_int] * 5
     L.oracle_error.restype = ctypes.c_char_p
     L.oracle_error.argtypes = [P]
     L.oracle_get.restype = ctypes.c_int
@@ -111,7 +111,10 @@ def env_knobs():
 
 
 def route(rowptr, col, val, partsize_t=32, partsize_c=32, shortsize=3, density=8, window=16, wide=16,
-          tc_min_util=0.0, skip_tc=None, unit_min_nnz=None, exact=False, faithful=False) -> RoutedMeta:
+          tc_min_util=0.0, skip_tc=None, unit_min_nnz=None, exact=False, faithful=False,
+          sptc_min_nnz=0) -> RoutedMeta:
+    """sptc_min_nnz > 0 (tuned routing, needs exact=True): 2:4 groups with fewer
+    kept slots leave the 2:4 route; their nonzeros join the residual route."""
     L = lib()
     rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
     col = np.ascontiguousarray(col, dtype=np.int32)
@@ -121,7 +124,7 @@ def route(rowptr, col, val, partsize_t=32, partsize_c=32, shortsize=3, density=8
     unit_min_nnz = eu if unit_min_nnz is None else int(unit_min_nnz)
     h = L.oracle_route(_ptr(rowptr), _ptr(col), _ptr(val), rowptr.size - 1,
                        partsize_t, partsize_c, shortsize, density, window, wide, float(tc_min_util),
-                       int(skip_tc), unit_min_nnz, int(bool(exact)), int(bool(faithful)))
+                       int(skip_tc), unit_min_nnz, int(bool(exact)), int(bool(faithful)), int(sptc_min_nnz))
     try:
         err = L.oracle_error(h)
         if err:

@@ -60,7 +60,7 @@ struct Unit {                      // block_online.cpp:28-34
 struct Params {
     int part_t, part_c, short_c, density, window;
     float tc_min_util;
-    int skip_tc, unit_min_nnz, exact, faithful;
+    int skip_tc, unit_min_nnz, exact, faithful, sptc_min_nnz;
 };
 
 struct ResidualPart {              // block_online.cpp:37-45
@@ -357,9 +357,19 @@ void route(const int32_t* rowptr, const int32_t* col, const float* val, int64_t 
             emit_group(g);
         }
 
-        // 6. per-row slots of each group (block_online.cpp:668-703); spills in exact mode
+        // 6. per-row slots of each group (block_online.cpp:668-703); spills in exact mode.
+        //    Tuned mode (sptc_min_nnz > 0): a group with fewer kept slots leaves the
+        //    2:4 route and all of its hits go to the residual route.
         std::unordered_set<int64_t> spilled;         // edge ids moved to the residual route
-        for (int g = g_begin; g < (int)out.s_window.size(); ++g) {
+        const int g_end = (int)out.s_window.size();
+        std::vector<int32_t> kept_cols;
+        int n_kept = 0;
+        for (int g = g_begin; g < g_end; ++g) {
+            std::vector<uint8_t> gm(W);
+            std::vector<float> gv(2 * W);
+            std::vector<int8_t> gp(2 * W);
+            std::vector<int64_t> hit_edges, over_edges;
+            int kept = 0;
             for (int lr = 0; lr < W; ++lr) {
                 uint8_t mask = 0;
                 int used = 0;
@@ -373,21 +383,36 @@ void route(const int32_t* rowptr, const int32_t* col, const float* val, int64_t 
                                            [&](const std::pair<int, int>& h) { return h.first == lr; });
                     if (it == hits.end()) continue;
                     mask |= (uint8_t)(1u << q);
+                    hit_edges.push_back(it->second);
                     if (used < 2) {
                         v[used] = val[it->second];
                         ps[used] = (int8_t)q;
+                        ++kept;
                     } else if (p.exact) {
-                        spilled.insert(it->second);
+                        over_edges.push_back(it->second);
                     }
                     ++used;
                 }
-                s_mask.push_back(mask);
-                out.s_value.push_back(v[0]);
-                out.s_value.push_back(v[1]);
-                out.s_pos.push_back(ps[0]);
-                out.s_pos.push_back(ps[1]);
+                gm[lr] = mask;
+                gv[2 * lr] = v[0];
+                gv[2 * lr + 1] = v[1];
+                gp[2 * lr] = ps[0];
+                gp[2 * lr + 1] = ps[1];
             }
+            if (p.sptc_min_nnz > 0 && kept < p.sptc_min_nnz) {
+                spilled.insert(hit_edges.begin(), hit_edges.end());
+                continue;
+            }
+            spilled.insert(over_edges.begin(), over_edges.end());
+            kept_cols.insert(kept_cols.end(), out.s_column.begin() + 4 * g, out.s_column.begin() + 4 * g + 4);
+            ++n_kept;
+            s_mask.insert(s_mask.end(), gm.begin(), gm.end());
+            out.s_value.insert(out.s_value.end(), gv.begin(), gv.end());
+            out.s_pos.insert(out.s_pos.end(), gp.begin(), gp.end());
         }
+        out.s_column.resize(4 * (size_t)g_begin);
+        out.s_column.insert(out.s_column.end(), kept_cols.begin(), kept_cols.end());
+        out.s_window.resize((size_t)g_begin + n_kept);
         out.s_offsets.push_back((int32_t)out.s_window.size());
 
         // 7. residual funnel (block_online.cpp:705-733)
@@ -551,7 +576,8 @@ extern "C" {
 // On invalid arguments the result carries an error string (oracle_error).
 void* oracle_route(const int32_t* rowptr, const int32_t* col, const float* val, int64_t rows,
                    int part_t, int part_c, int short_c, int density, int window, int wide,
-                   float tc_min_util, int skip_tc, int unit_min_nnz, int exact, int faithful) {
+                   float tc_min_util, int skip_tc, int unit_min_nnz, int exact, int faithful,
+                   int sptc_min_nnz) {
     Routed* r = new Routed();
     // block_online.cpp:551-553, narrowed to window <= 16 (see header)
     if (window <= 0 || window > 16 || wide <= 0 || wide > 32) {
@@ -562,7 +588,12 @@ void* oracle_route(const int32_t* rowptr, const int32_t* col, const float* val, 
         r->error = "partSize_t and partSize_c must be positive";
         return r;
     }
-    Params p{part_t, part_c, short_c, density, window, tc_min_util, skip_tc, unit_min_nnz, exact, faithful};
+    if (sptc_min_nnz > 0 && !exact) {
+        r->error = "sptc_min_nnz (tuned routing) needs exact mode";
+        return r;
+    }
+    Params p{part_t, part_c, short_c, density, window, tc_min_util, skip_tc, unit_min_nnz, exact, faithful,
+             sptc_min_nnz};
     route(rowptr, col, val, rows, p, *r);
     return r;
 }

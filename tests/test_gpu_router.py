@@ -17,10 +17,11 @@ def _to_trspmm_kwargs(kw):
 
 def _compare(csr, kw, mode):
     print(f"[router] nnz={csr.nnz} rows={csr.m_pad} kw={kw} mode={mode}", flush=True)
+    tuned = {"sptc_min_nnz": 8} if mode == "tuned" else {}
     meta = route(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val), csr.n,
-                 mode=mode, **_to_trspmm_kwargs(kw))
+                 mode=mode, **_to_trspmm_kwargs(kw), **tuned)
     got = meta.tensors("cpu")
-    ref = oracle.route(csr.rowptr, csr.col, csr.val, exact=(mode == "exact"), **kw)
+    ref = oracle.route(csr.rowptr, csr.col, csr.val, exact=(mode != "compat"), **kw, **tuned)
     bad = [n for n, gt in zip(META_NAMES, got)
            if not (gt.numpy().dtype == ref[n].dtype and gt.numpy().shape == ref[n].shape
                    and np.array_equal(gt.numpy(), ref[n]))]
@@ -28,7 +29,7 @@ def _compare(csr, kw, mode):
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
-@pytest.mark.parametrize("mode", ["compat", "exact"])
+@pytest.mark.parametrize("mode", ["compat", "exact", "tuned"])
 def test_router_bit_exact(cuda, case, mode):
     _compare(CASES[case](), {}, mode)
 

@@ -80,6 +80,24 @@ def test_exact_mode_represents_every_nonzero_once(case):
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("floor", [4, 8, 16])
+def test_tuned_mode_represents_every_nonzero_once(case, floor):
+    """Tuned routing moves sparse 2:4 groups to the residual route: still every
+    nonzero exactly once, TC untouched, every kept group at or above the floor."""
+    csr = CASES[case]()
+    meta = oracle.route(csr.rowptr, csr.col, csr.val, exact=True, sptc_min_nnz=floor)
+    r, c, v = oracle.decode(meta)
+    assert sorted(zip(r.tolist(), c.tolist(), v.tolist())) == _triplets(csr)
+    base = oracle.route(csr.rowptr, csr.col, csr.val, exact=True)
+    for n in oracle.NAMES:
+        if n.startswith("t_"):
+            assert np.array_equal(meta[n], base[n]), n
+    kept = (meta["s_pos"] >= 0).reshape(meta["s_pos"].shape[0], 32).sum(axis=1)
+    assert np.all(kept >= floor)
+    assert meta["s_window"].size <= base["s_window"].size
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
 def test_compat_mode_drops_exactly_the_overflow(case):
     """compat = exact minus the 3rd/4th hits of each 2:4 group row
     (block_online.cpp:690-694, :727); t_* and s_* identical in both modes."""

@@ -162,7 +162,7 @@ int trspmm_plan_stats(const trspmm_plan* plan, int64_t* c) {
     c[1] = d.n_tsub;
     c[2] = d.n_tval;
     c[3] = d.n_tiles;
-    c[4] = d.n_rparts;
+    c[4] = d.n_ritems;
     c[5] = d.n_rval;
     return TRSPMM_OK;
 }

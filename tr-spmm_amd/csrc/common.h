@@ -144,14 +144,22 @@ __device__ inline float bits16_to_f32(uint16_t b, int dtype) {
 //   sp_cols      [T*16] columns (-1 = empty)
 //   sp_a         [T*16*4] uint32: (tile, row r, group c) -> two 16-bit values (v0 | v1 << 16)
 //   sp_idx       [T*16]  uint16: (tile, row r) -> nibble c = p0 | p1 << 2 (p0 < p1)
-//   res_off      [W+1]  residual parts of window w
-//   res_row/beg/cnt [P] part row, first entry, entry count
+//   res_rowptr   [16W+1] residual route as a CSR over the padded rows
 //   res_cols     [R] / res_vals [R] (plan dtype)
+//   titems       tile-route work items {window, kind, begin, end}:
+//                kind 0 = all tile work of the window (plain stores of the
+//                whole 16-row tile), 1 / 2 = a chunk of TC sub-blocks / 2:4
+//                tiles of a split window (atomic adds)
+//   ritems       residual work items {window, kind | lr << 8, begin, end}:
+//                kind 0 = whole window, stores its rows; 1 = whole window,
+//                adds to rows the tile kernel wrote (read-modify-write);
+//                2 = entries [begin, end) of row 16w+lr of a split window (atomic adds)
+//   split_windows windows whose rows are zeroed first and only atomically updated
 struct PlanDev {
     int64_t windows = 0, m_out = 0, ncols = 0;
     int32_t window = 16;
     int32_t dtype = TRSPMM_F16;
-    int64_t n_tsub = 0, n_tval = 0, n_tiles = 0, n_rparts = 0, n_rval = 0;
+    int64_t n_tsub = 0, n_tval = 0, n_tiles = 0, n_rval = 0;
     const int32_t* tc_sub_off = nullptr;
     const int32_t* tc_cols = nullptr;
     const uint64_t* tc_bits = nullptr;
@@ -161,12 +169,21 @@ struct PlanDev {
     const int32_t* sp_cols = nullptr;
     const uint32_t* sp_a = nullptr;
     const uint16_t* sp_idx = nullptr;
-    const int32_t* res_off = nullptr;
-    const int32_t* res_row = nullptr;
-    const int32_t* res_beg = nullptr;
-    const int32_t* res_cnt = nullptr;
+    const int32_t* res_rowptr = nullptr;
     const int32_t* res_cols = nullptr;
     const uint16_t* res_vals = nullptr;
+    const int4* titems = nullptr;
+    int64_t n_titems = 0;
+    const int4* ritems = nullptr;
+    int64_t n_ritems = 0;
+    const int32_t* split_windows = nullptr;
+    int64_t n_split = 0;
+};
+
+// work-item caps: a window is split when it has more TC sub-blocks, 2:4
+// tiles or residual entries than these
+struct SplitCaps {
+    int32_t tc_sub = 32, tiles = 32, res_entries = 2048;
 };
 
 // Workgroup index -> window with consecutive windows kept on one XCD.

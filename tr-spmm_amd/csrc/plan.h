@@ -9,8 +9,11 @@ struct Plan {
     PlanDev dev;
     DevBuf tc_sub_off, tc_cols, tc_bits, tc_valoff, tc_vals;
     DevBuf sp_tile_off, sp_cols, sp_a, sp_idx;
-    DevBuf res_off, res_row, res_beg, res_cnt, res_cols, res_vals;
+    DevBuf res_rowptr, res_cols, res_vals;
+    DevBuf titems, ritems, split_windows;
+    SplitCaps caps;
     int finalize();
+    int build_items(hipStream_t s);   // work-item list from the per-window ranges
 };
 
 int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtype, hipStream_t s, Plan& P);

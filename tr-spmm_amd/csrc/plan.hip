@@ -5,13 +5,16 @@
 // window order; residual parts (long and short lists) are in row order; 2:4
 // groups are in window order with s_offsets[W+1].  The plan keeps the same
 // nonzeros and values; only layouts change:
-//   * per-window ranges for every route (so one workgroup owns one window),
+//   * per-window ranges of TC sub-blocks and 2:4 tiles,
 //   * 2:4 tiles repacked so a lane reads its row's 4 groups as 16 contiguous
 //     bytes plus one 16-bit index word (the SMFMAC 16x16x64 A-operand form),
-//   * long + short residual parts merged into one per-window part list,
+//   * the residual parts (long list + short list) merged into one CSR over
+//     rows: a row's long-part entries, then its short-part entries, which is
+//     the row's CSR order (block_online.cpp:415-439),
 //   * values converted once to the plan's 16-bit type (fp16 or bf16),
 //   * columns outside [0, ncols) replaced by -1 (contribute nothing), the
-//     bounds rule the reference kernels apply (mGCNkernel_online.cu:296-306).
+//     bounds rule the reference kernels apply (mGCNkernel_online.cu:296-306),
+//   * work-item lists for the tile-route and residual-route kernels.
 #include "plan.h"
 #include "scan.h"
 
@@ -33,49 +36,57 @@ __device__ inline uint16_t to_plan_bits(const void* src, int64_t i, int src_is_f
     return f32_to_bf16_bits((float)__builtin_bit_cast(_Float16, h));
 }
 
-// per window w in [0, W]: the first TC part / residual part at or after w
-__global__ void k_window_bounds(const int32_t* __restrict__ t_window_row, int64_t n_tparts,
-                                const int32_t* __restrict__ t_row_offset,
-                                const int32_t* __restrict__ c_row, int64_t n_cparts,
-                                const int32_t* __restrict__ cs_row, int64_t n_csparts, int64_t W,
-                                int32_t* __restrict__ tc_sub_off, int32_t* __restrict__ lb_long,
-                                int32_t* __restrict__ lb_short) {
+// per window w in [0, W]: first TC sub-block at or after window w
+__global__ void k_tc_bounds(const int32_t* __restrict__ t_window_row, int64_t n_tparts,
+                            const int32_t* __restrict__ t_row_offset, int64_t W, int32_t* __restrict__ tc_sub_off) {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w > W) return;
-    const int32_t row = (int32_t)(w * 16);
-    const int64_t q = lower_bound(t_window_row, n_tparts, row);
+    const int64_t q = lower_bound(t_window_row, n_tparts, (int32_t)(w * 16));
     tc_sub_off[w] = n_tparts ? t_row_offset[q] : 0;
-    lb_long[w] = (int32_t)lower_bound(c_row, n_cparts, row);
-    lb_short[w] = (int32_t)lower_bound(cs_row, n_csparts, row);
 }
 
-__global__ void k_residual_parts(const int32_t* __restrict__ lb_long, const int32_t* __restrict__ lb_short,
-                                 const int32_t* __restrict__ c_row_offset, const int32_t* __restrict__ c_row,
-                                 const int32_t* __restrict__ cs_row_offset, const int32_t* __restrict__ cs_row,
-                                 int64_t n_cval, int64_t W, int32_t* __restrict__ res_off,
-                                 int32_t* __restrict__ res_row, int32_t* __restrict__ res_beg,
-                                 int32_t* __restrict__ res_cnt) {
-    const int64_t w = blockIdx.x;
-    if (w >= W) return;
-    const int32_t l0 = lb_long[w], l1 = lb_long[w + 1], s0 = lb_short[w], s1 = lb_short[w + 1];
-    const int32_t base = l0 + s0;
-    if (threadIdx.x == 0) {
-        res_off[w] = base;
-        if (w == W - 1) res_off[W] = l1 + s1;
+// residual entries of row r: its long-part span then its short-part span
+struct ResSpans {
+    const int32_t *c_row, *c_row_offset, *cs_row, *cs_row_offset;
+    int64_t n_cparts, n_csparts;
+};
+
+__device__ inline void row_spans(const ResSpans& R, int32_t r, int32_t& l0, int32_t& l1, int32_t& s0, int32_t& s1) {
+    const int64_t a = lower_bound(R.c_row, R.n_cparts, r), b = lower_bound(R.c_row, R.n_cparts, r + 1);
+    l0 = R.n_cparts ? R.c_row_offset[a] : 0;
+    l1 = R.n_cparts ? R.c_row_offset[b] : 0;
+    const int64_t c = lower_bound(R.cs_row, R.n_csparts, r), d = lower_bound(R.cs_row, R.n_csparts, r + 1);
+    s0 = R.n_csparts ? R.cs_row_offset[c] : 0;
+    s1 = R.n_csparts ? R.cs_row_offset[d] : 0;
+}
+
+__global__ void k_res_row_counts(ResSpans R, int64_t rows, int32_t* __restrict__ cnt) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    int32_t l0, l1, s0, s1;
+    row_spans(R, (int32_t)r, l0, l1, s0, s1);
+    cnt[r] = (l1 - l0) + (s1 - s0);
+}
+
+// one wave per row: copies the row's long then short entries
+__global__ void k_res_row_fill(ResSpans R, int64_t rows, const int32_t* __restrict__ rowptr,
+                               const int32_t* __restrict__ c_column, const void* __restrict__ c_value,
+                               const int32_t* __restrict__ cs_column, const void* __restrict__ cs_value,
+                               int value_is_f32, int dtype, int64_t ncols, int32_t* __restrict__ cols,
+                               uint16_t* __restrict__ vals) {
+    const int64_t r = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (r >= rows) return;
+    const int lane = threadIdx.x & 63;
+    int32_t l0, l1, s0, s1;
+    row_spans(R, (int32_t)r, l0, l1, s0, s1);
+    const int32_t base = rowptr[r], nl = l1 - l0;
+    for (int32_t i = lane; i < nl; i += 64) {
+        cols[base + i] = clean_col(c_column[l0 + i], ncols);
+        vals[base + i] = to_plan_bits(c_value, l0 + i, value_is_f32, dtype);
     }
-    for (int32_t i = threadIdx.x; i < (l1 - l0) + (s1 - s0); i += blockDim.x) {
-        const int32_t u = base + i;
-        if (i < l1 - l0) {
-            const int32_t p = l0 + i;
-            res_row[u] = c_row[p];
-            res_beg[u] = c_row_offset[p];
-            res_cnt[u] = c_row_offset[p + 1] - c_row_offset[p];
-        } else {
-            const int32_t p = s0 + (i - (l1 - l0));
-            res_row[u] = cs_row[p];
-            res_beg[u] = (int32_t)n_cval + cs_row_offset[p];
-            res_cnt[u] = cs_row_offset[p + 1] - cs_row_offset[p];
-        }
+    for (int32_t i = lane; i < s1 - s0; i += 64) {
+        cols[base + nl + i] = clean_col(cs_column[s0 + i], ncols);
+        vals[base + nl + i] = to_plan_bits(cs_value, s0 + i, value_is_f32, dtype);
     }
 }
 
@@ -95,10 +106,32 @@ __global__ void k_tile_counts(const int32_t* __restrict__ s_offsets, int64_t W, 
     if (w < W) cnt[w] = (s_offsets[w + 1] - s_offsets[w] + 3) / 4;
 }
 
-// One workgroup per window; thread (tile j, row r, group c) fills one A word.
-// Slot normalisation follows block_online.cpp:190-210: both hits -> (p0<p1);
-// one hit at q -> a zero-valued companion index (1 if q == 0 else 0); no hit
-// or a missing group -> indices (0, 1) with zero values.
+// One row's (v0, v1, p0, p1) of a 2:4 group, normalised as block_online.cpp:190-210:
+// both hits -> p0 < p1; one hit at q -> zero-valued companion index (1 if q == 0
+// else 0); no hit or a missing group -> indices (0, 1) with zero values.
+__device__ inline void slot_pair(const float* s_value, const int8_t* s_pos, int32_t g, int32_t g1, int r, int window,
+                                 float& v0, float& v1, int& p0, int& p1) {
+    v0 = v1 = 0.f;
+    p0 = 0;
+    p1 = 1;
+    if (g >= g1 || r >= window) return;
+    const int64_t b = ((int64_t)g * window + r) * 2;
+    const int q0 = s_pos[b], q1 = s_pos[b + 1];
+    const float x0 = s_value[b], x1 = s_value[b + 1];
+    if (q0 >= 0 && q1 >= 0) {
+        if (q0 <= q1) { p0 = q0; p1 = q1; v0 = x0; v1 = x1; }
+        else          { p0 = q1; p1 = q0; v0 = x1; v1 = x0; }
+    } else if (q0 >= 0 || q1 >= 0) {
+        const int q = q0 >= 0 ? q0 : q1;
+        const float x = q0 >= 0 ? x0 : x1;
+        const int dummy = q == 0 ? 1 : 0;
+        if (dummy < q) { p0 = dummy; p1 = q; v1 = x; }
+        else           { p0 = q; p1 = dummy; v0 = x; }
+    }
+}
+
+// One workgroup per window; thread (tile j, row r) fills the row's 4 A words,
+// its index word and (row 0) the tile's columns.
 __global__ void k_tiles_from_groups(const int32_t* __restrict__ s_offsets, const int32_t* __restrict__ s_column,
                                     const float* __restrict__ s_value, const int8_t* __restrict__ s_pos,
                                     const int32_t* __restrict__ tile_off, int64_t W, int window,
@@ -108,55 +141,24 @@ __global__ void k_tiles_from_groups(const int32_t* __restrict__ s_offsets, const
     if (w >= W) return;
     const int32_t g0 = s_offsets[w], g1 = s_offsets[w + 1];
     const int32_t t0 = tile_off[w], nt = tile_off[w + 1] - t0;
-    for (int32_t i = threadIdx.x; i < nt * 64; i += blockDim.x) {
-        const int32_t j = i >> 6, r = (i >> 2) & 15, c = i & 3;
+    for (int32_t i = threadIdx.x; i < nt * 16; i += blockDim.x) {
+        const int32_t j = i >> 4, r = i & 15;
         const int64_t t = t0 + j;
-        const int32_t g = g0 + 4 * j + c;
-        float v0 = 0.f, v1 = 0.f;
-        int p0 = 0, p1 = 1;
-        if (g < g1 && r < window) {
-            const int64_t b = ((int64_t)g * window + r) * 2;
-            const int q0 = s_pos[b], q1 = s_pos[b + 1];
-            const float x0 = s_value[b], x1 = s_value[b + 1];
-            if (q0 >= 0 && q1 >= 0) {
-                if (q0 <= q1) { p0 = q0; p1 = q1; v0 = x0; v1 = x1; }
-                else          { p0 = q1; p1 = q0; v0 = x1; v1 = x0; }
-            } else if (q0 >= 0 || q1 >= 0) {
-                const int q = q0 >= 0 ? q0 : q1;
-                const float x = q0 >= 0 ? x0 : x1;
-                const int dummy = q == 0 ? 1 : 0;
-                if (dummy < q) { p0 = dummy; p1 = q; v1 = x; }
-                else           { p0 = q; p1 = dummy; v0 = x; }
-            }
+        uint32_t idx = 0;
+        for (int c = 0; c < 4; ++c) {
+            const int32_t g = g0 + 4 * j + c;
+            float v0, v1;
+            int p0, p1;
+            slot_pair(s_value, s_pos, g, g1, r, window, v0, v1, p0, p1);
+            const uint32_t lo = dtype == TRSPMM_BF16 ? f32_to_bf16_bits(v0) : f32_to_f16_bits(v0);
+            const uint32_t hi = dtype == TRSPMM_BF16 ? f32_to_bf16_bits(v1) : f32_to_f16_bits(v1);
+            sp_a[(t * 16 + r) * 4 + c] = lo | (hi << 16);
+            idx |= (uint32_t)((p0 & 3) | ((p1 & 3) << 2)) << (4 * c);
+            if (r == 0)
+                for (int q = 0; q < 4; ++q)
+                    sp_cols[t * 16 + 4 * c + q] = g < g1 ? clean_col(s_column[4 * g + q], ncols) : -1;
         }
-        const uint32_t lo = dtype == TRSPMM_BF16 ? f32_to_bf16_bits(v0) : f32_to_f16_bits(v0);
-        const uint32_t hi = dtype == TRSPMM_BF16 ? f32_to_bf16_bits(v1) : f32_to_f16_bits(v1);
-        sp_a[(t * 16 + r) * 4 + c] = lo | (hi << 16);
-        // the four groups of one row share one index word: thread c == 0 writes it
-        if (c == 0) {
-            uint32_t idx = 0;
-            for (int cc = 0; cc < 4; ++cc) {
-                const int32_t gg = g0 + 4 * j + cc;
-                int a0 = 0, a1 = 1;
-                if (gg < g1 && r < window) {
-                    const int64_t b = ((int64_t)gg * window + r) * 2;
-                    const int q0 = s_pos[b], q1 = s_pos[b + 1];
-                    if (q0 >= 0 && q1 >= 0) { a0 = q0 < q1 ? q0 : q1; a1 = q0 < q1 ? q1 : q0; }
-                    else if (q0 >= 0 || q1 >= 0) {
-                        const int q = q0 >= 0 ? q0 : q1;
-                        const int dummy = q == 0 ? 1 : 0;
-                        a0 = dummy < q ? dummy : q;
-                        a1 = dummy < q ? q : dummy;
-                    }
-                }
-                idx |= (uint32_t)((a0 & 3) | ((a1 & 3) << 2)) << (4 * cc);
-            }
-            sp_idx[t * 16 + r] = (uint16_t)idx;
-        }
-        if (r == 0) {
-            for (int q = 0; q < 4; ++q)
-                sp_cols[t * 16 + 4 * c + q] = g < g1 ? clean_col(s_column[4 * g + q], ncols) : -1;
-        }
+        sp_idx[t * 16 + r] = (uint16_t)idx;
     }
 }
 
@@ -174,6 +176,60 @@ __global__ void k_tiles_from_packed(const int32_t* __restrict__ s_tile_column, c
         if (c == 0) sp_idx[t * 16 + r] = (uint16_t)(s_packed_meta[t * 8 + (r & 7)] >> (16 * (r >> 3)));
         if (r == 0)
             for (int q = 0; q < 4; ++q) sp_cols[t * 16 + 4 * c + q] = clean_col(s_tile_column[t * 16 + 4 * c + q], ncols);
+    }
+}
+
+__host__ __device__ inline int32_t cdiv(int32_t a, int32_t b) { return (a + b - 1) / b; }
+
+// Work items.  A window is split when its tile work exceeds the caps or one of
+// its rows holds more residual entries than caps.res_entries; a split window's
+// rows are zeroed first and every contribution to them is an atomic add.
+__global__ void k_item_counts(const int32_t* __restrict__ tc_sub_off, const int32_t* __restrict__ tile_off,
+                              const int32_t* __restrict__ res_rowptr, int64_t W, SplitCaps caps,
+                              int32_t* __restrict__ n_titems, int32_t* __restrict__ n_ritems,
+                              int32_t* __restrict__ is_split) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W) return;
+    const int32_t S = tc_sub_off[w + 1] - tc_sub_off[w], T = tile_off[w + 1] - tile_off[w];
+    int32_t R = 0, rmax = 0, rchunks = 0;
+    for (int lr = 0; lr < 16; ++lr) {
+        const int32_t d = res_rowptr[16 * w + lr + 1] - res_rowptr[16 * w + lr];
+        R += d;
+        rmax = d > rmax ? d : rmax;
+        rchunks += cdiv(d, caps.res_entries);
+    }
+    const bool split = S > caps.tc_sub || T > caps.tiles || R > caps.res_entries;
+    const bool tile = S + T > 0;
+    n_titems[w] = !tile ? 0 : (split ? cdiv(S, caps.tc_sub) + cdiv(T, caps.tiles) : 1);
+    n_ritems[w] = split ? rchunks : ((R > 0 || !tile) ? 1 : 0);
+    is_split[w] = split ? 1 : 0;
+}
+
+__global__ void k_fill_items(const int32_t* __restrict__ tc_sub_off, const int32_t* __restrict__ tile_off,
+                             const int32_t* __restrict__ res_rowptr, int64_t W, SplitCaps caps,
+                             const int32_t* __restrict__ toff, const int32_t* __restrict__ roff,
+                             const int32_t* __restrict__ soff, int4* __restrict__ titems, int4* __restrict__ ritems,
+                             int32_t* __restrict__ split_windows) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W) return;
+    const bool split = soff[w + 1] != soff[w];
+    const int32_t s0 = tc_sub_off[w], s1 = tc_sub_off[w + 1], t0 = tile_off[w], t1 = tile_off[w + 1];
+    int32_t kt = toff[w], kr = roff[w];
+    if (!split) {
+        if (toff[w + 1] > kt) titems[kt] = make_int4((int)w, 0, 0, 0);
+        // residual item: STORE (no tile work) or read-modify-write (tile kernel wrote the rows)
+        if (roff[w + 1] > kr) ritems[kr] = make_int4((int)w, (s1 - s0) + (t1 - t0) > 0 ? 1 : 0, 0, 0);
+        return;
+    }
+    split_windows[soff[w]] = (int32_t)w;
+    for (int32_t b = s0; b < s1; b += caps.tc_sub)
+        titems[kt++] = make_int4((int)w, 1, b, b + caps.tc_sub < s1 ? b + caps.tc_sub : s1);
+    for (int32_t b = t0; b < t1; b += caps.tiles)
+        titems[kt++] = make_int4((int)w, 2, b, b + caps.tiles < t1 ? b + caps.tiles : t1);
+    for (int lr = 0; lr < 16; ++lr) {
+        const int32_t e0 = res_rowptr[16 * w + lr], e1 = res_rowptr[16 * w + lr + 1];
+        for (int32_t b = e0; b < e1; b += caps.res_entries)
+            ritems[kr++] = make_int4((int)w, 2 | (lr << 8), b, b + caps.res_entries < e1 ? b + caps.res_entries : e1);
     }
 }
 
@@ -196,12 +252,79 @@ int Plan::finalize() {
     dev.sp_cols = sp_cols.as<int32_t>();
     dev.sp_a = sp_a.as<uint32_t>();
     dev.sp_idx = sp_idx.as<uint16_t>();
-    dev.res_off = res_off.as<int32_t>();
-    dev.res_row = res_row.as<int32_t>();
-    dev.res_beg = res_beg.as<int32_t>();
-    dev.res_cnt = res_cnt.as<int32_t>();
+    dev.res_rowptr = res_rowptr.as<int32_t>();
     dev.res_cols = res_cols.as<int32_t>();
     dev.res_vals = res_vals.as<uint16_t>();
+    dev.titems = titems.as<int4>();
+    dev.ritems = ritems.as<int4>();
+    dev.split_windows = split_windows.as<int32_t>();
+    return TRSPMM_OK;
+}
+
+int Plan::build_items(hipStream_t s) {
+    const int64_t W = dev.windows;
+    if (W == 0) {
+        dev.n_titems = dev.n_ritems = dev.n_split = 0;
+        return finalize();
+    }
+    DevBuf nt, nr, sp, toff, roff, soff;
+    int rc;
+    if ((rc = nt.alloc(4 * (W + 1))) || (rc = nr.alloc(4 * (W + 1))) || (rc = sp.alloc(4 * (W + 1))) ||
+        (rc = toff.alloc(4 * (W + 1))) || (rc = roff.alloc(4 * (W + 1))) || (rc = soff.alloc(4 * (W + 1))))
+        return rc;
+    hipLaunchKernelGGL(k_item_counts, dim3(blocks_for(W)), dim3(256), 0, s, tc_sub_off.as<int32_t>(),
+                       sp_tile_off.as<int32_t>(), res_rowptr.as<int32_t>(), W, caps, nt.as<int32_t>(),
+                       nr.as<int32_t>(), sp.as<int32_t>());
+    TRSPMM_HIP(hipGetLastError());
+    if ((rc = exclusive_scan<int32_t>(nt.as<int32_t>(), toff.as<int32_t>(), W, s)) ||
+        (rc = exclusive_scan<int32_t>(nr.as<int32_t>(), roff.as<int32_t>(), W, s)) ||
+        (rc = exclusive_scan<int32_t>(sp.as<int32_t>(), soff.as<int32_t>(), W, s)))
+        return rc;
+    int32_t tot[3] = {0, 0, 0};
+    TRSPMM_HIP(hipMemcpyAsync(&tot[0], toff.as<int32_t>() + W, 4, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipMemcpyAsync(&tot[1], roff.as<int32_t>() + W, 4, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipMemcpyAsync(&tot[2], soff.as<int32_t>() + W, 4, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipStreamSynchronize(s));
+    if ((rc = titems.alloc(16 * (int64_t)(tot[0] ? tot[0] : 1))) || (rc = ritems.alloc(16 * (int64_t)(tot[1] ? tot[1] : 1))) ||
+        (rc = split_windows.alloc(4 * (int64_t)(tot[2] ? tot[2] : 1))))
+        return rc;
+    hipLaunchKernelGGL(k_fill_items, dim3(blocks_for(W)), dim3(256), 0, s, tc_sub_off.as<int32_t>(),
+                       sp_tile_off.as<int32_t>(), res_rowptr.as<int32_t>(), W, caps, toff.as<int32_t>(),
+                       roff.as<int32_t>(), soff.as<int32_t>(), titems.as<int4>(), ritems.as<int4>(),
+                       split_windows.as<int32_t>());
+    TRSPMM_HIP(hipGetLastError());
+    TRSPMM_HIP(hipStreamSynchronize(s));
+    dev.n_titems = tot[0];
+    dev.n_ritems = tot[1];
+    dev.n_split = tot[2];
+    return finalize();
+}
+
+// residual row CSR from the long/short part lists
+static int build_residual(const trspmm_meta_view& v, int64_t ncols, int dtype, hipStream_t s, Plan& P) {
+    const int64_t rows = 16 * P.dev.windows;
+    int rc;
+    DevBuf cnt;
+    if ((rc = cnt.alloc(4 * (rows + 1))) || (rc = P.res_rowptr.alloc(4 * (rows + 1)))) return rc;
+    ResSpans RS{v.c_row, v.c_row_offset, v.cs_row, v.cs_row_offset, v.n_cparts, v.n_csparts};
+    if (rows > 0) {
+        hipLaunchKernelGGL(k_res_row_counts, dim3(blocks_for(rows)), dim3(256), 0, s, RS, rows, cnt.as<int32_t>());
+        TRSPMM_HIP(hipGetLastError());
+    }
+    if ((rc = exclusive_scan<int32_t>(rows ? cnt.as<int32_t>() : nullptr, P.res_rowptr.as<int32_t>(), rows, s)))
+        return rc;
+    int32_t total = 0;
+    TRSPMM_HIP(hipMemcpyAsync(&total, P.res_rowptr.as<int32_t>() + rows, 4, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipStreamSynchronize(s));
+    TRSPMM_CHECK(total == v.n_cval + v.n_csval, "residual parts must lie inside the windows' rows");
+    P.dev.n_rval = total;
+    if (total > 0) {
+        if ((rc = P.res_cols.alloc(4 * (int64_t)total)) || (rc = P.res_vals.alloc(2 * (int64_t)total))) return rc;
+        hipLaunchKernelGGL(k_res_row_fill, dim3(blocks_for(rows, 4)), dim3(256), 0, s, RS, rows,
+                           P.res_rowptr.as<int32_t>(), v.c_column, v.c_value, v.cs_column, v.cs_value,
+                           v.c_value_is_f32, dtype, ncols, P.res_cols.as<int32_t>(), P.res_vals.as<uint16_t>());
+        TRSPMM_HIP(hipGetLastError());
+    }
     return TRSPMM_OK;
 }
 
@@ -218,19 +341,14 @@ int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtyp
     P.dev.dtype = dtype;
     P.dev.n_tsub = v.n_tsub;
     P.dev.n_tval = v.n_tval;
-    P.dev.n_rparts = v.n_cparts + v.n_csparts;
-    P.dev.n_rval = v.n_cval + v.n_csval;
 
     int rc;
-    DevBuf lb_long, lb_short, tile_cnt;
-    if ((rc = P.tc_sub_off.alloc(4 * (W + 1))) || (rc = P.res_off.alloc(4 * (W + 1))) ||
-        (rc = P.sp_tile_off.alloc(4 * (W + 1))) || (rc = lb_long.alloc(4 * (W + 1))) ||
-        (rc = lb_short.alloc(4 * (W + 1))) || (rc = tile_cnt.alloc(4 * (W + 1))))
+    DevBuf tile_cnt;
+    if ((rc = P.tc_sub_off.alloc(4 * (W + 1))) || (rc = P.sp_tile_off.alloc(4 * (W + 1))) ||
+        (rc = tile_cnt.alloc(4 * (W + 1))))
         return rc;
-
-    hipLaunchKernelGGL(k_window_bounds, dim3(blocks_for(W + 1)), dim3(256), 0, s, v.t_window_row, v.n_tparts,
-                       v.t_row_offset, v.c_row, v.n_cparts, v.cs_row, v.n_csparts, W, P.tc_sub_off.as<int32_t>(),
-                       lb_long.as<int32_t>(), lb_short.as<int32_t>());
+    hipLaunchKernelGGL(k_tc_bounds, dim3(blocks_for(W + 1)), dim3(256), 0, s, v.t_window_row, v.n_tparts,
+                       v.t_row_offset, W, P.tc_sub_off.as<int32_t>());
     TRSPMM_HIP(hipGetLastError());
 
     // TC sub-blocks
@@ -248,37 +366,8 @@ int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtyp
         TRSPMM_HIP(hipGetLastError());
     }
 
-    // residual parts
-    const int64_t nparts = v.n_cparts + v.n_csparts;
-    if ((rc = P.res_row.alloc(4 * (nparts ? nparts : 1))) || (rc = P.res_beg.alloc(4 * (nparts ? nparts : 1))) ||
-        (rc = P.res_cnt.alloc(4 * (nparts ? nparts : 1))))
-        return rc;
-    if (W > 0) {
-        hipLaunchKernelGGL(k_residual_parts, dim3((unsigned)W), dim3(64), 0, s, lb_long.as<int32_t>(),
-                           lb_short.as<int32_t>(), v.c_row_offset, v.c_row, v.cs_row_offset, v.cs_row, v.n_cval, W,
-                           P.res_off.as<int32_t>(), P.res_row.as<int32_t>(), P.res_beg.as<int32_t>(),
-                           P.res_cnt.as<int32_t>());
-        TRSPMM_HIP(hipGetLastError());
-    } else {
-        TRSPMM_HIP(hipMemsetAsync(P.res_off.p, 0, 4, s));
-    }
-    const int64_t nres = v.n_cval + v.n_csval;
-    if (nres > 0) {
-        if ((rc = P.res_cols.alloc(4 * nres)) || (rc = P.res_vals.alloc(2 * nres))) return rc;
-        if (v.n_cval > 0) {
-            hipLaunchKernelGGL(k_copy_cols, dim3(blocks_for(v.n_cval)), dim3(256), 0, s, v.c_column, v.n_cval, ncols,
-                               P.res_cols.as<int32_t>());
-            hipLaunchKernelGGL(k_copy_vals, dim3(blocks_for(v.n_cval)), dim3(256), 0, s, v.c_value, v.n_cval,
-                               v.c_value_is_f32, dtype, P.res_vals.as<uint16_t>());
-        }
-        if (v.n_csval > 0) {
-            hipLaunchKernelGGL(k_copy_cols, dim3(blocks_for(v.n_csval)), dim3(256), 0, s, v.cs_column, v.n_csval,
-                               ncols, P.res_cols.as<int32_t>() + v.n_cval);
-            hipLaunchKernelGGL(k_copy_vals, dim3(blocks_for(v.n_csval)), dim3(256), 0, s, v.cs_value, v.n_csval,
-                               v.c_value_is_f32, dtype, P.res_vals.as<uint16_t>() + v.n_cval);
-        }
-        TRSPMM_HIP(hipGetLastError());
-    }
+    // residual route: one CSR over the padded rows
+    if ((rc = build_residual(v, ncols, dtype, s, P))) return rc;
 
     // 2:4 tiles
     if (W > 0) {
@@ -301,7 +390,7 @@ int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtyp
         TRSPMM_HIP(hipGetLastError());
     }
     TRSPMM_HIP(hipStreamSynchronize(s));   // temporaries are freed on return
-    return P.finalize();
+    return P.build_items(s);
 }
 
 int build_plan_packed(const int32_t* s_tile_column, const int32_t* s_packed_a, const int32_t* s_packed_meta,
@@ -316,11 +405,11 @@ int build_plan_packed(const int32_t* s_tile_column, const int32_t* s_packed_a, c
     P.dev.dtype = TRSPMM_F16;
     P.dev.n_tiles = T;
     int rc;
-    if ((rc = P.tc_sub_off.alloc(4 * (W + 1))) || (rc = P.res_off.alloc(4 * (W + 1))) ||
+    if ((rc = P.tc_sub_off.alloc(4 * (W + 1))) || (rc = P.res_rowptr.alloc(4 * (16 * W + 1))) ||
         (rc = P.sp_tile_off.alloc(4 * (W + 1))))
         return rc;
     TRSPMM_HIP(hipMemsetAsync(P.tc_sub_off.p, 0, 4 * (W + 1), s));
-    TRSPMM_HIP(hipMemsetAsync(P.res_off.p, 0, 4 * (W + 1), s));
+    TRSPMM_HIP(hipMemsetAsync(P.res_rowptr.p, 0, 4 * (16 * W + 1), s));
     TRSPMM_HIP(hipMemcpyAsync(P.sp_tile_off.p, s_window_tile_offset, 4 * (W + 1), hipMemcpyDeviceToDevice, s));
     if (T > 0) {
         if ((rc = P.sp_cols.alloc(4 * 16 * T)) || (rc = P.sp_a.alloc(4 * 64 * T)) || (rc = P.sp_idx.alloc(2 * 16 * T)))
@@ -331,7 +420,7 @@ int build_plan_packed(const int32_t* s_tile_column, const int32_t* s_packed_a, c
         TRSPMM_HIP(hipGetLastError());
     }
     TRSPMM_HIP(hipStreamSynchronize(s));
-    return P.finalize();
+    return P.build_items(s);
 }
 
 }  // namespace trspmm

@@ -305,6 +305,11 @@ __global__ __launch_bounds__(64) void k_window_decide(RunArrays R, WinOut O, int
     }
 }
 
+__global__ void k_widen(const uint8_t* __restrict__ in, int64_t n, int32_t* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = in[i];
+}
+
 __global__ void k_part_counts(const int32_t* __restrict__ n_tcsub, const int32_t* __restrict__ n_groups, int64_t W,
                               int part_t, int32_t* __restrict__ n_tparts, int32_t* __restrict__ n_tiles) {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -324,8 +329,11 @@ struct Emit {
     const int32_t* tc_sub_off;
     const int32_t* tc_val_off;
     const int32_t* tc_part_off;
-    const int32_t* grp_off;
+    const int32_t* grp_off;       // kept 2:4 groups per window (= s_offsets)
     const int32_t* tile_off;
+    const int32_t* grp_all_off;   // every group the reference builds, per window
+    const int32_t* gmap;          // group (all) -> index among kept groups
+    uint8_t* keep;                // group (all) stays a 2:4 group (TUNED mode drops sparse ones)
     // outputs (block_online.cpp:844-868 order)
     int32_t *t_row_offset, *t_block_offset, *t_column;
     float* t_value;
@@ -416,13 +424,12 @@ __global__ void k_emit_tc(RunArrays R, WinOut O, Emit E, int64_t W, int window, 
     }
 }
 
-// 2:4 groups: one thread per group
-__global__ void k_emit_groups(RunArrays R, WinOut O, Emit E, int64_t W, int window, int exact, int64_t G) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= G) return;
-    const int64_t w = upper_bound_window(E.grp_off, W, (int32_t)g);
+// the runs (<= 4) of group ga (index among all groups), sorted by column
+__device__ inline int group_runs(const RunArrays& R, const WinOut& O, const Emit& E, int64_t ga, int64_t W,
+                                 int64_t& w, int32_t (&runs)[4], int32_t (&cols)[4]) {
+    w = upper_bound_window(E.grp_all_off, W, (int32_t)ga);
     const int32_t rb = R.win_rb[w];
-    const int32_t k = (int32_t)(g - E.grp_off[w]);
+    const int32_t k = (int32_t)(ga - E.grp_all_off[w]);
     int32_t ua, ub = -1;
     if (k < O.n_pairs[w]) {
         ua = R.pair_a[rb + k];
@@ -432,7 +439,6 @@ __global__ void k_emit_groups(RunArrays R, WinOut O, Emit E, int64_t W, int wind
     }
     // columns of the unit(s): rest[2u], rest[2u+1] if that unit is a pair
     const int32_t nr = O.n_rest[w];
-    int32_t runs[4], cols[4];
     int n = 0;
     const int32_t us[2] = {ua, ub};
     for (int t = 0; t < 2; ++t) {
@@ -457,6 +463,55 @@ __global__ void k_emit_groups(RunArrays R, WinOut O, Emit E, int64_t W, int wind
         runs[b + 1] = rr;
         cols[b + 1] = cc;
     }
+    return n;
+}
+
+// TUNED mode: a group whose kept slots (at most 2 hits per row) number fewer
+// than min_nnz leaves the 2:4 route; min_nnz = 0 keeps every group.
+__global__ void k_group_keep(RunArrays R, WinOut O, Emit E, int64_t W, int window, int min_nnz, int64_t GA) {
+    const int64_t ga = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ga >= GA) return;
+    if (min_nnz <= 0) {
+        E.keep[ga] = 1;
+        return;
+    }
+    int64_t w;
+    int32_t runs[4], cols[4];
+    const int n = group_runs(R, O, E, ga, W, w, runs, cols);
+    int hits[16] = {0};
+    for (int q = 0; q < n; ++q)
+        for (int32_t i = R.start[runs[q]]; i < R.start[runs[q] + 1]; ++i) {
+            const int lr = (int)(E.erow[E.sval[i]] - w * window);
+            if (lr >= 0 && lr < 16) ++hits[lr];
+        }
+    int kept = 0;
+    for (int lr = 0; lr < 16; ++lr) kept += hits[lr] < 2 ? hits[lr] : 2;
+    E.keep[ga] = kept >= min_nnz ? 1 : 0;
+}
+
+// kept-group offsets per window and 2:4 tiles per window
+__global__ void k_group_offsets(const int32_t* __restrict__ grp_all_off, const int32_t* __restrict__ gmap, int64_t W,
+                                int32_t* __restrict__ s_offsets, int32_t* __restrict__ n_tiles) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w > W) return;
+    s_offsets[w] = gmap[grp_all_off[w]];
+    if (w < W) n_tiles[w] = (gmap[grp_all_off[w + 1]] - gmap[grp_all_off[w]] + 3) / 4;
+}
+
+// 2:4 groups: one thread per group the reference builds; kept groups are
+// emitted at their kept index, dropped ones send every hit to the residual route
+__global__ void k_emit_groups(RunArrays R, WinOut O, Emit E, int64_t W, int window, int exact, int64_t GA) {
+    const int64_t ga = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ga >= GA) return;
+    int64_t w;
+    int32_t runs[4], cols[4];
+    const int n = group_runs(R, O, E, ga, W, w, runs, cols);
+    if (!E.keep[ga]) {
+        for (int q = 0; q < n; ++q)
+            for (int32_t i = R.start[runs[q]]; i < R.start[runs[q] + 1]; ++i) E.res_flag[E.sval[i]] = 1;
+        return;
+    }
+    const int64_t g = E.gmap[ga];
     for (int q = 0; q < 4; ++q) E.s_column[4 * g + q] = q < n ? cols[q] : -1;
     E.s_window[g] = (int32_t)w;
     int32_t ptr[4];
@@ -669,8 +724,9 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
     TRSPMM_CHECK(cfg.window > 0 && cfg.window <= 16, "window must be in [1, 16]");
     TRSPMM_CHECK(cfg.wide > 0 && cfg.wide <= 32, "wide must be in [1, 32]");
     TRSPMM_CHECK(cfg.part_t > 0 && cfg.part_c > 0, "partSize_t and partSize_c must be positive");
-    TRSPMM_CHECK(cfg.mode == TRSPMM_MODE_COMPAT || cfg.mode == TRSPMM_MODE_EXACT,
-                 "router mode must be COMPAT or EXACT");
+    TRSPMM_CHECK(cfg.mode == TRSPMM_MODE_COMPAT || cfg.mode == TRSPMM_MODE_EXACT || cfg.mode == TRSPMM_MODE_TUNED,
+                 "router mode must be COMPAT, EXACT or TUNED");
+    TRSPMM_CHECK(cfg.sptc_min_nnz >= 0 && cfg.sptc_min_nnz <= 32, "sptc_min_nnz must be in [0, 32]");
     TRSPMM_CHECK(rows >= 0 && nnz >= 0 && nnz < ((int64_t)1 << 31), "bad rows / nnz");
     TRSPMM_CHECK(ncols > 0 && ncols < ((int64_t)1 << 31), "ncols must be in [1, 2^31)");
     const int window = cfg.window;
@@ -772,10 +828,10 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
     }
 
     // 4. window prefix sums -> output sizes
-    DevBuf n_tparts, n_tiles, tc_sub_off, tc_val_off, tc_part_off;
+    DevBuf n_tparts, n_tiles, tc_sub_off, tc_val_off, tc_part_off, grp_all_off;
     if ((rc = n_tparts.alloc(4 * (W + 1))) || (rc = n_tiles.alloc(4 * (W + 1))) ||
         (rc = tc_sub_off.alloc(4 * (W + 1))) || (rc = tc_val_off.alloc(4 * (W + 1))) ||
-        (rc = tc_part_off.alloc(4 * (W + 1))))
+        (rc = tc_part_off.alloc(4 * (W + 1))) || (rc = grp_all_off.alloc(4 * (W + 1))))
         return rc;
     if ((rc = M.alloc(22, W + 1)) || (rc = M.alloc(28, W + 1))) return rc;   // s_offsets, s_window_tile_offset
     if (W > 0) {
@@ -787,17 +843,47 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
     if ((rc = exclusive_scan<int32_t>(W ? n_tcsub.as<int32_t>() : nz, tc_sub_off.as<int32_t>(), W, s)) ||
         (rc = exclusive_scan<int32_t>(W ? n_tcval.as<int32_t>() : nz, tc_val_off.as<int32_t>(), W, s)) ||
         (rc = exclusive_scan<int32_t>(W ? n_tparts.as<int32_t>() : nz, tc_part_off.as<int32_t>(), W, s)) ||
-        (rc = exclusive_scan<int32_t>(W ? n_groups.as<int32_t>() : nz, (int32_t*)M.data[22], W, s)) ||
-        (rc = exclusive_scan<int32_t>(W ? n_tiles.as<int32_t>() : nz, (int32_t*)M.data[28], W, s)))
+        (rc = exclusive_scan<int32_t>(W ? n_groups.as<int32_t>() : nz, grp_all_off.as<int32_t>(), W, s)))
         return rc;
     int32_t tot[5];
     TRSPMM_HIP(hipMemcpyAsync(&tot[0], tc_sub_off.as<int32_t>() + W, 4, hipMemcpyDeviceToHost, s));
     TRSPMM_HIP(hipMemcpyAsync(&tot[1], tc_val_off.as<int32_t>() + W, 4, hipMemcpyDeviceToHost, s));
     TRSPMM_HIP(hipMemcpyAsync(&tot[2], tc_part_off.as<int32_t>() + W, 4, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipMemcpyAsync(&tot[3], grp_all_off.as<int32_t>() + W, 4, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipStreamSynchronize(s));
+    const int64_t S = tot[0], TV = tot[1], PT = tot[2], GA = tot[3];
+
+    // 2:4 groups that stay on the 2:4 route (all but sparse ones in TUNED mode)
+    DevBuf res_flag, keep, gmap;
+    if ((rc = res_flag.alloc(n1)) || (rc = keep.alloc(GA > 0 ? GA : 1)) || (rc = gmap.alloc(4 * (GA + 1))))
+        return rc;
+    TRSPMM_HIP(hipMemsetAsync(res_flag.p, 0, n1, s));
+    Emit E{svals.as<int32_t>(), erow.as<int32_t>(), val, tc_sub_off.as<int32_t>(), tc_val_off.as<int32_t>(),
+           tc_part_off.as<int32_t>(), (const int32_t*)M.data[22], (const int32_t*)M.data[28],
+           grp_all_off.as<int32_t>(), gmap.as<int32_t>(), keep.as<uint8_t>()};
+    if (GA > 0) {
+        hipLaunchKernelGGL(k_group_keep, dim3(nblk(GA)), dim3(256), 0, s, R, O, E, W, window,
+                           cfg.mode == TRSPMM_MODE_TUNED ? cfg.sptc_min_nnz : 0, GA);
+        TRSPMM_HIP(hipGetLastError());
+    }
+    // gmap = exclusive scan of keep (bytes widened through the int32 counts buffer)
+    {
+        DevBuf keep32;
+        if ((rc = keep32.alloc(4 * (GA > 0 ? GA : 1)))) return rc;
+        if (GA > 0) {
+            hipLaunchKernelGGL(k_widen, dim3(nblk(GA)), dim3(256), 0, s, keep.as<uint8_t>(), GA, keep32.as<int32_t>());
+            TRSPMM_HIP(hipGetLastError());
+        }
+        if ((rc = exclusive_scan<int32_t>(GA ? keep32.as<int32_t>() : nz, gmap.as<int32_t>(), GA, s))) return rc;
+    }
+    hipLaunchKernelGGL(k_group_offsets, dim3(nblk(W + 1)), dim3(256), 0, s, grp_all_off.as<int32_t>(),
+                       gmap.as<int32_t>(), W, (int32_t*)M.data[22], n_tiles.as<int32_t>());
+    TRSPMM_HIP(hipGetLastError());
+    if ((rc = exclusive_scan<int32_t>(W ? n_tiles.as<int32_t>() : nz, (int32_t*)M.data[28], W, s))) return rc;
     TRSPMM_HIP(hipMemcpyAsync(&tot[3], (int32_t*)M.data[22] + W, 4, hipMemcpyDeviceToHost, s));
     TRSPMM_HIP(hipMemcpyAsync(&tot[4], (int32_t*)M.data[28] + W, 4, hipMemcpyDeviceToHost, s));
     TRSPMM_HIP(hipStreamSynchronize(s));
-    const int64_t S = tot[0], TV = tot[1], PT = tot[2], G = tot[3], T = tot[4];
+    const int64_t G = tot[3], T = tot[4];
 
     // 5. outputs
     if ((rc = M.alloc(0, PT + 1)) || (rc = M.alloc(1, S + 1)) || (rc = M.alloc(2, 8 * S)) || (rc = M.alloc(3, TV)) ||
@@ -806,23 +892,34 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
         (rc = M.alloc(21, G)) || (rc = M.alloc(23, T)) || (rc = M.alloc(24, T)) || (rc = M.alloc(25, 16 * T)) ||
         (rc = M.alloc(26, 64 * T)) || (rc = M.alloc(27, 8 * T)))
         return rc;
-    DevBuf res_flag;
-    if ((rc = res_flag.alloc(n1))) return rc;
-    TRSPMM_HIP(hipMemsetAsync(res_flag.p, 0, n1, s));
-    Emit E{svals.as<int32_t>(), erow.as<int32_t>(), val, tc_sub_off.as<int32_t>(), tc_val_off.as<int32_t>(),
-           tc_part_off.as<int32_t>(), (const int32_t*)M.data[22], (const int32_t*)M.data[28],
-           (int32_t*)M.data[0], (int32_t*)M.data[1], (int32_t*)M.data[2], (float*)M.data[3], (int32_t*)M.data[4],
-           (int32_t*)M.data[5], (int64_t*)M.data[6], (int32_t*)M.data[17], (uint8_t*)M.data[18],
-           (float*)M.data[19], (int8_t*)M.data[20], (int32_t*)M.data[21], (int32_t*)M.data[22],
-           (int32_t*)M.data[23], (int32_t*)M.data[24], (int32_t*)M.data[25], (int32_t*)M.data[26],
-           (int32_t*)M.data[27], (int32_t*)M.data[28], res_flag.as<uint8_t>()};
+    E.t_row_offset = (int32_t*)M.data[0];
+    E.t_block_offset = (int32_t*)M.data[1];
+    E.t_column = (int32_t*)M.data[2];
+    E.t_value = (float*)M.data[3];
+    E.t_window_row = (int32_t*)M.data[4];
+    E.t_atomic = (int32_t*)M.data[5];
+    E.t_binary = (int64_t*)M.data[6];
+    E.s_column = (int32_t*)M.data[17];
+    E.s_mask = (uint8_t*)M.data[18];
+    E.s_value = (float*)M.data[19];
+    E.s_pos = (int8_t*)M.data[20];
+    E.s_window = (int32_t*)M.data[21];
+    E.s_offsets = (int32_t*)M.data[22];
+    E.s_tile_window = (int32_t*)M.data[23];
+    E.s_tile_group = (int32_t*)M.data[24];
+    E.s_tile_column = (int32_t*)M.data[25];
+    E.s_packed_a = (int32_t*)M.data[26];
+    E.s_packed_meta = (int32_t*)M.data[27];
+    E.s_window_tile_offset = (int32_t*)M.data[28];
+    E.res_flag = res_flag.as<uint8_t>();
     if (S > 0) {
         hipLaunchKernelGGL(k_emit_tc, dim3(nblk(S)), dim3(256), 0, s, R, O, E, W, window, cfg.part_t, S);
         TRSPMM_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_tail, dim3(1), dim3(1), 0, s, E, S, PT, TV);
-    if (G > 0) {
-        hipLaunchKernelGGL(k_emit_groups, dim3(nblk(G)), dim3(256), 0, s, R, O, E, W, window, exact ? 1 : 0, G);
+    if (GA > 0) {
+        hipLaunchKernelGGL(k_emit_groups, dim3(nblk(GA)), dim3(256), 0, s, R, O, E, W, window,
+                           exact || cfg.mode == TRSPMM_MODE_TUNED ? 1 : 0, GA);
         TRSPMM_HIP(hipGetLastError());
     }
     if (W > 0) {

@@ -106,15 +106,26 @@ class RoutedMeta:
         return out
 
 
+TUNED_SPTC_MIN_NNZ = 8   # default 2:4 occupancy floor of the tuned router (kept slots of 32)
+
+
 def route(rowptr: torch.Tensor, col: torch.Tensor, val: torch.Tensor, ncols: int, *, part_t=32, part_c=32,
           short_c=3, density=8, window=16, wide=16, tc_min_util=0.0, skip_tc=None, unit_min_nnz=None,
-          mode="exact") -> RoutedMeta:
-    """Routes a canonical CSR on the GPU (see trspmm_route)."""
+          mode="exact", sptc_min_nnz=None) -> RoutedMeta:
+    """Routes a canonical CSR on the GPU (see trspmm_route).
+
+    mode "compat": the reference's metadata bit for bit; "exact": the same t_*/s_*
+    with the reference's dropped 2:4 overflow moved to the residual route;
+    "tuned": exact plus 2:4 groups with fewer than `sptc_min_nnz` kept slots
+    moved to the residual route."""
     _require_cuda()
     es, eu = env_router_knobs()
+    mode_code = MODES[mode] if isinstance(mode, str) else int(mode)
+    if sptc_min_nnz is None:
+        sptc_min_nnz = TUNED_SPTC_MIN_NNZ if mode_code == _lib.MODE_TUNED else 0
     cfg = _lib.RouteCfg(part_t, part_c, short_c, density, window, wide, float(tc_min_util),
                         int(es if skip_tc is None else bool(skip_tc)), int(eu if unit_min_nnz is None else unit_min_nnz),
-                        MODES[mode] if isinstance(mode, str) else int(mode), 0, 0)
+                        mode_code, int(sptc_min_nnz), 0)
     dev = torch.device("cuda", torch.cuda.current_device())
     rp = rowptr.to(dev, torch.int32).contiguous()
     ci = col.to(dev, torch.int32).contiguous()
@@ -136,7 +147,7 @@ class Plan:
         self._keep = keep          # device tensors the plan was built from (kept alive by the caller's choice)
         c = (ctypes.c_int64 * 6)()
         check(lib().trspmm_plan_stats(handle, c), "trspmm_plan_stats")
-        self.stats = dict(zip(["windows", "tc_subblocks", "tc_nnz", "sptc_tiles", "residual_parts",
+        self.stats = dict(zip(["windows", "tc_subblocks", "tc_nnz", "sptc_tiles", "residual_items",
                                "residual_nnz"], list(c)))
 
     def __del__(self):
