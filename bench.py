@@ -41,6 +41,9 @@ def parse():
     p.add_argument("--dtype", default="fp16", choices=["fp16", "bf16"])
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-check", action="store_true")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="also run a plain 256 MiB device copy (the reference kernel for scaling "
+                        "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     return p.parse_args()
 
 
@@ -129,6 +132,15 @@ def main():
     gflops = flops / (ms * 1e-3) / 1e9
     byts = algorithmic_bytes(csr.nnz, csr.m, csr.n, K)
     achieved = byts / (ms * 1e-3) / 1e9
+
+    if a.copy_calibration:
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)                     # 256 MiB read + 256 MiB written per launch
+        torch.cuda.synchronize()
+        del src, dst
 
     err = None
     if not a.no_check:

@@ -50,7 +50,8 @@ def _run(csr, K, mask=ROUTE_ALL, mode="exact", dtype=torch.float16, seed=1000):
     C = A.matmul(Bt.cuda(), routes=mask)
     torch.cuda.synchronize()
     Bq = Bt.float().numpy()
-    ref = _ref(csr, Bq, mask, mode == "exact", dtype=np.float16 if dtype == torch.float16 else "bf16")
+    kw = {"sptc_min_nnz": 8} if mode == "tuned" else {}
+    ref = _ref(csr, Bq, mask, mode != "compat", kw, dtype=np.float16 if dtype == torch.float16 else "bf16")
     got = C.cpu().numpy()
     den = max(np.linalg.norm(ref), 1e-30)
     err = np.linalg.norm(got - ref) / den
@@ -63,6 +64,21 @@ def _run(csr, K, mask=ROUTE_ALL, mode="exact", dtype=torch.float16, seed=1000):
 @pytest.mark.parametrize("K", [64, 128])
 def test_spmm_all_routes(cuda, case, K):
     err, worst, A = _run(CASES[case](), K)
+    assert err < TOL_FRO and worst < 1e-4, (err, worst, A.stats)
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_spmm_tuned_routing(cuda, case):
+    err, worst, A = _run(CASES[case](), 128, mode="tuned")
+    assert err < TOL_FRO and worst < 1e-4, (err, worst, A.stats)
+
+
+@pytest.mark.parametrize("case", ["hub", "rmat10", "ragged", "dense"])
+@pytest.mark.parametrize("mode", ["exact", "tuned"])
+def test_spmm_matrix_core_residual_engine(cuda, case, mode, monkeypatch):
+    """The matrix-core residual engine (one-hot fragments) gives the same product."""
+    monkeypatch.setenv("TRSPMM_RESIDUAL", "mfma")
+    err, worst, A = _run(CASES[case](), 128, mode=mode)
     assert err < TOL_FRO and worst < 1e-4, (err, worst, A.stats)
 
 

@@ -146,10 +146,11 @@ __device__ inline float bits16_to_f32(uint16_t b, int dtype) {
 //   sp_idx       [T*16]  uint16: (tile, row r) -> nibble c = p0 | p1 << 2 (p0 < p1)
 //   res_rowptr   [16W+1] residual route as a CSR over the padded rows
 //   res_cols     [R] / res_vals [R] (plan dtype)
-//   titems       tile-route work items {window, kind, begin, end}:
-//                kind 0 = all tile work of the window (plain stores of the
-//                whole 16-row tile), 1 / 2 = a chunk of TC sub-blocks / 2:4
-//                tiles of a split window (atomic adds)
+//   titems       matrix-core work items {window, kind, begin, end}:
+//                kind 0 = all work of the window (TC, 2:4 and, with the
+//                matrix-core residual engine, residual; plain stores of the
+//                whole 16-row tile), 1 / 2 / 3 = a chunk of TC sub-blocks / 2:4
+//                tiles / residual entries of a split window (atomic adds)
 //   ritems       residual work items {window, kind | lr << 8, begin, end}:
 //                kind 0 = whole window, stores its rows; 1 = whole window,
 //                adds to rows the tile kernel wrote (read-modify-write);
@@ -172,6 +173,12 @@ struct PlanDev {
     const int32_t* res_rowptr = nullptr;
     const int32_t* res_cols = nullptr;
     const uint16_t* res_vals = nullptr;
+    // residual entries as 8-byte records col | value << 32 | (row % 16) << 48,
+    // the form the matrix-core residual engine reads
+    const uint64_t* res_rec = nullptr;
+    // residual engine: 1 = matrix cores inside k_tile_routes (one-hot A fragments),
+    // 0 = the vector-ALU CSR kernel k_residual
+    int32_t res_mfma = 0;
     const int4* titems = nullptr;
     int64_t n_titems = 0;
     const int4* ritems = nullptr;

@@ -15,6 +15,9 @@
 //   * columns outside [0, ncols) replaced by -1 (contribute nothing), the
 //     bounds rule the reference kernels apply (mGCNkernel_online.cu:296-306),
 //   * work-item lists for the tile-route and residual-route kernels.
+#include <cstdlib>
+#include <cstring>
+
 #include "plan.h"
 #include "scan.h"
 
@@ -184,29 +187,39 @@ __host__ __device__ inline int32_t cdiv(int32_t a, int32_t b) { return (a + b - 
 // Work items.  A window is split when its tile work exceeds the caps or one of
 // its rows holds more residual entries than caps.res_entries; a split window's
 // rows are zeroed first and every contribution to them is an atomic add.
+//
+// Matrix-core residual engine (res_mfma): every window gets matrix-core items
+// only (kind 0, or kinds 1/2/3 chunks when split).  Vector-ALU engine: tile
+// items for windows with TC/2:4 work, residual items for the rest (kind 0
+// store, kind 1 read-modify-write after the tile kernel, kind 2 per-row
+// chunks of split windows).
 __global__ void k_item_counts(const int32_t* __restrict__ tc_sub_off, const int32_t* __restrict__ tile_off,
-                              const int32_t* __restrict__ res_rowptr, int64_t W, SplitCaps caps,
+                              const int32_t* __restrict__ res_rowptr, int64_t W, SplitCaps caps, int res_mfma,
                               int32_t* __restrict__ n_titems, int32_t* __restrict__ n_ritems,
                               int32_t* __restrict__ is_split) {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= W) return;
     const int32_t S = tc_sub_off[w + 1] - tc_sub_off[w], T = tile_off[w + 1] - tile_off[w];
-    int32_t R = 0, rmax = 0, rchunks = 0;
+    int32_t R = 0, rchunks = 0;
     for (int lr = 0; lr < 16; ++lr) {
         const int32_t d = res_rowptr[16 * w + lr + 1] - res_rowptr[16 * w + lr];
         R += d;
-        rmax = d > rmax ? d : rmax;
         rchunks += cdiv(d, caps.res_entries);
     }
     const bool split = S > caps.tc_sub || T > caps.tiles || R > caps.res_entries;
     const bool tile = S + T > 0;
-    n_titems[w] = !tile ? 0 : (split ? cdiv(S, caps.tc_sub) + cdiv(T, caps.tiles) : 1);
-    n_ritems[w] = split ? rchunks : ((R > 0 || !tile) ? 1 : 0);
+    if (res_mfma) {
+        n_titems[w] = split ? cdiv(S, caps.tc_sub) + cdiv(T, caps.tiles) + cdiv(R, caps.res_entries) : 1;
+        n_ritems[w] = 0;
+    } else {
+        n_titems[w] = !tile ? 0 : (split ? cdiv(S, caps.tc_sub) + cdiv(T, caps.tiles) : 1);
+        n_ritems[w] = split ? rchunks : ((R > 0 || !tile) ? 1 : 0);
+    }
     is_split[w] = split ? 1 : 0;
 }
 
 __global__ void k_fill_items(const int32_t* __restrict__ tc_sub_off, const int32_t* __restrict__ tile_off,
-                             const int32_t* __restrict__ res_rowptr, int64_t W, SplitCaps caps,
+                             const int32_t* __restrict__ res_rowptr, int64_t W, SplitCaps caps, int res_mfma,
                              const int32_t* __restrict__ toff, const int32_t* __restrict__ roff,
                              const int32_t* __restrict__ soff, int4* __restrict__ titems, int4* __restrict__ ritems,
                              int32_t* __restrict__ split_windows) {
@@ -226,11 +239,27 @@ __global__ void k_fill_items(const int32_t* __restrict__ tc_sub_off, const int32
         titems[kt++] = make_int4((int)w, 1, b, b + caps.tc_sub < s1 ? b + caps.tc_sub : s1);
     for (int32_t b = t0; b < t1; b += caps.tiles)
         titems[kt++] = make_int4((int)w, 2, b, b + caps.tiles < t1 ? b + caps.tiles : t1);
+    if (res_mfma) {
+        const int32_t e0 = res_rowptr[16 * w], e1 = res_rowptr[16 * w + 16];
+        for (int32_t b = e0; b < e1; b += caps.res_entries)
+            titems[kt++] = make_int4((int)w, 3, b, b + caps.res_entries < e1 ? b + caps.res_entries : e1);
+        return;
+    }
     for (int lr = 0; lr < 16; ++lr) {
         const int32_t e0 = res_rowptr[16 * w + lr], e1 = res_rowptr[16 * w + lr + 1];
         for (int32_t b = e0; b < e1; b += caps.res_entries)
             ritems[kr++] = make_int4((int)w, 2 | (lr << 8), b, b + caps.res_entries < e1 ? b + caps.res_entries : e1);
     }
+}
+
+// 8-byte residual records (col | value << 32 | local row << 48), one wave per row
+__global__ void k_res_records(const int32_t* __restrict__ rowptr, int64_t rows, const int32_t* __restrict__ cols,
+                              const uint16_t* __restrict__ vals, uint64_t* __restrict__ rec) {
+    const int64_t r = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (r >= rows) return;
+    const uint64_t lr = (uint64_t)(r & 15) << 48;
+    for (int32_t e = rowptr[r] + (threadIdx.x & 63); e < rowptr[r + 1]; e += 64)
+        rec[e] = (uint64_t)(uint32_t)cols[e] | ((uint64_t)vals[e] << 32) | lr;
 }
 
 inline unsigned blocks_for(int64_t n, int threads = 256) {
@@ -255,6 +284,7 @@ int Plan::finalize() {
     dev.res_rowptr = res_rowptr.as<int32_t>();
     dev.res_cols = res_cols.as<int32_t>();
     dev.res_vals = res_vals.as<uint16_t>();
+    dev.res_rec = res_rec.as<uint64_t>();
     dev.titems = titems.as<int4>();
     dev.ritems = ritems.as<int4>();
     dev.split_windows = split_windows.as<int32_t>();
@@ -273,8 +303,8 @@ int Plan::build_items(hipStream_t s) {
         (rc = toff.alloc(4 * (W + 1))) || (rc = roff.alloc(4 * (W + 1))) || (rc = soff.alloc(4 * (W + 1))))
         return rc;
     hipLaunchKernelGGL(k_item_counts, dim3(blocks_for(W)), dim3(256), 0, s, tc_sub_off.as<int32_t>(),
-                       sp_tile_off.as<int32_t>(), res_rowptr.as<int32_t>(), W, caps, nt.as<int32_t>(),
-                       nr.as<int32_t>(), sp.as<int32_t>());
+                       sp_tile_off.as<int32_t>(), res_rowptr.as<int32_t>(), W, caps, (int)dev.res_mfma,
+                       nt.as<int32_t>(), nr.as<int32_t>(), sp.as<int32_t>());
     TRSPMM_HIP(hipGetLastError());
     if ((rc = exclusive_scan<int32_t>(nt.as<int32_t>(), toff.as<int32_t>(), W, s)) ||
         (rc = exclusive_scan<int32_t>(nr.as<int32_t>(), roff.as<int32_t>(), W, s)) ||
@@ -289,7 +319,7 @@ int Plan::build_items(hipStream_t s) {
         (rc = split_windows.alloc(4 * (int64_t)(tot[2] ? tot[2] : 1))))
         return rc;
     hipLaunchKernelGGL(k_fill_items, dim3(blocks_for(W)), dim3(256), 0, s, tc_sub_off.as<int32_t>(),
-                       sp_tile_off.as<int32_t>(), res_rowptr.as<int32_t>(), W, caps, toff.as<int32_t>(),
+                       sp_tile_off.as<int32_t>(), res_rowptr.as<int32_t>(), W, caps, (int)dev.res_mfma, toff.as<int32_t>(),
                        roff.as<int32_t>(), soff.as<int32_t>(), titems.as<int4>(), ritems.as<int4>(),
                        split_windows.as<int32_t>());
     TRSPMM_HIP(hipGetLastError());
@@ -324,12 +354,25 @@ static int build_residual(const trspmm_meta_view& v, int64_t ncols, int dtype, h
                            P.res_rowptr.as<int32_t>(), v.c_column, v.c_value, v.cs_column, v.cs_value,
                            v.c_value_is_f32, dtype, ncols, P.res_cols.as<int32_t>(), P.res_vals.as<uint16_t>());
         TRSPMM_HIP(hipGetLastError());
+        if ((rc = P.res_rec.alloc(8 * (int64_t)total))) return rc;
+        hipLaunchKernelGGL(k_res_records, dim3(blocks_for(rows, 4)), dim3(256), 0, s, P.res_rowptr.as<int32_t>(),
+                           rows, P.res_cols.as<int32_t>(), P.res_vals.as<uint16_t>(), P.res_rec.as<uint64_t>());
+        TRSPMM_HIP(hipGetLastError());
     }
     return TRSPMM_OK;
 }
 
+// residual engine of a new plan: the vector-ALU CSR kernel (default; measured
+// faster on the R-MAT workloads) or, with TRSPMM_RESIDUAL=mfma in the
+// environment when the plan is built, one-hot fragments on the matrix cores
+static int residual_on_matrix_cores() {
+    const char* e = std::getenv("TRSPMM_RESIDUAL");
+    return (e && std::strcmp(e, "mfma") == 0) ? 1 : 0;
+}
+
 int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtype, hipStream_t s, Plan& P) {
     TRSPMM_CHECK(v.window == 16, "the routed multiply requires window == 16 (16-row MFMA tiles)");
+    P.dev.res_mfma = residual_on_matrix_cores();
     TRSPMM_CHECK(dtype == TRSPMM_F16 || dtype == TRSPMM_BF16, "dtype must be TRSPMM_F16 or TRSPMM_BF16");
     TRSPMM_CHECK(m_out > 0 && ncols > 0, "m_out and ncols must be positive");
     TRSPMM_CHECK(v.n_windows >= 0 && v.n_windows * 16 >= m_out, "windows must cover the output rows");

@@ -85,17 +85,27 @@ struct Gather {
     u32x4 v[MAXK];
 };
 
-template <int F, int R>
+// cols[row * STRIDE] is the column of staged row `row` (STRIDE 2: the low word
+// of a residual entry record)
+template <int F, int R, int STRIDE = 1>
 __device__ inline void gather_rows(Gather<F>& g, const uint16_t* __restrict__ B, int64_t ldb, int f0, int fw,
                                    const int32_t* __restrict__ cols, int nrows, int lane) {
     constexpr int PIECES = F / 8;
+    // unconditional loads from clamped in-range addresses, masked afterwards
+    // (a predicated load would make the compiler wait for each one)
+    int col[R * PIECES / 64];
+#pragma unroll
+    for (int k = 0; k < R * PIECES / 64; ++k) {
+        const int row = (lane + 64 * k) / PIECES;
+        col[k] = cols[(row < nrows ? row : nrows - 1) * STRIDE];
+    }
 #pragma unroll
     for (int k = 0; k < R * PIECES / 64; ++k) {
         const int c = lane + 64 * k;
         const int row = c / PIECES, piece = c % PIECES;
-        const int col = row < nrows ? cols[row] : -1;
-        g.v[k] = u32x4{0u, 0u, 0u, 0u};
-        if (col >= 0 && piece * 8 < fw) g.v[k] = *(const u32x4*)(B + (int64_t)col * ldb + f0 + piece * 8);
+        const bool ok = row < nrows && col[k] >= 0 && piece * 8 < fw;
+        const u32x4 x = *(const u32x4*)(B + (int64_t)(ok ? col[k] : 0) * ldb + f0 + (ok ? piece * 8 : 0));
+        g.v[k] = ok ? x : u32x4{0u, 0u, 0u, 0u};
     }
 }
 
@@ -144,19 +154,24 @@ __global__ __launch_bounds__(64) void k_tile_routes(PlanDev p, const uint16_t* _
     const int4 item = p.titems[it];
     const int64_t w = item.x;
     const bool split = item.y != 0;
-    int32_t s0 = 0, s1 = 0, t0 = 0, t1 = 0;
+    int32_t s0 = 0, s1 = 0, t0 = 0, t1 = 0, e0 = 0, e1 = 0;
     if (item.y == 0) {
         s0 = p.tc_sub_off[w], s1 = p.tc_sub_off[w + 1];
         t0 = p.sp_tile_off[w], t1 = p.sp_tile_off[w + 1];
+        if (p.res_mfma) e0 = p.res_rowptr[w * 16], e1 = p.res_rowptr[w * 16 + 16];
     } else if (item.y == 1) {
         s0 = item.z, s1 = item.w;
-    } else {
+    } else if (item.y == 2) {
         t0 = item.z, t1 = item.w;
+    } else {
+        e0 = item.z, e1 = item.w;
     }
     if (!(mask & TRSPMM_ROUTE_TC)) s1 = s0;
     if (!(mask & TRSPMM_ROUTE_SPTC)) t1 = t0;
+    if (!(mask & TRSPMM_ROUTE_RESIDUAL)) e1 = e0;
     const int r = lane & 15, kg = lane >> 4;
-    const int nts = (s1 - s0 + 3) / 4, ntt = (t1 - t0 + 3) / 4, nsteps = nts + ntt;
+    const int nts = (s1 - s0 + 3) / 4, ntt = (t1 - t0 + 3) / 4, nte = (e1 - e0 + 31) / 32;
+    const int nsteps = nts + ntt + nte;
     const int64_t row0 = w * 16;
     const uint16_t* bp = Bs + (kg * 8 + (r >> 2)) * LDS_STRIDE + 4 * (r & 3);
 
@@ -167,25 +182,45 @@ __global__ __launch_bounds__(64) void k_tile_routes(PlanDev p, const uint16_t* _
         for (int nb = 0; nb < NB; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
 
         Gather<F> g;
+        const int sp0 = nts, re0 = nts + ntt;      // first 2:4 step, first residual step
         auto issue = [&](int step) {
-            if (step < nts) {
+            if (step < sp0) {
                 const int32_t sb = s0 + 4 * step;
                 gather_rows<F, 32>(g, B, ldb, f0, fw, p.tc_cols + (int64_t)sb * 8, (s1 - sb) * 8 < 32 ? (s1 - sb) * 8 : 32,
                                    lane);
-            } else if (step < nsteps) {
-                const int32_t t = t0 + 4 * (step - nts);
+            } else if (step < re0) {
+                const int32_t t = t0 + 4 * (step - sp0);
                 gather_rows<F, 64>(g, B, ldb, f0, fw, p.sp_cols + (int64_t)t * 16, (t1 - t) * 16 < 64 ? (t1 - t) * 16 : 64,
                                    lane);
+            } else if (step < nsteps) {
+                const int32_t eb = e0 + 32 * (step - re0);
+                gather_rows<F, 32, 2>(g, B, ldb, f0, fw, (const int32_t*)(p.res_rec + eb), e1 - eb < 32 ? e1 - eb : 32,
+                                      lane);
             }
         };
         issue(0);
         for (int step = 0; step < nsteps; ++step) {
             __syncthreads();                       // previous step's LDS reads are done
-            if (step < nts) park_rows<F, 32, LDS_STRIDE>(g, Bs, lane);
+            if (step < sp0 || step >= re0) park_rows<F, 32, LDS_STRIDE>(g, Bs, lane);
             else park_rows<F, 64, LDS_STRIDE>(g, Bs, lane);
             issue(step + 1);                       // next step's gathers overlap this step's math
-            if (step < nts) {
-                const X8 a = __builtin_bit_cast(X8, tc_fragment(p, s0 + 4 * step + kg, s1, r));
+            if (step < sp0 || step >= re0) {
+                u16x8 abits;
+                if (step < sp0) {
+                    abits = tc_fragment(p, s0 + 4 * step + kg, s1, r);
+                } else {
+                    // residual entries as a 16x32 one-hot fragment: staged row k = entry k, and
+                    // A[r][k] = value of entry k if its local row is r (records: col | val << 32 | lrow << 48)
+                    const int32_t eb = e0 + 32 * (step - re0) + 8 * kg;
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const int32_t e = eb + q < e1 ? eb + q : e1 - 1;
+                        const uint64_t rec = p.res_rec[e];
+                        const bool hit = eb + q < e1 && (int)((rec >> 48) & 0xffu) == r && (int32_t)rec >= 0;
+                        abits[q] = hit ? (uint16_t)(rec >> 32) : (uint16_t)0;
+                    }
+                }
+                const X8 a = __builtin_bit_cast(X8, abits);
                 __syncthreads();
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb) {
@@ -195,7 +230,7 @@ __global__ __launch_bounds__(64) void k_tile_routes(PlanDev p, const uint16_t* _
                     acc[nb] = mma_dense<DT>(a, __builtin_bit_cast(X8, bb), acc[nb]);
                 }
             } else {
-                const int32_t my = t0 + 4 * (step - nts) + kg;
+                const int32_t my = t0 + 4 * (step - sp0) + kg;
                 u32x4 av = {0u, 0u, 0u, 0u};
                 int idx = 0x4444;                  // (0,1) in every group: valid, multiplied by zeros
                 if (my < t1) {
@@ -265,7 +300,8 @@ __global__ __launch_bounds__(256) void k_residual(PlanDev p, const uint16_t* __r
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     float* tile = tile_all[wv];
     int32_t* rp = rp_all[wv];
-    const int64_t it = xcd_window((int64_t)blockIdx.x * 4 + wv, grid);
+    // blocks (4 items each) are what the dispatcher spreads over the XCDs
+    const int64_t it = xcd_window(blockIdx.x, grid) * 4 + wv;
     if (it >= p.n_ritems) return;                       // the whole wave leaves together (no block barriers below)
     const int4 item = p.ritems[it];
     const int64_t w = item.x;
@@ -298,46 +334,81 @@ __global__ __launch_bounds__(256) void k_residual(PlanDev p, const uint16_t* __r
 #pragma unroll
         for (int q = 0; q < 8; ++q) acc[q] = 0.f;
     };
-    bool first = true;
-    for (int32_t eb = E0 + g * U; eb < E1; eb += G * U) {
-        int32_t c[U];
-        float v[U];
+    // All loads are unconditional from clamped in-range addresses and masked
+    // afterwards: a predicated load would make the compiler wait for each one.
+    const int32_t elast = E1 > E0 ? E1 - 1 : E0;
+    const int fl = on ? f : 0;
+    auto load_cv = [&](int32_t eb, int32_t (&c)[U], uint16_t (&v)[U]) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int32_t e = eb + u;
-            c[u] = e < E1 ? p.res_cols[e] : -1;
-            v[u] = e < E1 ? to_f32<DT>(p.res_vals[e]) : 0.f;
+            const int32_t e = eb + u < elast ? eb + u : elast;
+            c[u] = p.res_cols[e];
+            v[u] = p.res_vals[e];
         }
-        u32x4 b[U];
+    };
+    // B rows addressed with 32-bit element offsets (the launcher checks ncols * ldb < 2^31)
+    const uint16_t* Bf = B + fl;
+    const uint32_t ldb32 = (uint32_t)ldb;
+    auto load_b = [&](const int32_t (&c)[U], u32x4 (&b)[U]) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            b[u] = u32x4{0u, 0u, 0u, 0u};
-            if (on && c[u] >= 0) b[u] = *(const u32x4*)(B + (int64_t)c[u] * ldb + f);
-        }
-        if (first) {                                    // row of this group's first entry
-            while (eb >= rp[row + 1]) ++row;
-            first = false;
-        }
+        for (int u = 0; u < U; ++u) b[u] = *(const u32x4*)(Bf + (uint32_t)(c[u] > 0 ? c[u] : 0) * ldb32);
+    };
+    int32_t eb = E0 + g * U;
+    if (eb < E1) {
+        while (eb >= rp[row + 1]) ++row;                // row of this group's first entry
+        int32_t row_end = rp[row + 1];                  // kept in a register; LDS is read on row changes only
+        int32_t c0[U], c1[U];
+        uint16_t v0[U], v1[U];
+        u32x4 b0[U];
+        load_cv(eb, c0, v0);
+        load_b(c0, b0);
+        load_cv(eb + G * U, c1, v1);
+#pragma unroll 2
+        for (; eb < E1; eb += G * U) {
+            // two-stage pipeline: the next batch's gathers and the one after's
+            // indices are in flight while this batch is multiplied
+            // (loads past the end are clamped to the last entry and never used)
+            u32x4 b1[U];
+            int32_t c2[U];
+            uint16_t v2[U];
+            load_b(c1, b1);
+            load_cv(eb + 2 * G * U, c2, v2);
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int32_t e = eb + u;
-            if (e < E1) {
-                if (e >= rp[row + 1]) {
+            for (int u = 0; u < U; ++u) {
+                const int32_t e = eb + u;
+                if (e >= row_end && e < E1) {
                     flush();
-                    while (e >= rp[row + 1]) ++row;
+                    do {
+                        ++row;
+                        row_end = rp[row + 1];
+                    } while (e >= row_end);
                 }
-                const u16x8 h = __builtin_bit_cast(u16x8, b[u]);
+                // past-the-end / invalid entries read row 0: zero both factors (no 0 * inf)
+                const bool ok = e < E1 && c0[u] >= 0;
+                const float v = ok ? to_f32<DT>(v0[u]) : 0.f;
+                const u16x8 h = __builtin_bit_cast(u16x8, ok ? b0[u] : u32x4{0u, 0u, 0u, 0u});
 #pragma unroll
-                for (int q = 0; q < 8; ++q) acc[q] = __builtin_fmaf(v[u], to_f32<DT>(h[q]), acc[q]);
+                for (int q = 0; q < 8; ++q) acc[q] = __builtin_fmaf(v, to_f32<DT>(h[q]), acc[q]);
+            }
+            // the group's next batch starts G*U entries later: usually a new row
+            if (eb + G * U < E1 && eb + G * U >= row_end) {
+                flush();
+                do {
+                    ++row;
+                    row_end = rp[row + 1];
+                } while (eb + G * U >= row_end);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                b0[u] = b1[u];
+                c0[u] = c1[u];
+                v0[u] = v1[u];
+                c1[u] = c2[u];
+                v1[u] = v2[u];
             }
         }
-        // the next batch of this group starts G*U entries later: usually a new row
-        if (eb + G * U < E1 && eb + G * U >= rp[row + 1]) {
-            flush();
-            while (eb + G * U >= rp[row + 1]) ++row;
-        }
+        flush();
     }
-    if (!first) flush();
     wave_sync();
 
     // write the 16-row tile: 0 = store, 1 = add to the tile kernel's rows, 2 = atomic add
@@ -405,10 +476,10 @@ int launch_typed(const PlanDev& p, const void* B, int64_t ldb, int64_t K, float*
         // lanes per B-row slab: 8 (64 features per pass) or 16 (128 per pass; wider K
         // runs as grid.y slabs so the per-wave LDS tile stays 8 KB)
         const int lpr = K <= 64 ? 8 : 16;
-        const int64_t grid = xcd_grid(p.n_ritems);
+        const int64_t grid = xcd_grid((p.n_ritems + 3) / 4);     // blocks of 4 items
         const unsigned slabs = (unsigned)((K + 8 * lpr - 1) / (8 * lpr));
-        TRSPMM_CHECK(grid / 4 + 1 <= 0x7fffffff, "too many work items");
-        const dim3 g((unsigned)((grid + 3) / 4), slabs);
+        TRSPMM_CHECK(grid <= 0x7fffffff, "too many work items");
+        const dim3 g((unsigned)grid, slabs);
         if (lpr == 8)
             hipLaunchKernelGGL((k_residual<8, DT>), g, dim3(256), 0, s, p, (const uint16_t*)B, ldb, (int)K, C, ldc,
                                mask, grid);
@@ -429,6 +500,7 @@ int launch_spmm(const PlanDev& p, const void* B, int64_t ldb, int64_t K, float* 
     TRSPMM_CHECK(ldc >= K && ldc % 4 == 0, "ldc must be >= K and a multiple of 4");
     TRSPMM_CHECK(((uintptr_t)B & 15) == 0 && ((uintptr_t)C & 15) == 0, "B and C must be 16-byte aligned");
     TRSPMM_CHECK(K <= (int64_t)1 << 24, "K too large");
+    TRSPMM_CHECK(p.ncols * ldb < ((int64_t)1 << 31), "B must hold fewer than 2^31 elements");
     if (p.windows == 0) return TRSPMM_OK;
     if (p.dtype == TRSPMM_BF16) return launch_typed<BF16>(p, B, ldb, K, C, ldc, mask, s);
     return launch_typed<F16>(p, B, ldb, K, C, ldc, mask, s);
