@@ -75,11 +75,16 @@ def cpu_baseline(csr, K, B16, budget_s=12.0):
                       f"routed metadata + FP64 CSR product, {dt:.2f} s"}
 
 
-def traffic_record(workload):
-    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+def traffic_record(workload, mode, dtype):
+    """Measured HBM bytes per routed multiply (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+    scaled by a plain copy's readings), recorded for this workload, routing mode
+    and operand type; None when no such measurement is recorded."""
+    path = os.path.join(ROOT, "profiles", f"spmm_{workload}_traffic.json")
     if os.path.exists(path):
         with open(path) as f:
-            return json.load(f).get("traffic_bytes")
+            rec = json.load(f)
+        if rec.get("mode") == mode and rec.get("dtype", "fp16") == dtype:
+            return rec.get("traffic_bytes")
     return None
 
 
@@ -149,7 +154,7 @@ def main():
         ref = np.asarray(Af @ B16.astype(np.float64))[: csr.m]
         got = C.cpu().numpy().astype(np.float64)
         err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
-    traffic = traffic_record(a.workload)
+    traffic = traffic_record(a.workload, a.mode, a.dtype)
     line = {
         "metric": "SpMM GFLOP/s (2·nnz·K) + %HBM roofline",
         "value": round(gflops, 2), "unit": "GFLOP/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
