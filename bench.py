@@ -92,7 +92,7 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(the multi-GPU sharding is described in DESIGN.md)"}))
+                                   "(one GPU's share of a sharded run is what this benchmark measures)"}))
         sys.exit(2)
     import torch
 

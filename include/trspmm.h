@@ -21,8 +21,10 @@
  *                            single-owner output rows instead of 4 streams + add)
  *   trspmm_spmm_timed     <- the warm-up / timed-epoch loop of the same launcher
  *                            (mGCNkernel_online.cu:1604-1704) and its ms outputs
- * Entry points with no reference counterpart (north-star additions):
- *   trspmm_sddmm, trspmm_plan_transpose, trspmm_batch_*, trspmm_host_sort_order.
+ * Entry points with no reference counterpart (the reference has no backward,
+ * no batching and no device router):
+ *   trspmm_csr_transpose, trspmm_sddmm (backward pass), trspmm_batch_* (many
+ *   graphs in one launch), trspmm_memcpy, trspmm_host_sort_order (diagnostics).
  */
 #ifndef TRSPMM_H
 #define TRSPMM_H
@@ -154,6 +156,27 @@ int trspmm_spmm_timed(const trspmm_plan* plan, const void* B, int64_t ldb, int64
 /* Copies `bytes` between any host/device addresses, ordered on `stream`;
  * returns after the copy completed (used to hand router output to callers). */
 int trspmm_memcpy(void* dst, const void* src, int64_t bytes, void* stream);
+
+/* ------------------------------------------------------------------ backward
+ * No counterpart in the reference (its SpMM has no backward). */
+
+/* A^T of a canonical CSR A (rows x ncols), on the device: t_rowptr[ncols + 1],
+ * t_col[nnz] (row indices of A, ascending within each column), t_val[nnz]
+ * (may be null) and perm[nnz] (A's edge index of each A^T entry), all
+ * caller-allocated device memory.  Synchronous (returns after the stream
+ * finished).  Rejects column indices outside [0, ncols). */
+int trspmm_csr_transpose(const int32_t* d_rowptr, const int32_t* d_col, const float* d_val, int64_t rows,
+                         int64_t nnz, int64_t ncols, int32_t* t_rowptr, int32_t* t_col, float* t_val,
+                         int32_t* perm, void* stream);
+
+/* dvals[e] = sum_f X[row_e, f] * Y[col_e, f] for every edge e of the CSR
+ * (rowptr[rows + 1], col[nnz]; device memory): the gradient of
+ * sum(dY * (A·B)) with respect to A's values, X = dY [rows, K] (ldx),
+ * Y = B [ncols, K] (ldy), both `dtype` (TRSPMM_F16 / TRSPMM_BF16), fp32
+ * accumulation, fp32 dvals in A's edge order.  K, ldx, ldy multiples of 8;
+ * X and Y 16-byte aligned.  Stream-ordered. */
+int trspmm_sddmm(const int32_t* d_rowptr, int64_t rows, const int32_t* d_col, int64_t nnz, const void* X,
+                 int64_t ldx, const void* Y, int64_t ldy, int64_t K, int dtype, float* dvals, void* stream);
 
 /* ------------------------------------------------------------------ diagnostics */
 

@@ -1,0 +1,268 @@
+// Backward-pass building blocks of the routed SpMM (no counterpart in the
+// reference, which has no backward):
+//
+//   csr_transpose  A (rows x ncols, canonical CSR) -> A^T as a canonical CSR
+//                  plus the edge permutation perm[i] = A-edge of A^T entry i.
+//                  A stable radix sort of the edges by column keeps the rows of
+//                  each column ascending, so A^T is canonical and its routed
+//                  plan is the reference router's tiling of A^T.
+//   sddmm          dvals[e] = sum_f X[row_e, f] * Y[col_e, f] over A's pattern
+//                  (X = dY rows x K, Y = B ncols x K, fp16/bf16, fp32 result):
+//                  the gradient of sum(C * dY), C = A·B, with respect to A's
+//                  values, in A's edge order.
+#include <cstring>   // before rocPRIM: its host code calls memset
+
+#include <rocprim/rocprim.hpp>
+
+#include "backward.h"
+#include "scan.h"
+
+namespace trspmm {
+
+namespace {
+
+inline int nblk(int64_t n, int per = 256) { return (int)((n + per - 1) / per); }
+
+__global__ void k_check_cols(const int32_t* __restrict__ col, int64_t nnz, int64_t ncols, int32_t* __restrict__ err) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nnz && (col[e] < 0 || col[e] >= ncols)) atomicOr(err, 1);
+}
+
+__global__ void k_iota(int32_t* __restrict__ v, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) v[e] = (int32_t)e;
+}
+
+// row of edge e: the last r with rowptr[r] <= e (rowptr non-decreasing)
+__device__ inline int32_t row_of(const int32_t* __restrict__ rowptr, int64_t rows, int32_t e) {
+    int64_t lo = 0, hi = rows;            // invariant: rowptr[lo] <= e < rowptr[hi]
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (rowptr[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    return (int32_t)lo;
+}
+
+__global__ void k_transpose_fill(const int32_t* __restrict__ rowptr, int64_t rows, const float* __restrict__ val,
+                                 const int32_t* __restrict__ perm, int64_t nnz, int32_t* __restrict__ t_col,
+                                 float* __restrict__ t_val) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnz) return;
+    const int32_t e = perm[i];
+    t_col[i] = row_of(rowptr, rows, e);
+    if (t_val) t_val[i] = val[e];
+}
+
+// t_rowptr[c] = first sorted position with column >= c, c = 0..ncols
+__global__ void k_transpose_rowptr(const int32_t* __restrict__ skeys, int64_t nnz, int64_t ncols,
+                                   int32_t* __restrict__ t_rowptr) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > ncols) return;
+    t_rowptr[c] = (int32_t)lower_bound<int32_t>(skeys, nnz, (int32_t)c);
+}
+
+// ------------------------------------------------------------------ SDDMM
+//
+// Lane groups of LPR lanes; a group covers 8*LPR features of a row with
+// 16-byte loads (more passes when K is wider) and takes U = 8 consecutive
+// edges per batch from its own contiguous range of EPG edges.  Each lane
+// accumulates its features' partial dot product for the 8 edges; a
+// reduce-scatter butterfly over the group (halving the edge set at each
+// lane-bit) leaves every edge's full sum in 8/LPR lane(s), about one shuffle
+// per edge.
+constexpr int SDDMM_U = 8;
+constexpr int SDDMM_EPG = 256;           // edges per lane group
+constexpr int SDDMM_BLOCK = 256;
+
+// 16-bit value in the low half of `w` as fp32
+template <typename DT>
+__device__ inline float lo16(uint32_t w) {
+    if constexpr (DT::code == TRSPMM_BF16) return __builtin_bit_cast(float, w << 16);
+    else return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
+}
+
+// dot product of 8 packed 16-bit values.  The halves are taken from the 32-bit
+// words with shifts: indexing an 8 x 16-bit vector bit-cast from the load
+// compiled to eight reads of element 0 (hipcc, ROCm 7.2, gfx950).
+template <typename DT>
+__device__ inline float dot8(u32x4 a, u32x4 b) {
+    float s = 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        s = __builtin_fmaf(lo16<DT>(a[i]), lo16<DT>(b[i]), s);
+        s = __builtin_fmaf(lo16<DT>(a[i] >> 16), lo16<DT>(b[i] >> 16), s);
+    }
+    return s;
+}
+
+// one butterfly step over H pairs: lanes with `hi` keep part[H..2H), the others part[0..H)
+template <int H>
+__device__ inline void rs_step(float (&part)[SDDMM_U], bool hi, int bit) {
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+        const float send = hi ? part[k] : part[k + H];
+        const float keep = hi ? part[k + H] : part[k];
+        part[k] = keep + __shfl_xor(send, bit, 64);
+    }
+}
+
+template <int LPR, typename DT>
+__global__ __launch_bounds__(SDDMM_BLOCK) void k_sddmm(const int32_t* __restrict__ rowptr, int64_t rows,
+                                                       const int32_t* __restrict__ col, int64_t nnz,
+                                                       const uint16_t* __restrict__ X, int64_t ldx,
+                                                       const uint16_t* __restrict__ Y, int64_t ldy, int K,
+                                                       float* __restrict__ out) {
+    constexpr int U = SDDMM_U;
+    const int lane = threadIdx.x & 63;
+    const int j = lane % LPR;
+    const int64_t group = ((int64_t)blockIdx.x * SDDMM_BLOCK + threadIdx.x) / LPR;
+    const int64_t g0 = group * SDDMM_EPG;
+    if (g0 >= nnz) return;                // the group leaves together; no barriers below
+    const int64_t g1 = g0 + SDDMM_EPG < nnz ? g0 + SDDMM_EPG : nnz;
+    const int64_t elast = nnz - 1;
+    int32_t row = row_of(rowptr, rows, (int32_t)g0);
+    int32_t row_end = rowptr[row + 1];
+
+    for (int64_t eb = g0; eb < g1; eb += U) {
+        int32_t c[U], r[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t e = eb + u < elast ? eb + u : elast;   // clamped: unconditional loads
+            c[u] = col[e];
+            if (eb + u < g1) {
+                while (e >= row_end) {
+                    ++row;
+                    row_end = rowptr[row + 1];
+                }
+            }
+            r[u] = row;
+        }
+        float part[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) part[u] = 0.f;
+        for (int f = 8 * j; f < K; f += 8 * LPR) {
+            u32x4 xv[U], yv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                xv[u] = *(const u32x4*)(X + (int64_t)r[u] * ldx + f);
+                yv[u] = *(const u32x4*)(Y + (int64_t)c[u] * ldy + f);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) part[u] += dot8<DT>(xv[u], yv[u]);
+        }
+        // reduce-scatter over the group's lanes: at each lane-bit keep the half of
+        // the edge set that bit selects and add the partner lane's copy of it
+        int base = 0;                     // first edge of the lane's remaining set
+        if constexpr (LPR >= 2) {
+            const bool hi = (j & (LPR / 2)) != 0;
+            rs_step<4>(part, hi, LPR / 2);
+            base += hi ? 4 : 0;
+        }
+        if constexpr (LPR >= 4) {
+            const bool hi = (j & (LPR / 4)) != 0;
+            rs_step<2>(part, hi, LPR / 4);
+            base += hi ? 2 : 0;
+        }
+        if constexpr (LPR >= 8) {
+            const bool hi = (j & (LPR / 8)) != 0;
+            rs_step<1>(part, hi, LPR / 8);
+            base += hi ? 1 : 0;
+        }
+        // lane-bits left over once each lane holds one edge: plain all-reduce
+#pragma unroll
+        for (int b = LPR / 16; b >= 1; b >>= 1) part[0] += __shfl_xor(part[0], b, 64);
+        constexpr int NV = LPR >= 8 ? 1 : (LPR >= 4 ? 2 : (LPR >= 2 ? 4 : 8));   // edges per lane
+        constexpr int LOWMASK = LPR >= 16 ? LPR / 8 - 1 : 0;                      // lanes sharing an edge
+        if ((j & LOWMASK) == 0) {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+                const int64_t e = eb + base + k;
+                if (e < g1) out[e] = part[k];
+            }
+        }
+    }
+}
+
+template <typename DT>
+int launch_sddmm_typed(const int32_t* rowptr, int64_t rows, const int32_t* col, int64_t nnz, const void* X,
+                       int64_t ldx, const void* Y, int64_t ldy, int64_t K, float* out, hipStream_t s) {
+    const int lpr = K >= 128 ? 16 : (K >= 64 ? 8 : 4);
+    const int64_t groups = (nnz + SDDMM_EPG - 1) / SDDMM_EPG;
+    const int64_t threads = groups * lpr;
+    const dim3 grid((unsigned)((threads + SDDMM_BLOCK - 1) / SDDMM_BLOCK));
+    const auto* x = (const uint16_t*)X;
+    const auto* y = (const uint16_t*)Y;
+    if (lpr == 16)
+        hipLaunchKernelGGL((k_sddmm<16, DT>), grid, dim3(SDDMM_BLOCK), 0, s, rowptr, rows, col, nnz, x, ldx, y, ldy,
+                           (int)K, out);
+    else if (lpr == 8)
+        hipLaunchKernelGGL((k_sddmm<8, DT>), grid, dim3(SDDMM_BLOCK), 0, s, rowptr, rows, col, nnz, x, ldx, y, ldy,
+                           (int)K, out);
+    else
+        hipLaunchKernelGGL((k_sddmm<4, DT>), grid, dim3(SDDMM_BLOCK), 0, s, rowptr, rows, col, nnz, x, ldx, y, ldy,
+                           (int)K, out);
+    TRSPMM_HIP(hipGetLastError());
+    return TRSPMM_OK;
+}
+
+}  // namespace
+
+int csr_transpose(const int32_t* rowptr, const int32_t* col, const float* val, int64_t rows, int64_t nnz,
+                  int64_t ncols, int32_t* t_rowptr, int32_t* t_col, float* t_val, int32_t* perm, hipStream_t s) {
+    TRSPMM_CHECK(rows >= 0 && nnz >= 0 && ncols >= 0, "csr_transpose: negative size");
+    TRSPMM_CHECK(nnz < (int64_t)1 << 31 && ncols < (int64_t)1 << 31 && rows < (int64_t)1 << 31,
+                 "csr_transpose: sizes must fit int32");
+    TRSPMM_CHECK(t_rowptr && (nnz == 0 || (rowptr && col && t_col && perm)), "csr_transpose: null pointer");
+    int rc;
+    if (nnz == 0) {
+        hipLaunchKernelGGL(k_transpose_rowptr, dim3(nblk(ncols + 1)), dim3(256), 0, s, (const int32_t*)nullptr,
+                           (int64_t)0, ncols, t_rowptr);
+        TRSPMM_HIP(hipGetLastError());
+        return TRSPMM_OK;
+    }
+    DevBuf err, keys_sorted, ids;
+    if ((rc = err.alloc(4)) || (rc = keys_sorted.alloc(4 * nnz)) || (rc = ids.alloc(4 * nnz))) return rc;
+    TRSPMM_HIP(hipMemsetAsync(err.p, 0, 4, s));
+    hipLaunchKernelGGL(k_check_cols, dim3(nblk(nnz)), dim3(256), 0, s, col, nnz, ncols, err.as<int32_t>());
+    TRSPMM_HIP(hipGetLastError());
+    int32_t herr = 0;
+    TRSPMM_HIP(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipStreamSynchronize(s));
+    TRSPMM_CHECK(herr == 0, "csr_transpose: column index out of range [0, ncols)");
+    hipLaunchKernelGGL(k_iota, dim3(nblk(nnz)), dim3(256), 0, s, ids.as<int32_t>(), nnz);
+    TRSPMM_HIP(hipGetLastError());
+    int bits = 1;
+    while (bits < 31 && ((int64_t)1 << bits) < ncols) ++bits;
+    size_t tmp_bytes = 0;
+    TRSPMM_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, col, keys_sorted.as<int32_t>(), ids.as<int32_t>(), perm,
+                                         (size_t)nnz, 0, bits, s));
+    DevBuf tmp;
+    if ((rc = tmp.alloc(tmp_bytes ? tmp_bytes : 1))) return rc;
+    TRSPMM_HIP(rocprim::radix_sort_pairs(tmp.p, tmp_bytes, col, keys_sorted.as<int32_t>(), ids.as<int32_t>(), perm,
+                                         (size_t)nnz, 0, bits, s));
+    hipLaunchKernelGGL(k_transpose_fill, dim3(nblk(nnz)), dim3(256), 0, s, rowptr, rows, val, perm, nnz, t_col, t_val);
+    TRSPMM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_transpose_rowptr, dim3(nblk(ncols + 1)), dim3(256), 0, s, keys_sorted.as<int32_t>(), nnz,
+                       ncols, t_rowptr);
+    TRSPMM_HIP(hipGetLastError());
+    // the temporaries are freed on return: finish the stream's use of them first
+    TRSPMM_HIP(hipStreamSynchronize(s));
+    return TRSPMM_OK;
+}
+
+int launch_sddmm(const int32_t* rowptr, int64_t rows, const int32_t* col, int64_t nnz, const void* X, int64_t ldx,
+                 const void* Y, int64_t ldy, int64_t K, int dtype, float* out, hipStream_t s) {
+    TRSPMM_CHECK(rows >= 0 && nnz >= 0, "sddmm: negative size");
+    TRSPMM_CHECK(K > 0 && K % 8 == 0 && ldx % 8 == 0 && ldy % 8 == 0 && ldx >= K && ldy >= K,
+                 "sddmm: K and the row strides must be multiples of 8 with stride >= K");
+    TRSPMM_CHECK(((uintptr_t)X % 16) == 0 && ((uintptr_t)Y % 16) == 0, "sddmm: X and Y must be 16-byte aligned");
+    TRSPMM_CHECK(dtype == TRSPMM_F16 || dtype == TRSPMM_BF16, "sddmm: dtype must be TRSPMM_F16 or TRSPMM_BF16");
+    TRSPMM_CHECK(nnz < (int64_t)1 << 31, "sddmm: nnz must fit int32");
+    if (nnz == 0) return TRSPMM_OK;
+    TRSPMM_CHECK(rowptr && col && X && Y && out, "sddmm: null pointer");
+    if (dtype == TRSPMM_F16) return launch_sddmm_typed<F16>(rowptr, rows, col, nnz, X, ldx, Y, ldy, K, out, s);
+    return launch_sddmm_typed<BF16>(rowptr, rows, col, nnz, X, ldx, Y, ldy, K, out, s);
+}
+
+}  // namespace trspmm

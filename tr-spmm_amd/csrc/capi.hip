@@ -3,6 +3,7 @@
 #include <new>
 #include <vector>
 
+#include "backward.h"
 #include "lsort.h"
 #include "plan.h"
 #include "router.h"
@@ -215,6 +216,18 @@ int trspmm_memcpy(void* dst, const void* src, int64_t bytes, void* stream) {
 }
 
 // ---------------------------------------------------------------- diagnostics
+// ---------------------------------------------------------------- backward
+int trspmm_csr_transpose(const int32_t* d_rowptr, const int32_t* d_col, const float* d_val, int64_t rows,
+                         int64_t nnz, int64_t ncols, int32_t* t_rowptr, int32_t* t_col, float* t_val,
+                         int32_t* perm, void* stream) {
+    return csr_transpose(d_rowptr, d_col, d_val, rows, nnz, ncols, t_rowptr, t_col, t_val, perm, (hipStream_t)stream);
+}
+
+int trspmm_sddmm(const int32_t* d_rowptr, int64_t rows, const int32_t* d_col, int64_t nnz, const void* X,
+                 int64_t ldx, const void* Y, int64_t ldy, int64_t K, int dtype, float* dvals, void* stream) {
+    return launch_sddmm(d_rowptr, rows, d_col, nnz, X, ldx, Y, ldy, K, dtype, dvals, (hipStream_t)stream);
+}
+
 int trspmm_host_sort_order(const int32_t* key, int32_t n, int32_t* order) {
     TRSPMM_CHECK(n >= 0 && n < (1 << 24), "n must be in [0, 2^24)");
     TRSPMM_CHECK(n == 0 || (key && order), "key and order must not be null");

@@ -23,6 +23,7 @@ EXPORTS = [
     "trspmm_route", "trspmm_meta_count", "trspmm_meta_data", "trspmm_meta_seconds", "trspmm_meta_free",
     "trspmm_plan_create", "trspmm_plan_from_meta", "trspmm_plan_create_packed", "trspmm_plan_free",
     "trspmm_plan_stats", "trspmm_spmm", "trspmm_spmm_timed", "trspmm_memcpy", "trspmm_host_sort_order",
+    "trspmm_csr_transpose", "trspmm_sddmm",
 ]
 
 
@@ -100,6 +101,8 @@ def lib():
                                     ctypes.POINTER(ctypes.c_float)]
     L.trspmm_memcpy.argtypes = [_P, _P, _I64, _P]
     L.trspmm_host_sort_order.argtypes = [_P, ctypes.c_int32, _P]
+    L.trspmm_csr_transpose.argtypes = [_P, _P, _P, _I64, _I64, _I64, _P, _P, _P, _P, _P]
+    L.trspmm_sddmm.argtypes = [_P, _I64, _P, _I64, _P, _I64, _P, _I64, _I64, ctypes.c_int, _P, _P]
     _LIB = L
     return L
 

@@ -4,13 +4,20 @@ Preprocess once (device router -> device plan), multiply many times:
 
     A = RoutedSparse(rowptr, col, val, ncols)      # CSR on CPU or GPU
     C = A @ B                                      # B: [ncols, K] fp16/bf16 on the GPU, C fp32
+
+For the backward pass the handle also builds, once and on first use, the
+routed plan of A^T (`transpose()`), and computes the value gradient over A's
+pattern (`sddmm()`); `trspmm.autograd.spmm` wires both into torch.autograd.
 """
 from __future__ import annotations
+
+import ctypes
 
 import torch
 
 from . import _lib
-from .core import Plan, route
+from ._lib import check, lib, ptr
+from .core import Plan, dtype_code, route
 
 
 class RoutedSparse:
@@ -21,13 +28,24 @@ class RoutedSparse:
         val = torch.as_tensor(val)
         self.rows = int(rowptr.numel() - 1)
         self.m = int(m if m is not None else self.rows)
+        if not 0 <= self.m <= self.rows:
+            raise ValueError(f"m={self.m} must be within [0, rows={self.rows}]")
         self.ncols = int(ncols)
         self.nnz = int(col.numel())
         self.dtype = dtype
         self.mode = mode
+        self.router_kwargs = dict(router_kwargs)
         self.window = int(router_kwargs.get("window", 16))
         self.meta = route(rowptr, col, val, self.ncols, mode=mode, **router_kwargs)
         self.plan = Plan.from_meta(self.meta, self.m, self.ncols, dtype)
+        # device CSR (validated by the router): the SDDMM pattern and the
+        # source of the transposed plan
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self.rowptr = rowptr.to(dev, torch.int32).contiguous()
+        self.col = col.to(dev, torch.int32).contiguous()
+        self.values = val.to(dev, torch.float32).contiguous()
+        self._t = None
+        self._t_perm = None
 
     @property
     def stats(self):
@@ -42,3 +60,40 @@ class RoutedSparse:
         if out is None:
             out = torch.empty((self.m, B.shape[1]), device=B.device, dtype=torch.float32)
         return self.plan.timed(B, out, routes, warmup, iters)
+
+    # ------------------------------------------------------------ backward
+    def transpose(self) -> "RoutedSparse":
+        """A^T (ncols x m) as a routed matrix with the same routing mode and
+        operand type, built on the device on first use and cached."""
+        if self._t is None:
+            if self.m < self.rows and int(self.rowptr[self.m].item()) != self.nnz:
+                raise ValueError("rows beyond m hold nonzeros: A^T would reference rows outside dY")
+            t_rowptr = torch.empty(self.ncols + 1, dtype=torch.int32, device=self.rowptr.device)
+            t_col = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=self.rowptr.device)
+            t_val = torch.empty(max(self.nnz, 1), dtype=torch.float32, device=self.rowptr.device)
+            perm = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=self.rowptr.device)
+            check(lib().trspmm_csr_transpose(ptr(self.rowptr), ptr(self.col), ptr(self.values), self.rows, self.nnz,
+                                             self.ncols, ptr(t_rowptr), ptr(t_col), ptr(t_val), ptr(perm),
+                                             _lib.current_stream()), "trspmm_csr_transpose")
+            n = self.nnz
+            self._t = RoutedSparse(t_rowptr, t_col[:n], t_val[:n], self.m, self.ncols, mode=self.mode,
+                                   dtype=self.dtype, **self.router_kwargs)
+            self._t_perm = perm[:n]
+        return self._t
+
+    def sddmm(self, X: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
+        """dvals[e] = <X[row_e], Y[col_e]> over A's edges (fp32, A's edge order):
+        the gradient of sum(X * (A @ Y)) with respect to A's values."""
+        for name, t, rows in (("X", X, self.m), ("Y", Y, self.ncols)):
+            if not t.is_cuda or t.dtype != self.dtype or t.dim() != 2 or t.shape[0] != rows:
+                raise ValueError(f"{name} must be a [{rows}, K] {self.dtype} device tensor, got "
+                                 f"{tuple(t.shape)} {t.dtype} on {t.device}")
+            if t.stride(1) != 1 or t.stride(0) % 8 or t.data_ptr() % 16:
+                raise ValueError(f"{name} must be row-major with a row stride multiple of 8, 16-byte aligned")
+        if X.shape[1] != Y.shape[1] or X.shape[1] % 8:
+            raise ValueError("X and Y need the same K, a multiple of 8")
+        out = torch.empty(self.nnz, dtype=torch.float32, device=X.device)
+        check(lib().trspmm_sddmm(ptr(self.rowptr), self.rows, ptr(self.col), self.nnz, ptr(X), X.stride(0), ptr(Y),
+                                 Y.stride(0), X.shape[1], dtype_code(self.dtype), ptr(out), _lib.current_stream()),
+              "trspmm_sddmm")
+        return out
