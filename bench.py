@@ -14,6 +14,9 @@ Prints ONE JSON line (rank 0):
              (rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/) when recorded, else null
   cpu_baseline  the oracle's routed multiply (decode + FP64 CSR product) timed on
              the host over a bounded row sample, 1 core
+  fwd_bwd    (default run; --no-extras skips it) the same R-MAT graph at K=256 in
+             fp16 and bf16: plain forward vs forward + dB = A^T·dY (routed plan
+             of A^T, built once) + dvals = SDDMM(dY, B), with errors vs FP64
 """
 from __future__ import annotations
 
@@ -41,6 +44,9 @@ def parse():
     p.add_argument("--dtype", default="fp16", choices=["fp16", "bf16"])
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-check", action="store_true")
+    p.add_argument("--no-extras", action="store_true", help="skip the fwd+bwd and batched measurements")
+    p.add_argument("--backward", action="store_true",
+                   help="measure only forward vs forward+backward on --workload (fp16 and bf16)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="also run a plain 256 MiB device copy (the reference kernel for scaling "
                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -88,6 +94,84 @@ def traffic_record(workload, mode, dtype):
     return None
 
 
+def time_steps(fn, steps, warmup):
+    """Mean ms per call of fn() over `steps` calls after `warmup`, HIP events on
+    the current stream (the stream the library launches on)."""
+    import torch
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def measure_fwd_bwd(csr, K, dtype_name, mode, steps, warmup, check=True):
+    """Forward vs forward + backward (dB = A^T dY on A^T's routed plan, dvals =
+    SDDMM(dY, B)) on one graph; preprocessing (both plans) outside the timing."""
+    import torch
+
+    from trspmm import graphs
+    from trspmm.sparse import RoutedSparse
+    dtype = torch.float16 if dtype_name == "fp16" else torch.bfloat16
+    dev = torch.device("cuda", 0)
+    B = torch.from_numpy(graphs.dense_features(csr.n, K, seed=1001).astype(np.float32)).to(dev).to(dtype)
+    dY = torch.from_numpy(graphs.dense_features(csr.m, K, seed=1003).astype(np.float32)).to(dev).to(dtype)
+    t0 = time.time()
+    A = RoutedSparse(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val), csr.n,
+                     m=csr.m, mode=mode, dtype=dtype)
+    At = A.transpose()
+    torch.cuda.synchronize()
+    prep = time.time() - t0
+    C = torch.empty((csr.m, K), device=dev, dtype=torch.float32)
+    dB = torch.empty((csr.n, K), device=dev, dtype=torch.float32)
+    dv = torch.empty(csr.nnz, device=dev, dtype=torch.float32)
+
+    def fwd():
+        A.matmul(B, C)
+
+    def fwd_bwd():
+        A.matmul(B, C)
+        At.matmul(dY, dB)
+        A.sddmm(dY, B, dv)
+
+    ms_f = time_steps(fwd, steps, warmup)
+    ms_fb = time_steps(fwd_bwd, steps, warmup)
+    ms_db = time_steps(lambda: At.matmul(dY, dB), steps, warmup)
+    ms_sd = time_steps(lambda: A.sddmm(dY, B, dv), steps, warmup)
+    flops = 3 * 2.0 * csr.nnz * K
+    out = {"workload": f"rmat18 K={K}", "dtype": dtype_name, "mode": mode, "ms_fwd": round(ms_f, 5),
+           "ms_fwd_bwd": round(ms_fb, 5), "ratio_fwd_bwd_over_fwd": round(ms_fb / ms_f, 3),
+           "ms_dB": round(ms_db, 5), "ms_sddmm": round(ms_sd, 5),
+           "gflops_fwd_bwd": round(flops / (ms_fb * 1e-3) / 1e9, 1),
+           "roofline_frac_fwd_bwd": round(
+               (algorithmic_bytes(csr.nnz, csr.m, csr.n, K) + algorithmic_bytes(csr.nnz, csr.n, csr.m, K)
+                + csr.nnz * 8 + 4 * (csr.m + 1) + (csr.m + csr.n) * K * 2) / (ms_fb * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "prep_s": round(prep, 3)}
+    if check:
+        import scipy.sparse as sp
+        Af = sp.csr_matrix((csr.val.astype(np.float64), csr.col, csr.rowptr), shape=(csr.m_pad, csr.n))
+        dY64 = np.zeros((csr.m_pad, K))
+        dY64[: csr.m] = dY.float().cpu().numpy()
+        B64 = B.float().cpu().numpy().astype(np.float64)
+        ref = np.asarray(Af.T @ dY64)
+        got = dB.cpu().numpy().astype(np.float64)
+        out["rel_err_dB_vs_fp64"] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        rng = np.random.default_rng(0)
+        sample = rng.choice(csr.nnz, min(csr.nnz, 50000), replace=False)
+        rows = np.searchsorted(csr.rowptr, sample, side="right") - 1
+        ref_v = np.einsum("ef,ef->e", dY64[rows], B64[csr.col[sample]])
+        got_v = dv.cpu().numpy()[sample].astype(np.float64)
+        out["rel_err_dvals_vs_fp64"] = float(np.linalg.norm(got_v - ref_v) / np.linalg.norm(ref_v))
+    del A, At
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -103,6 +187,10 @@ def main():
     t0 = time.time()
     csr = builder()
     gen_s = time.time() - t0
+    if a.backward:
+        for dt in ("fp16", "bf16"):
+            print(json.dumps(measure_fwd_bwd(csr, K, dt, a.mode, a.steps, a.warmup, not a.no_check)), flush=True)
+        return
     dtype = torch.float16 if a.dtype == "fp16" else torch.bfloat16
     B16 = graphs.dense_features(csr.n, K, seed=bseed)                    # fp16 N(0,1)
     dev = torch.device("cuda", 0)
@@ -169,6 +257,11 @@ def main():
     }
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(csr, K, B16)
+    if not a.no_extras and a.workload == "rmat18_k128":
+        del A, C, Bt
+        torch.cuda.empty_cache()
+        print("[bench] fwd+bwd at K=256", file=sys.stderr, flush=True)
+        line["fwd_bwd"] = [measure_fwd_bwd(csr, 256, dt, a.mode, 20, 5, not a.no_check) for dt in ("fp16", "bf16")]
     print(json.dumps(line), flush=True)
 
 

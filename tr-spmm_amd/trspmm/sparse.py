@@ -81,7 +81,7 @@ class RoutedSparse:
             self._t_perm = perm[:n]
         return self._t
 
-    def sddmm(self, X: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
+    def sddmm(self, X: torch.Tensor, Y: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """dvals[e] = <X[row_e], Y[col_e]> over A's edges (fp32, A's edge order):
         the gradient of sum(X * (A @ Y)) with respect to A's values."""
         for name, t, rows in (("X", X, self.m), ("Y", Y, self.ncols)):
@@ -92,7 +92,10 @@ class RoutedSparse:
                 raise ValueError(f"{name} must be row-major with a row stride multiple of 8, 16-byte aligned")
         if X.shape[1] != Y.shape[1] or X.shape[1] % 8:
             raise ValueError("X and Y need the same K, a multiple of 8")
-        out = torch.empty(self.nnz, dtype=torch.float32, device=X.device)
+        if out is None:
+            out = torch.empty(self.nnz, dtype=torch.float32, device=X.device)
+        elif out.dtype != torch.float32 or out.shape != (self.nnz,) or not out.is_contiguous() or not out.is_cuda:
+            raise ValueError(f"out must be a contiguous float32 device tensor of shape ({self.nnz},)")
         check(lib().trspmm_sddmm(ptr(self.rowptr), self.rows, ptr(self.col), self.nnz, ptr(X), X.stride(0), ptr(Y),
                                  Y.stride(0), X.shape[1], dtype_code(self.dtype), ptr(out), _lib.current_stream()),
               "trspmm_sddmm")
