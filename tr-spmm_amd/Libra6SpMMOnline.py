@@ -109,11 +109,6 @@ def _plan_from_view(view_tensors, n_windows, mOri, kOri, keep):
     return Plan(h, int(mOri), int(kOri), torch.float16, keep)
 
 
-def _empty_tc_residual():
-    z1 = torch.zeros(1, dtype=torch.int32, device="cuda")
-    return z1
-
-
 def forward_fp16_tcu_cuda_sptc_mma_parallel_online(
         t_row_offset, t_blockNew_offset, t_column, t_value, t_window_row, t_atomic, t_binary,
         c_row_offsets, c_row, c_atomic, c_column, c_value,
