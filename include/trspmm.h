@@ -23,7 +23,8 @@
  *                            (mGCNkernel_online.cu:1604-1704) and its ms outputs
  * Entry points with no reference counterpart (the reference has no backward,
  * no batching and no device router):
- *   trspmm_csr_transpose, trspmm_sddmm (backward pass), trspmm_batch_* (many
+ *   trspmm_csr_transpose, trspmm_sddmm, trspmm_bwd_plan_*, trspmm_spmm_t_sddmm
+ *   (backward pass), trspmm_batch_* (many
  *   graphs in one launch), trspmm_memcpy, trspmm_host_sort_order (diagnostics).
  */
 #ifndef TRSPMM_H
@@ -123,6 +124,16 @@ typedef struct trspmm_meta_view {
 
 typedef struct trspmm_plan trspmm_plan;
 
+/* Plan options (no reference counterpart; the reference's kernels are fixed).
+ * Zero-initialised = defaults (environment TRSPMM_RESIDUAL / TRSPMM_SLAB, else
+ * the vector-ALU residual engine and 64-feature slabs). */
+enum { TRSPMM_ENGINE_DEFAULT = 0, TRSPMM_ENGINE_VALU = 1, TRSPMM_ENGINE_MFMA = 2 };
+typedef struct trspmm_plan_opts {
+    int32_t residual_engine;  /* TRSPMM_ENGINE_* */
+    int32_t slab;             /* tile-kernel features per pass: 0 (default), 64 or 128 */
+    int32_t reserved[6];
+} trspmm_plan_opts;
+
 /* Builds the device plan (AMD lane layouts) from reference-format metadata.
  * m_out: output rows (mOri); ncols: rows of B (kOri); dtype: TRSPMM_F16/BF16. */
 int trspmm_plan_create(const trspmm_meta_view* view, int64_t m_out, int64_t ncols, int dtype,
@@ -130,6 +141,11 @@ int trspmm_plan_create(const trspmm_meta_view* view, int64_t m_out, int64_t ncol
 /* Same, from the router's own output. */
 int trspmm_plan_from_meta(const trspmm_meta* meta, int64_t m_out, int64_t ncols, int dtype,
                           void* stream, trspmm_plan** out);
+/* The same two, with plan options (opts may be null = defaults). */
+int trspmm_plan_create_ex(const trspmm_meta_view* view, int64_t m_out, int64_t ncols, int dtype,
+                          const trspmm_plan_opts* opts, void* stream, trspmm_plan** out);
+int trspmm_plan_from_meta_ex(const trspmm_meta* meta, int64_t m_out, int64_t ncols, int dtype,
+                             const trspmm_plan_opts* opts, void* stream, trspmm_plan** out);
 /* 2:4 route only, from the mma.sp packed tiles (fp16 values). */
 int trspmm_plan_create_packed(const int32_t* s_tile_column, const int32_t* s_packed_a,
                               const int32_t* s_packed_meta, const int32_t* s_window_tile_offset,
@@ -177,6 +193,23 @@ int trspmm_csr_transpose(const int32_t* d_rowptr, const int32_t* d_col, const fl
  * X and Y 16-byte aligned.  Stream-ordered. */
 int trspmm_sddmm(const int32_t* d_rowptr, int64_t rows, const int32_t* d_col, int64_t nnz, const void* X,
                  int64_t ldx, const void* Y, int64_t ldy, int64_t K, int dtype, float* dvals, void* stream);
+
+/* Fused backward over the rows of A^T: one pass computes both
+ *   dB[N, K]  = A^T · dY          (fp32, ldd: every row written)
+ *   dvals[e]  = <dY[row_e], B[col_e]>   (fp32, A's edge order)
+ * gathering each dY row once for both.  The plan is built once from A^T as
+ * trspmm_csr_transpose returns it (t_rowptr[rows + 1], t_col, t_val, perm;
+ * device memory; rows = ncols of A, ncols = rows of A) and holds its own
+ * copies, the values in `dtype`, and a scratch array of nnz floats: calls on
+ * one plan must not overlap.  K multiple of 8, at most 512; dY [ncols, K]
+ * (lddy) and B [rows, K] (ldb) in `dtype`, 16-byte aligned. */
+typedef struct trspmm_bwd_plan trspmm_bwd_plan;
+int trspmm_bwd_plan_create(const int32_t* t_rowptr, const int32_t* t_col, const float* t_val,
+                           const int32_t* perm, int64_t rows, int64_t nnz, int64_t ncols, int dtype,
+                           void* stream, trspmm_bwd_plan** out);
+void trspmm_bwd_plan_free(trspmm_bwd_plan* plan);
+int trspmm_spmm_t_sddmm(const trspmm_bwd_plan* plan, const void* dY, int64_t lddy, const void* B,
+                        int64_t ldb, int64_t K, float* dB, int64_t lddb, float* dvals, void* stream);
 
 /* ------------------------------------------------------------------ diagnostics */
 

@@ -41,7 +41,18 @@ def _rect(m=700, n=1100, deg=9, seed=21):
     return g.csr_from_pairs(rows, cols, rng.uniform(-1, 1, rows.size), m, n)
 
 
-MATS = dict(CASES, rect=_rect)
+def _star(n=1500, seed=23):
+    """Column 0 in every row (A^T row 0 has n entries: split into atomic chunks),
+    column 1 in every other row, plus a sparse random rest."""
+    rng = np.random.default_rng(seed)
+    rows = [np.arange(n), np.arange(0, n, 2), np.repeat(np.arange(n), 3)]
+    cols = [np.zeros(n, np.int64), np.ones((n + 1) // 2, np.int64), rng.integers(2, n, 3 * n)]
+    rows = np.concatenate(rows)
+    cols = np.concatenate(cols)
+    return g.csr_from_pairs(rows, cols, rng.uniform(-1, 1, rows.size), n, n)
+
+
+MATS = dict(CASES, rect=_rect, star=_star)
 
 
 @pytest.mark.parametrize("case", sorted(MATS))
@@ -106,16 +117,17 @@ def test_sddmm_closed_form(cuda, K, dtype):
 
 @pytest.mark.parametrize("case", sorted(MATS))
 @pytest.mark.parametrize("dtype", [torch.float16, torch.bfloat16])
-def test_backward_closed_form(cuda, case, dtype):
+@pytest.mark.parametrize("path", ["fused", "routed"])
+def test_backward_closed_form(cuda, case, dtype, path):
     csr = MATS[case]()
-    print(f"[backward] {case} {dtype} nnz={csr.nnz}", flush=True)
+    print(f"[backward] {case} {dtype} {path} nnz={csr.nnz}", flush=True)
     K = 64
     A = _routed(csr, dtype)
     gen = torch.Generator().manual_seed(3)
     B = torch.randn(csr.n, K, generator=gen).to(dtype).to(cuda).requires_grad_()
     vals = A.values.clone().requires_grad_()
     dY = torch.randn(csr.m, K, generator=gen)
-    C = tag.spmm(A, B, vals)
+    C = tag.spmm(A, B, vals, backward=path)
     C.backward(dY.to(cuda))
     dYq = _q(dY, dtype)
     Bq = B.detach().cpu().double().numpy()
@@ -219,6 +231,45 @@ def test_backward_finite_differences(cuda, dtype):
         want.append(fd)
     got, want = np.array(got), np.array(want)
     assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-2
+
+
+@pytest.mark.parametrize("K", [8, 24, 64, 128, 200, 256, 264, 512])
+@pytest.mark.parametrize("dtype", [torch.float16, torch.bfloat16])
+@pytest.mark.parametrize("case", ["star", "hub", "ragged", "rect"])
+def test_fused_backward_closed_form(cuda, K, dtype, case):
+    """One pass over A^T: dB = A^T dY (fp32) and dvals = SDDMM(dY, B) against
+    their FP64 closed forms on the operands as the kernel consumes them."""
+    csr = MATS[case]()
+    print(f"[fused] {case} K={K} {dtype} nnz={csr.nnz}", flush=True)
+    A = _routed(csr, dtype)
+    gen = torch.Generator().manual_seed(K + 1)
+    B = torch.randn(csr.n, K, generator=gen).to(dtype)
+    dY = torch.randn(csr.m, K, generator=gen).to(dtype)
+    dB, dv = A.spmm_t_sddmm(dY.to(cuda), B.to(cuda))
+    Aq = sp.csr_matrix((_q(torch.from_numpy(csr.val), dtype), csr.col, csr.rowptr), shape=(csr.m_pad, csr.n))
+    dYp = np.zeros((csr.m_pad, K))
+    dYp[: csr.m] = dY.double().numpy()
+    dB_ref = np.asarray(Aq.T @ dYp)
+    dv_ref = _sddmm_ref(csr, dYp, B.double().numpy())
+    dB = dB.double().cpu().numpy()
+    dv = dv.double().cpu().numpy()
+    assert np.linalg.norm(dB - dB_ref) / np.linalg.norm(dB_ref) < 1e-5
+    assert np.abs(dB - dB_ref).max() / np.abs(dB_ref).max() < 1e-4
+    assert np.linalg.norm(dv - dv_ref) / np.linalg.norm(dv_ref) < 1e-5
+    assert np.abs(dv - dv_ref).max() / np.abs(dv_ref).max() < 1e-4
+
+
+def test_fused_backward_repeatable(cuda):
+    """Two calls on one plan (shared scratch) give the same result."""
+    csr = _star()
+    A = _routed(csr)
+    gen = torch.Generator().manual_seed(5)
+    B = torch.randn(csr.n, 64, generator=gen).half().to(cuda)
+    dY = torch.randn(csr.m, 64, generator=gen).half().to(cuda)
+    dB1, dv1 = A.spmm_t_sddmm(dY, B)
+    dB1, dv1 = dB1.clone(), dv1.clone()
+    dB2, dv2 = A.spmm_t_sddmm(dY, B)
+    assert torch.allclose(dB1, dB2, rtol=1e-6, atol=1e-6) and torch.equal(dv1, dv2)
 
 
 def test_transpose_is_cached_and_routed(cuda):

@@ -14,6 +14,8 @@
 
 #include <rocprim/rocprim.hpp>
 
+#include <vector>
+
 #include "backward.h"
 #include "scan.h"
 
@@ -107,6 +109,39 @@ __device__ inline void rs_step(float (&part)[SDDMM_U], bool hi, int bit) {
     }
 }
 
+// Reduce-scatter of 8 per-edge partial sums over a lane group of LPR lanes
+// (j = lane within the group): at each lane-bit keep the half of the edge set
+// that bit selects and add the partner lane's copy of it; lane-bits left once
+// each lane holds one edge are plain all-reduces.  Afterwards the lane holds
+// the full sums of edges [base, base + NV) in part[0..NV), and the lanes whose
+// bits in LOWMASK are zero are the writers.
+template <int LPR>
+struct EdgeReduce {
+    static constexpr int NV = LPR >= 8 ? 1 : (LPR >= 4 ? 2 : (LPR >= 2 ? 4 : 8));
+    static constexpr int LOWMASK = LPR >= 16 ? LPR / 8 - 1 : 0;
+    __device__ static inline int run(float (&part)[SDDMM_U], int j) {
+        int base = 0;
+        if constexpr (LPR >= 2) {
+            const bool hi = (j & (LPR / 2)) != 0;
+            rs_step<4>(part, hi, LPR / 2);
+            base += hi ? 4 : 0;
+        }
+        if constexpr (LPR >= 4) {
+            const bool hi = (j & (LPR / 4)) != 0;
+            rs_step<2>(part, hi, LPR / 4);
+            base += hi ? 2 : 0;
+        }
+        if constexpr (LPR >= 8) {
+            const bool hi = (j & (LPR / 8)) != 0;
+            rs_step<1>(part, hi, LPR / 8);
+            base += hi ? 1 : 0;
+        }
+#pragma unroll
+        for (int b = LPR / 16; b >= 1; b >>= 1) part[0] += __shfl_xor(part[0], b, 64);
+        return base;
+    }
+};
+
 template <int LPR, typename DT>
 __global__ __launch_bounds__(SDDMM_BLOCK) void k_sddmm(const int32_t* __restrict__ rowptr, int64_t rows,
                                                        const int32_t* __restrict__ col, int64_t nnz,
@@ -151,32 +186,11 @@ __global__ __launch_bounds__(SDDMM_BLOCK) void k_sddmm(const int32_t* __restrict
 #pragma unroll
             for (int u = 0; u < U; ++u) part[u] += dot8<DT>(xv[u], yv[u]);
         }
-        // reduce-scatter over the group's lanes: at each lane-bit keep the half of
-        // the edge set that bit selects and add the partner lane's copy of it
-        int base = 0;                     // first edge of the lane's remaining set
-        if constexpr (LPR >= 2) {
-            const bool hi = (j & (LPR / 2)) != 0;
-            rs_step<4>(part, hi, LPR / 2);
-            base += hi ? 4 : 0;
-        }
-        if constexpr (LPR >= 4) {
-            const bool hi = (j & (LPR / 4)) != 0;
-            rs_step<2>(part, hi, LPR / 4);
-            base += hi ? 2 : 0;
-        }
-        if constexpr (LPR >= 8) {
-            const bool hi = (j & (LPR / 8)) != 0;
-            rs_step<1>(part, hi, LPR / 8);
-            base += hi ? 1 : 0;
-        }
-        // lane-bits left over once each lane holds one edge: plain all-reduce
+        using R = EdgeReduce<LPR>;
+        const int base = R::run(part, j);
+        if ((j & R::LOWMASK) == 0) {
 #pragma unroll
-        for (int b = LPR / 16; b >= 1; b >>= 1) part[0] += __shfl_xor(part[0], b, 64);
-        constexpr int NV = LPR >= 8 ? 1 : (LPR >= 4 ? 2 : (LPR >= 2 ? 4 : 8));   // edges per lane
-        constexpr int LOWMASK = LPR >= 16 ? LPR / 8 - 1 : 0;                      // lanes sharing an edge
-        if ((j & LOWMASK) == 0) {
-#pragma unroll
-            for (int k = 0; k < NV; ++k) {
+            for (int k = 0; k < R::NV; ++k) {
                 const int64_t e = eb + base + k;
                 if (e < g1) out[e] = part[k];
             }
@@ -203,6 +217,187 @@ int launch_sddmm_typed(const int32_t* rowptr, int64_t rows, const int32_t* col, 
         hipLaunchKernelGGL((k_sddmm<4, DT>), grid, dim3(SDDMM_BLOCK), 0, s, rowptr, rows, col, nnz, x, ldx, y, ldy,
                            (int)K, out);
     TRSPMM_HIP(hipGetLastError());
+    return TRSPMM_OK;
+}
+
+
+// ------------------------------------------------------------------ fused A^T backward
+//
+// One pass over the rows of A^T (row c of A^T = column c of A).  For an entry
+// (c, r, v) of A^T the dY row r is gathered once and used twice:
+//     dB[c]    += v * dY[r]           (dB = A^T dY, fp32 accumulation)
+//     dvals_t  =  <dY[r], B[c]>       (the SDDMM value gradient, A^T order)
+// A lane group of LPR lanes owns whole rows: it keeps B[c] (its features,
+// NCH 16-byte chunks per lane) and the dB accumulator in registers, streams
+// the row's entries U = 8 at a time (U * NCH loads per lane in flight) and
+// writes the dB row once.  Work items (built on the host from A^T's row
+// pointers): kind 0 = a run of whole rows [r0, r1) shared round-robin by the
+// wave's groups (plain stores); kind 1 = a chunk [b, e) of one long row split
+// over the groups (fp32 atomic adds into a row zeroed beforehand).
+constexpr int BWD_BLOCK = 256;
+constexpr int BWD_ROW_CAP = 512;        // entries of one row handled as a whole
+constexpr int BWD_ITEM_CAP = 1024;      // entries of a kind-0 item
+constexpr int BWD_ITEM_ROWS = 16;       // rows of a kind-0 item
+
+template <typename DT>
+__device__ inline void fma8(float (&acc)[8], float v, u32x4 x) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        acc[2 * i] = __builtin_fmaf(v, lo16<DT>(x[i]), acc[2 * i]);
+        acc[2 * i + 1] = __builtin_fmaf(v, lo16<DT>(x[i] >> 16), acc[2 * i + 1]);
+    }
+}
+
+template <int LPR, int NCH, typename DT>
+__device__ inline void bwd_row_range(int64_t c, int32_t e0, int32_t e1, bool atomic, int j,
+                                     const int32_t* __restrict__ col, const uint16_t* __restrict__ val,
+                                     const uint16_t* __restrict__ dY, int64_t lddy,
+                                     const uint16_t* __restrict__ B, int64_t ldb, int K,
+                                     float* __restrict__ dB, int64_t lddb, float* __restrict__ dvt) {
+    constexpr int U = SDDMM_U;
+    using R = EdgeReduce<LPR>;
+    u32x4 bc[NCH];
+    float acc[NCH][8];
+#pragma unroll
+    for (int k = 0; k < NCH; ++k) {
+        const int f = 8 * j + 8 * LPR * k;
+        bc[k] = f < K ? *(const u32x4*)(B + c * ldb + f) : u32x4{0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[k][q] = 0.f;
+    }
+    for (int32_t eb = e0; eb < e1; eb += U) {
+        int32_t r[U];
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int32_t e = eb + u < e1 ? eb + u : e1 - 1;     // clamped: unconditional loads
+            r[u] = col[e];
+            v[u] = eb + u < e1 ? lo16<DT>(val[e]) : 0.f;
+        }
+        u32x4 y[U][NCH];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < NCH; ++k) {
+                const int f = 8 * j + 8 * LPR * k;
+                y[u][k] = *(const u32x4*)(dY + (int64_t)r[u] * lddy + (f < K ? f : 0));
+            }
+        float part[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            part[u] = 0.f;
+#pragma unroll
+            for (int k = 0; k < NCH; ++k) {
+                if (8 * j + 8 * LPR * k < K) {
+                    fma8<DT>(acc[k], v[u], y[u][k]);
+                    part[u] += dot8<DT>(y[u][k], bc[k]);
+                }
+            }
+        }
+        const int base = R::run(part, j);
+        if ((j & R::LOWMASK) == 0) {
+#pragma unroll
+            for (int k = 0; k < R::NV; ++k)
+                if (eb + base + k < e1) dvt[eb + base + k] = part[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NCH; ++k) {
+        const int f = 8 * j + 8 * LPR * k;
+        if (f >= K) continue;
+        float* dst = dB + c * lddb + f;
+        if (!atomic) {
+            *(f32x4*)dst = f32x4{acc[k][0], acc[k][1], acc[k][2], acc[k][3]};
+            *(f32x4*)(dst + 4) = f32x4{acc[k][4], acc[k][5], acc[k][6], acc[k][7]};
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (acc[k][q] != 0.f) unsafeAtomicAdd(dst + q, acc[k][q]);
+        }
+    }
+}
+
+template <int LPR, int NCH, typename DT>
+__global__ __launch_bounds__(BWD_BLOCK) void k_bwd_fused(const int4* __restrict__ items, int64_t n_items,
+                                                        const int32_t* __restrict__ rowptr,
+                                                        const int32_t* __restrict__ col,
+                                                        const uint16_t* __restrict__ val,
+                                                        const uint16_t* __restrict__ dY, int64_t lddy,
+                                                        const uint16_t* __restrict__ B, int64_t ldb, int K,
+                                                        float* __restrict__ dB, int64_t lddb,
+                                                        float* __restrict__ dvt) {
+    constexpr int G = 64 / LPR;
+    const int64_t it = (int64_t)blockIdx.x * (BWD_BLOCK / 64) + threadIdx.x / 64;
+    if (it >= n_items) return;                    // whole waves leave; no block barriers below
+    const int lane = threadIdx.x & 63, g = lane / LPR, j = lane % LPR;
+    const int4 item = items[it];
+    if (item.x == 0) {
+        for (int32_t c = item.y + g; c < item.z; c += G)
+            bwd_row_range<LPR, NCH, DT>(c, rowptr[c], rowptr[c + 1], false, j, col, val, dY, lddy, B, ldb, K, dB,
+                                        lddb, dvt);
+    } else {
+        const int32_t n = item.w - item.z, per = (n + G - 1) / G;
+        const int32_t b = item.z + g * per, e = b + per < item.w ? b + per : item.w;
+        if (b < e)
+            bwd_row_range<LPR, NCH, DT>(item.y, b, e, true, j, col, val, dY, lddy, B, ldb, K, dB, lddb, dvt);
+    }
+}
+
+__global__ void k_zero_list(const int32_t* __restrict__ rows, int64_t n, int K, float* __restrict__ dst,
+                            int64_t ld) {
+    const int64_t i = blockIdx.x;
+    if (i >= n) return;
+    for (int f = threadIdx.x; f < K; f += blockDim.x) dst[(int64_t)rows[i] * ld + f] = 0.f;
+}
+
+__global__ void k_invert(const int32_t* __restrict__ perm, int64_t n, int32_t* __restrict__ inv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) inv[perm[i]] = (int32_t)i;
+}
+
+__global__ void k_gather_f32(const float* __restrict__ src, const int32_t* __restrict__ idx, int64_t n,
+                             float* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
+__global__ void k_to16(const float* __restrict__ src, int64_t n, int dtype, uint16_t* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = dtype == TRSPMM_BF16 ? f32_to_bf16_bits(src[i]) : f32_to_f16_rne(src[i]);
+}
+
+template <typename DT>
+int launch_bwd_typed(const BwdPlan& P, const void* dY, int64_t lddy, const void* B, int64_t ldb, int64_t K,
+                     float* dB, int64_t lddb, float* dvals, hipStream_t s) {
+    if (P.n_zero > 0) {
+        hipLaunchKernelGGL(k_zero_list, dim3((unsigned)P.n_zero), dim3(256), 0, s, P.zero_rows.as<int32_t>(),
+                           P.n_zero, (int)K, dB, lddb);
+        TRSPMM_HIP(hipGetLastError());
+    }
+    const dim3 grid((unsigned)((P.n_items + BWD_BLOCK / 64 - 1) / (BWD_BLOCK / 64)));
+    const auto* y = (const uint16_t*)dY;
+    const auto* b = (const uint16_t*)B;
+    const auto* items = P.items.as<int4>();
+    const auto* rp = P.rowptr.as<int32_t>();
+    const auto* cl = P.col.as<int32_t>();
+    const auto* vl = P.val.as<uint16_t>();
+    float* dvt = P.dvals_t.as<float>();
+#define TRSPMM_BWD(LPR, NCH)                                                                                      \
+    hipLaunchKernelGGL((k_bwd_fused<LPR, NCH, DT>), grid, dim3(BWD_BLOCK), 0, s, items, P.n_items, rp, cl, vl, y,  \
+                       lddy, b, ldb, (int)K, dB, lddb, dvt)
+    if (P.n_items > 0) {
+        if (K <= 64) TRSPMM_BWD(8, 1);
+        else if (K <= 128) TRSPMM_BWD(16, 1);
+        else if (K <= 256) TRSPMM_BWD(32, 1);
+        else TRSPMM_BWD(32, 2);
+        TRSPMM_HIP(hipGetLastError());
+    }
+#undef TRSPMM_BWD
+    if (P.nnz > 0) {
+        hipLaunchKernelGGL(k_gather_f32, dim3(nblk(P.nnz)), dim3(256), 0, s, dvt, P.invperm.as<int32_t>(), P.nnz,
+                           dvals);
+        TRSPMM_HIP(hipGetLastError());
+    }
     return TRSPMM_OK;
 }
 
@@ -263,6 +458,78 @@ int launch_sddmm(const int32_t* rowptr, int64_t rows, const int32_t* col, int64_
     TRSPMM_CHECK(rowptr && col && X && Y && out, "sddmm: null pointer");
     if (dtype == TRSPMM_F16) return launch_sddmm_typed<F16>(rowptr, rows, col, nnz, X, ldx, Y, ldy, K, out, s);
     return launch_sddmm_typed<BF16>(rowptr, rows, col, nnz, X, ldx, Y, ldy, K, out, s);
+}
+
+int bwd_plan_build(const int32_t* t_rowptr, const int32_t* t_col, const float* t_val, const int32_t* perm,
+                   int64_t rows, int64_t nnz, int64_t ncols, int dtype, hipStream_t s, BwdPlan& P) {
+    TRSPMM_CHECK(rows >= 0 && nnz >= 0 && ncols >= 0, "bwd plan: negative size");
+    TRSPMM_CHECK(nnz < (int64_t)1 << 31 && rows < (int64_t)1 << 31, "bwd plan: sizes must fit int32");
+    TRSPMM_CHECK(dtype == TRSPMM_F16 || dtype == TRSPMM_BF16, "bwd plan: dtype must be TRSPMM_F16 or TRSPMM_BF16");
+    TRSPMM_CHECK(t_rowptr && (nnz == 0 || (t_col && t_val && perm)), "bwd plan: null pointer");
+    P.rows = rows;
+    P.nnz = nnz;
+    P.ncols = ncols;
+    P.dtype = dtype;
+    std::vector<int32_t> rp((size_t)rows + 1);
+    TRSPMM_HIP(hipMemcpyAsync(rp.data(), t_rowptr, 4 * (rows + 1), hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipStreamSynchronize(s));
+    TRSPMM_CHECK(rp[0] == 0 && rp[rows] == nnz, "bwd plan: rowptr must run from 0 to nnz");
+    std::vector<int4> items;
+    std::vector<int32_t> zero;
+    int64_t r = 0;
+    while (r < rows) {
+        const int32_t len = rp[r + 1] - rp[r];
+        TRSPMM_CHECK(len >= 0, "bwd plan: rowptr must be non-decreasing");
+        if (len > BWD_ROW_CAP) {                        // long row: chunks with atomic adds
+            zero.push_back((int32_t)r);
+            for (int32_t b = rp[r]; b < rp[r + 1]; b += BWD_ROW_CAP)
+                items.push_back(int4{1, (int32_t)r, b, b + BWD_ROW_CAP < rp[r + 1] ? b + BWD_ROW_CAP : rp[r + 1]});
+            ++r;
+            continue;
+        }
+        int64_t r1 = r + 1;                             // a run of whole rows
+        while (r1 < rows && r1 - r < BWD_ITEM_ROWS && rp[r1 + 1] - rp[r1] <= BWD_ROW_CAP &&
+               rp[r1 + 1] - rp[r] <= BWD_ITEM_CAP)
+            ++r1;
+        items.push_back(int4{0, (int32_t)r, (int32_t)r1, 0});
+        r = r1;
+    }
+    P.n_items = (int64_t)items.size();
+    P.n_zero = (int64_t)zero.size();
+    int rc;
+    const int64_t n1 = nnz > 0 ? nnz : 1;
+    if ((rc = P.rowptr.alloc(4 * (rows + 1))) || (rc = P.col.alloc(4 * n1)) || (rc = P.val.alloc(2 * n1)) ||
+        (rc = P.invperm.alloc(4 * n1)) || (rc = P.dvals_t.alloc(4 * n1)) ||
+        (rc = P.items.alloc(sizeof(int4) * (items.size() ? items.size() : 1))) ||
+        (rc = P.zero_rows.alloc(4 * (zero.size() ? zero.size() : 1))))
+        return rc;
+    TRSPMM_HIP(hipMemcpyAsync(P.rowptr.p, t_rowptr, 4 * (rows + 1), hipMemcpyDeviceToDevice, s));
+    if (!items.empty())
+        TRSPMM_HIP(hipMemcpyAsync(P.items.p, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice, s));
+    if (!zero.empty())
+        TRSPMM_HIP(hipMemcpyAsync(P.zero_rows.p, zero.data(), 4 * zero.size(), hipMemcpyHostToDevice, s));
+    if (nnz > 0) {
+        TRSPMM_HIP(hipMemcpyAsync(P.col.p, t_col, 4 * nnz, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_to16, dim3(nblk(nnz)), dim3(256), 0, s, t_val, nnz, dtype, P.val.as<uint16_t>());
+        TRSPMM_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_invert, dim3(nblk(nnz)), dim3(256), 0, s, perm, nnz, P.invperm.as<int32_t>());
+        TRSPMM_HIP(hipGetLastError());
+    }
+    TRSPMM_HIP(hipStreamSynchronize(s));               // host vectors go out of scope
+    return TRSPMM_OK;
+}
+
+int launch_bwd(const BwdPlan& P, const void* dY, int64_t lddy, const void* B, int64_t ldb, int64_t K, float* dB,
+               int64_t lddb, float* dvals, hipStream_t s) {
+    TRSPMM_CHECK(K > 0 && K % 8 == 0 && K <= 512, "fused backward: K must be a multiple of 8 in [8, 512]");
+    TRSPMM_CHECK(lddy % 8 == 0 && ldb % 8 == 0 && lddy >= K && ldb >= K && lddb % 4 == 0 && lddb >= K,
+                 "fused backward: row strides must be >= K, dY/B strides multiples of 8, dB stride of 4");
+    TRSPMM_CHECK(((uintptr_t)dY % 16) == 0 && ((uintptr_t)B % 16) == 0 && ((uintptr_t)dB % 16) == 0,
+                 "fused backward: dY, B and dB must be 16-byte aligned");
+    TRSPMM_CHECK(P.rows == 0 || dB, "fused backward: dB must not be null");
+    TRSPMM_CHECK(P.nnz == 0 || (dY && B && dvals), "fused backward: null pointer");
+    if (P.dtype == TRSPMM_F16) return launch_bwd_typed<F16>(P, dY, lddy, B, ldb, K, dB, lddb, dvals, s);
+    return launch_bwd_typed<BF16>(P, dY, lddy, B, ldb, K, dB, lddb, dvals, s);
 }
 
 }  // namespace trspmm

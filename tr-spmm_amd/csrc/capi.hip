@@ -77,6 +77,11 @@ void trspmm_meta_free(trspmm_meta* meta) { delete meta; }
 // ---------------------------------------------------------------- plans
 int trspmm_plan_create(const trspmm_meta_view* view, int64_t m_out, int64_t ncols, int dtype, void* stream,
                        trspmm_plan** out) {
+    return trspmm_plan_create_ex(view, m_out, ncols, dtype, nullptr, stream, out);
+}
+
+int trspmm_plan_create_ex(const trspmm_meta_view* view, int64_t m_out, int64_t ncols, int dtype,
+                          const trspmm_plan_opts* opts, void* stream, trspmm_plan** out) {
     TRSPMM_NOTNULL(view, "view");
     TRSPMM_NOTNULL(out, "out");
     *out = nullptr;
@@ -85,6 +90,7 @@ int trspmm_plan_create(const trspmm_meta_view* view, int64_t m_out, int64_t ncol
         set_error("out of host memory");
         return TRSPMM_ENOMEM;
     }
+    if (opts) p->plan.opts = *opts;
     const int rc = build_plan(*view, m_out, ncols, dtype, (hipStream_t)stream, p->plan);
     if (rc != TRSPMM_OK) {
         delete p;
@@ -96,6 +102,11 @@ int trspmm_plan_create(const trspmm_meta_view* view, int64_t m_out, int64_t ncol
 
 int trspmm_plan_from_meta(const trspmm_meta* meta, int64_t m_out, int64_t ncols, int dtype, void* stream,
                           trspmm_plan** out) {
+    return trspmm_plan_from_meta_ex(meta, m_out, ncols, dtype, nullptr, stream, out);
+}
+
+int trspmm_plan_from_meta_ex(const trspmm_meta* meta, int64_t m_out, int64_t ncols, int dtype,
+                             const trspmm_plan_opts* opts, void* stream, trspmm_plan** out) {
     TRSPMM_NOTNULL(meta, "meta");
     const Meta& M = meta->meta;
     trspmm_meta_view v{};
@@ -129,7 +140,7 @@ int trspmm_plan_from_meta(const trspmm_meta* meta, int64_t m_out, int64_t ncols,
     v.t_value_is_f32 = 1;
     v.c_value_is_f32 = 1;
     v.window = M.window;
-    return trspmm_plan_create(&v, m_out, ncols, dtype, stream, out);
+    return trspmm_plan_create_ex(&v, m_out, ncols, dtype, opts, stream, out);
 }
 
 int trspmm_plan_create_packed(const int32_t* s_tile_column, const int32_t* s_packed_a, const int32_t* s_packed_meta,
@@ -226,6 +237,32 @@ int trspmm_csr_transpose(const int32_t* d_rowptr, const int32_t* d_col, const fl
 int trspmm_sddmm(const int32_t* d_rowptr, int64_t rows, const int32_t* d_col, int64_t nnz, const void* X,
                  int64_t ldx, const void* Y, int64_t ldy, int64_t K, int dtype, float* dvals, void* stream) {
     return launch_sddmm(d_rowptr, rows, d_col, nnz, X, ldx, Y, ldy, K, dtype, dvals, (hipStream_t)stream);
+}
+
+int trspmm_bwd_plan_create(const int32_t* t_rowptr, const int32_t* t_col, const float* t_val, const int32_t* perm,
+                           int64_t rows, int64_t nnz, int64_t ncols, int dtype, void* stream, trspmm_bwd_plan** out) {
+    TRSPMM_NOTNULL(out, "out");
+    *out = nullptr;
+    trspmm_bwd_plan* p = new (std::nothrow) trspmm_bwd_plan();
+    if (!p) {
+        set_error("out of host memory");
+        return TRSPMM_ENOMEM;
+    }
+    const int rc = bwd_plan_build(t_rowptr, t_col, t_val, perm, rows, nnz, ncols, dtype, (hipStream_t)stream, p->plan);
+    if (rc != TRSPMM_OK) {
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return TRSPMM_OK;
+}
+
+void trspmm_bwd_plan_free(trspmm_bwd_plan* plan) { delete plan; }
+
+int trspmm_spmm_t_sddmm(const trspmm_bwd_plan* plan, const void* dY, int64_t lddy, const void* B, int64_t ldb,
+                        int64_t K, float* dB, int64_t lddb, float* dvals, void* stream) {
+    TRSPMM_NOTNULL(plan, "plan");
+    return launch_bwd(plan->plan, dY, lddy, B, ldb, K, dB, lddb, dvals, (hipStream_t)stream);
 }
 
 int trspmm_host_sort_order(const int32_t* key, int32_t n, int32_t* order) {

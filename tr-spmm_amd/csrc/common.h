@@ -179,6 +179,8 @@ struct PlanDev {
     // residual engine: 1 = matrix cores inside k_tile_routes (one-hot A fragments),
     // 0 = the vector-ALU CSR kernel k_residual
     int32_t res_mfma = 0;
+    // features per pass of the tile kernel (64 or 128)
+    int32_t slab = 64;
     const int4* titems = nullptr;
     int64_t n_titems = 0;
     const int4* ritems = nullptr;

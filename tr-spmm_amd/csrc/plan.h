@@ -12,6 +12,7 @@ struct Plan {
     DevBuf res_rowptr, res_cols, res_vals, res_rec;
     DevBuf titems, ritems, split_windows;
     SplitCaps caps;
+    trspmm_plan_opts opts{};   // requested engine / slab (0 = default)
     int finalize();
     int build_items(hipStream_t s);   // work-item list from the per-window ranges
 };

@@ -363,16 +363,30 @@ static int build_residual(const trspmm_meta_view& v, int64_t ncols, int dtype, h
 }
 
 // residual engine of a new plan: the vector-ALU CSR kernel (default; measured
-// faster on the R-MAT workloads) or, with TRSPMM_RESIDUAL=mfma in the
-// environment when the plan is built, one-hot fragments on the matrix cores
-static int residual_on_matrix_cores() {
+// faster on the R-MAT workloads) or one-hot fragments on the matrix cores,
+// chosen by the plan options, else by TRSPMM_RESIDUAL=valu|mfma in the
+// environment when the plan is built
+static int residual_on_matrix_cores(const trspmm_plan_opts& o) {
+    if (o.residual_engine == TRSPMM_ENGINE_VALU) return 0;
+    if (o.residual_engine == TRSPMM_ENGINE_MFMA) return 1;
     const char* e = std::getenv("TRSPMM_RESIDUAL");
     return (e && std::strcmp(e, "mfma") == 0) ? 1 : 0;
 }
 
+// feature slab of the tile kernel: the plan options, else TRSPMM_SLAB=64|128, else 64
+static int tile_slab(const trspmm_plan_opts& o) {
+    if (o.slab == 64 || o.slab == 128) return o.slab;
+    const char* e = std::getenv("TRSPMM_SLAB");
+    return (e && std::atoi(e) == 128) ? 128 : 64;
+}
+
 int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtype, hipStream_t s, Plan& P) {
     TRSPMM_CHECK(v.window == 16, "the routed multiply requires window == 16 (16-row MFMA tiles)");
-    P.dev.res_mfma = residual_on_matrix_cores();
+    TRSPMM_CHECK(P.opts.residual_engine >= 0 && P.opts.residual_engine <= TRSPMM_ENGINE_MFMA,
+                 "plan options: residual_engine must be TRSPMM_ENGINE_DEFAULT, _VALU or _MFMA");
+    TRSPMM_CHECK(P.opts.slab == 0 || P.opts.slab == 64 || P.opts.slab == 128, "plan options: slab must be 0, 64 or 128");
+    P.dev.res_mfma = residual_on_matrix_cores(P.opts);
+    P.dev.slab = tile_slab(P.opts);
     TRSPMM_CHECK(dtype == TRSPMM_F16 || dtype == TRSPMM_BF16, "dtype must be TRSPMM_F16 or TRSPMM_BF16");
     TRSPMM_CHECK(m_out > 0 && ncols > 0, "m_out and ncols must be positive");
     TRSPMM_CHECK(v.n_windows >= 0 && v.n_windows * 16 >= m_out, "windows must cover the output rows");

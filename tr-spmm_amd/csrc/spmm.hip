@@ -444,15 +444,6 @@ __global__ __launch_bounds__(256) void k_zero_rows(const int32_t* __restrict__ s
     }
 }
 
-// feature slab of the tile kernel: TRSPMM_SLAB=64|128 overrides the default 64
-int tile_slab() {
-    static const int forced = [] {
-        const char* e = std::getenv("TRSPMM_SLAB");
-        return e ? std::atoi(e) : 0;
-    }();
-    return forced == 128 ? 128 : 64;
-}
-
 template <typename DT>
 int launch_typed(const PlanDev& p, const void* B, int64_t ldb, int64_t K, float* C, int64_t ldc, uint32_t mask,
                  hipStream_t s) {
@@ -464,7 +455,7 @@ int launch_typed(const PlanDev& p, const void* B, int64_t ldb, int64_t K, float*
     if (p.n_titems > 0) {
         const int64_t grid = xcd_grid(p.n_titems);
         TRSPMM_CHECK(grid <= 0x7fffffff, "too many work items");
-        if (tile_slab() == 128)
+        if (p.slab == 128)
             hipLaunchKernelGGL((k_tile_routes<128, DT>), dim3((unsigned)grid), dim3(64), 0, s, p, (const uint16_t*)B,
                                ldb, (int)K, C, ldc, mask, grid);
         else

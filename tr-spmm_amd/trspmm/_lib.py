@@ -23,8 +23,11 @@ EXPORTS = [
     "trspmm_route", "trspmm_meta_count", "trspmm_meta_data", "trspmm_meta_seconds", "trspmm_meta_free",
     "trspmm_plan_create", "trspmm_plan_from_meta", "trspmm_plan_create_packed", "trspmm_plan_free",
     "trspmm_plan_stats", "trspmm_spmm", "trspmm_spmm_timed", "trspmm_memcpy", "trspmm_host_sort_order",
-    "trspmm_csr_transpose", "trspmm_sddmm",
+    "trspmm_csr_transpose", "trspmm_sddmm", "trspmm_bwd_plan_create", "trspmm_bwd_plan_free",
+    "trspmm_spmm_t_sddmm", "trspmm_plan_create_ex", "trspmm_plan_from_meta_ex",
 ]
+
+ENGINE_DEFAULT, ENGINE_VALU, ENGINE_MFMA = 0, 1, 2
 
 
 class RouteCfg(ctypes.Structure):
@@ -65,6 +68,10 @@ class MetaView(ctypes.Structure):
     ]
 
 
+class PlanOpts(ctypes.Structure):
+    _fields_ = [("residual_engine", ctypes.c_int32), ("slab", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
+
+
 class TrspmmError(RuntimeError):
     """A libtrspmm call returned an error status."""
 
@@ -94,6 +101,10 @@ def lib():
     L.trspmm_plan_from_meta.argtypes = [_P, _I64, _I64, ctypes.c_int, _P, ctypes.POINTER(_P)]
     L.trspmm_plan_create_packed.argtypes = [_P, _P, _P, _P, _I64, _I64, ctypes.c_int32, _I64, _I64, _P,
                                             ctypes.POINTER(_P)]
+    L.trspmm_plan_create_ex.argtypes = [ctypes.POINTER(MetaView), _I64, _I64, ctypes.c_int, ctypes.POINTER(PlanOpts),
+                                        _P, ctypes.POINTER(_P)]
+    L.trspmm_plan_from_meta_ex.argtypes = [_P, _I64, _I64, ctypes.c_int, ctypes.POINTER(PlanOpts), _P,
+                                           ctypes.POINTER(_P)]
     L.trspmm_plan_free.argtypes = [_P]
     L.trspmm_plan_stats.argtypes = [_P, ctypes.POINTER(_I64)]
     L.trspmm_spmm.argtypes = [_P, _P, _I64, _I64, _P, _I64, ctypes.c_uint32, _P]
@@ -103,6 +114,9 @@ def lib():
     L.trspmm_host_sort_order.argtypes = [_P, ctypes.c_int32, _P]
     L.trspmm_csr_transpose.argtypes = [_P, _P, _P, _I64, _I64, _I64, _P, _P, _P, _P, _P]
     L.trspmm_sddmm.argtypes = [_P, _I64, _P, _I64, _P, _I64, _P, _I64, _I64, ctypes.c_int, _P, _P]
+    L.trspmm_bwd_plan_create.argtypes = [_P, _P, _P, _P, _I64, _I64, _I64, ctypes.c_int, _P, ctypes.POINTER(_P)]
+    L.trspmm_bwd_plan_free.argtypes = [_P]
+    L.trspmm_spmm_t_sddmm.argtypes = [_P, _P, _I64, _P, _I64, _I64, _P, _I64, _P, _P]
     _LIB = L
     return L
 

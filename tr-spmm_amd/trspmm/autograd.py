@@ -7,8 +7,11 @@
     C.backward(dY)
 
 Backward (dY = dL/dC, cast to the operand type of the routed kernels):
-    dB    = A^T · dY       the routed multiply on A^T's cached plan
-    dvals = SDDMM(dY, B)   dvals[e] = <dY[row_e], B[col_e]> over A's pattern
+    dB    = A^T · dY       dvals = SDDMM(dY, B), dvals[e] = <dY[row_e], B[col_e]>
+When both are needed (and K <= 512) one fused pass over A^T's rows computes
+them together, gathering each dY row once ("fused", the default); otherwise,
+or with backward="routed", dB runs on the routed kernels over A^T's cached
+plan and dvals on the SDDMM kernel.
 
 The forward multiplies with the plan built from A's values: `vals` carries the
 gradient and must hold those values (the plan is not rebuilt per call).
@@ -22,8 +25,11 @@ from .sparse import RoutedSparse
 
 class RoutedSpMM(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, A: RoutedSparse, values: torch.Tensor, B: torch.Tensor):
+    def forward(ctx, A: RoutedSparse, values: torch.Tensor, B: torch.Tensor, backward: str = "fused"):
+        if backward not in ("fused", "routed"):
+            raise ValueError("backward must be 'fused' or 'routed'")
         ctx.A = A
+        ctx.mode = backward
         ctx.save_for_backward(B)
         return A.matmul(B)
 
@@ -33,18 +39,23 @@ class RoutedSpMM(torch.autograd.Function):
         (B,) = ctx.saved_tensors
         dY = dC.to(A.dtype).contiguous()
         d_values = d_B = None
-        if ctx.needs_input_grad[2]:
+        want_v, want_b = ctx.needs_input_grad[1], ctx.needs_input_grad[2]
+        if want_v and want_b and ctx.mode == "fused" and dY.shape[1] <= 512:
+            d_B, d_values = A.spmm_t_sddmm(dY, B)
+            return None, d_values, d_B.to(B.dtype), None
+        if want_b:
             d_B = A.transpose().matmul(dY).to(B.dtype)
-        if ctx.needs_input_grad[1]:
+        if want_v:
             d_values = A.sddmm(dY, B)
-        return None, d_values, d_B
+        return None, d_values, d_B, None
 
 
-def spmm(A: RoutedSparse, B: torch.Tensor, values: torch.Tensor | None = None) -> torch.Tensor:
+def spmm(A: RoutedSparse, B: torch.Tensor, values: torch.Tensor | None = None,
+         backward: str = "fused") -> torch.Tensor:
     """Differentiable C = A · B (fp32).  `values` ([nnz], A's edge order) receives
     the value gradient; by default A's own value tensor is used."""
     if values is None:
         values = A.values
     if values.shape != (A.nnz,):
         raise ValueError(f"values must have shape ({A.nnz},), got {tuple(values.shape)}")
-    return RoutedSpMM.apply(A, values, B)
+    return RoutedSpMM.apply(A, values, B, backward)

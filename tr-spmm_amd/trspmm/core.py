@@ -163,10 +163,16 @@ class Plan:
         return self._h
 
     @staticmethod
-    def from_meta(meta: RoutedMeta, m_out: int, ncols: int, dtype=torch.float16) -> "Plan":
+    def from_meta(meta: RoutedMeta, m_out: int, ncols: int, dtype=torch.float16, engine: str | None = None,
+                  slab: int | None = None) -> "Plan":
+        """engine: None (default), "valu" or "mfma" residual engine; slab: None, 64 or 128
+        features per pass of the tile kernel."""
+        opts = _lib.PlanOpts()
+        opts.residual_engine = {None: _lib.ENGINE_DEFAULT, "valu": _lib.ENGINE_VALU, "mfma": _lib.ENGINE_MFMA}[engine]
+        opts.slab = int(slab or 0)
         h = ctypes.c_void_p()
-        check(lib().trspmm_plan_from_meta(meta.handle, int(m_out), int(ncols), dtype_code(dtype),
-                                          _lib.current_stream(), ctypes.byref(h)), "trspmm_plan_from_meta")
+        check(lib().trspmm_plan_from_meta_ex(meta.handle, int(m_out), int(ncols), dtype_code(dtype), ctypes.byref(opts),
+                                             _lib.current_stream(), ctypes.byref(h)), "trspmm_plan_from_meta_ex")
         return Plan(h, m_out, ncols, dtype)
 
     def matmul(self, B: torch.Tensor, out: torch.Tensor | None = None, route_mask: int = _lib.ROUTE_ALL):

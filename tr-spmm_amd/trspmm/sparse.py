@@ -20,9 +20,25 @@ from ._lib import check, lib, ptr
 from .core import Plan, dtype_code, route
 
 
+class _BwdPlan:
+    """Owns a trspmm_bwd_plan handle."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    def __del__(self):
+        h, self.handle = getattr(self, "handle", None), None
+        if h:
+            try:
+                lib().trspmm_bwd_plan_free(h)
+            except Exception:
+                pass
+
+
 class RoutedSparse:
     def __init__(self, rowptr, col, val, ncols: int, m: int | None = None, *, mode: str = "exact",
-                 dtype: torch.dtype = torch.float16, **router_kwargs):
+                 dtype: torch.dtype = torch.float16, engine: str | None = None, slab: int | None = None,
+                 **router_kwargs):
         rowptr = torch.as_tensor(rowptr)
         col = torch.as_tensor(col)
         val = torch.as_tensor(val)
@@ -37,7 +53,8 @@ class RoutedSparse:
         self.router_kwargs = dict(router_kwargs)
         self.window = int(router_kwargs.get("window", 16))
         self.meta = route(rowptr, col, val, self.ncols, mode=mode, **router_kwargs)
-        self.plan = Plan.from_meta(self.meta, self.m, self.ncols, dtype)
+        self.engine, self.slab = engine, slab
+        self.plan = Plan.from_meta(self.meta, self.m, self.ncols, dtype, engine=engine, slab=slab)
         # device CSR (validated by the router): the SDDMM pattern and the
         # source of the transposed plan
         dev = torch.device("cuda", torch.cuda.current_device())
@@ -45,7 +62,8 @@ class RoutedSparse:
         self.col = col.to(dev, torch.int32).contiguous()
         self.values = val.to(dev, torch.float32).contiguous()
         self._t = None
-        self._t_perm = None
+        self._tcsr = None
+        self._bwd = None
 
     @property
     def stats(self):
@@ -62,24 +80,69 @@ class RoutedSparse:
         return self.plan.timed(B, out, routes, warmup, iters)
 
     # ------------------------------------------------------------ backward
-    def transpose(self) -> "RoutedSparse":
-        """A^T (ncols x m) as a routed matrix with the same routing mode and
-        operand type, built on the device on first use and cached."""
-        if self._t is None:
+    def _transposed_csr(self):
+        """A^T as a canonical device CSR (t_rowptr, t_col, t_val) plus perm (A's
+        edge of each A^T entry), computed on the device once."""
+        if self._tcsr is None:
             if self.m < self.rows and int(self.rowptr[self.m].item()) != self.nnz:
                 raise ValueError("rows beyond m hold nonzeros: A^T would reference rows outside dY")
-            t_rowptr = torch.empty(self.ncols + 1, dtype=torch.int32, device=self.rowptr.device)
-            t_col = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=self.rowptr.device)
-            t_val = torch.empty(max(self.nnz, 1), dtype=torch.float32, device=self.rowptr.device)
-            perm = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=self.rowptr.device)
+            dev = self.rowptr.device
+            n1 = max(self.nnz, 1)
+            t_rowptr = torch.empty(self.ncols + 1, dtype=torch.int32, device=dev)
+            t_col = torch.empty(n1, dtype=torch.int32, device=dev)
+            t_val = torch.empty(n1, dtype=torch.float32, device=dev)
+            perm = torch.empty(n1, dtype=torch.int32, device=dev)
             check(lib().trspmm_csr_transpose(ptr(self.rowptr), ptr(self.col), ptr(self.values), self.rows, self.nnz,
                                              self.ncols, ptr(t_rowptr), ptr(t_col), ptr(t_val), ptr(perm),
                                              _lib.current_stream()), "trspmm_csr_transpose")
             n = self.nnz
-            self._t = RoutedSparse(t_rowptr, t_col[:n], t_val[:n], self.m, self.ncols, mode=self.mode,
-                                   dtype=self.dtype, **self.router_kwargs)
-            self._t_perm = perm[:n]
+            self._tcsr = (t_rowptr, t_col[:n], t_val[:n], perm[:n])
+        return self._tcsr
+
+    @property
+    def _t_perm(self):
+        return self._transposed_csr()[3]
+
+    def transpose(self) -> "RoutedSparse":
+        """A^T (ncols x m) as a routed matrix with the same routing mode and
+        operand type, built on the device on first use and cached."""
+        if self._t is None:
+            t_rowptr, t_col, t_val, _ = self._transposed_csr()
+            self._t = RoutedSparse(t_rowptr, t_col, t_val, self.m, self.ncols, mode=self.mode, dtype=self.dtype,
+                                   **self.router_kwargs)
         return self._t
+
+    def _backward_plan(self):
+        if self._bwd is None:
+            t_rowptr, t_col, t_val, perm = self._transposed_csr()
+            h = ctypes.c_void_p()
+            check(lib().trspmm_bwd_plan_create(ptr(t_rowptr), ptr(t_col), ptr(t_val), ptr(perm), self.ncols, self.nnz,
+                                               self.m, dtype_code(self.dtype), _lib.current_stream(),
+                                               ctypes.byref(h)), "trspmm_bwd_plan_create")
+            self._bwd = _BwdPlan(h)
+        return self._bwd
+
+    def spmm_t_sddmm(self, dY: torch.Tensor, B: torch.Tensor, dB: torch.Tensor | None = None,
+                     dvals: torch.Tensor | None = None):
+        """Fused backward: (dB = A^T · dY  [ncols, K] fp32, dvals = SDDMM(dY, B)  [nnz]
+        fp32 in A's edge order) in one pass over A^T's rows (K <= 512)."""
+        for name, t, rows in (("dY", dY, self.m), ("B", B, self.ncols)):
+            if not t.is_cuda or t.dtype != self.dtype or t.dim() != 2 or t.shape[0] != rows:
+                raise ValueError(f"{name} must be a [{rows}, K] {self.dtype} device tensor, got "
+                                 f"{tuple(t.shape)} {t.dtype} on {t.device}")
+            if t.stride(1) != 1 or t.stride(0) % 8 or t.data_ptr() % 16:
+                raise ValueError(f"{name} must be row-major with a row stride multiple of 8, 16-byte aligned")
+        K = dY.shape[1]
+        if B.shape[1] != K:
+            raise ValueError("dY and B need the same K")
+        if dB is None:
+            dB = torch.empty((self.ncols, K), dtype=torch.float32, device=dY.device)
+        if dvals is None:
+            dvals = torch.empty(self.nnz, dtype=torch.float32, device=dY.device)
+        plan = self._backward_plan()
+        check(lib().trspmm_spmm_t_sddmm(plan.handle, ptr(dY), dY.stride(0), ptr(B), B.stride(0), K, ptr(dB),
+                                        dB.stride(0), ptr(dvals), _lib.current_stream()), "trspmm_spmm_t_sddmm")
+        return dB, dvals
 
     def sddmm(self, X: torch.Tensor, Y: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """dvals[e] = <X[row_e], Y[col_e]> over A's edges (fp32, A's edge order):

@@ -1,0 +1,63 @@
+"""Autotuner: picks routing thresholds and kernel variants for one matrix by
+measuring them on the device.
+
+The reference fixes its choices at build time (TC width 8, density 8, one
+2:4 kernel).  Here the candidates are
+  router    mode exact / tuned with a 2:4 occupancy floor (sptc_min_nnz),
+            TC density threshold (columns with >= density rows of a window
+            become dense 16x8 sub-blocks)
+  kernels   residual engine (vector ALU / matrix cores), tile-kernel
+            features per pass (64 / 128)
+The window height stays 16: the MFMA/SMFMAC tiles are 16 rows tall.  Each
+router configuration is routed once and every kernel variant gets its own
+plan; each candidate is timed with HIP events over `iters` launches after a
+warm-up.  The result names the fastest candidate and keeps the whole table.
+"""
+from __future__ import annotations
+
+import itertools
+from dataclasses import dataclass, field
+
+import torch
+
+from .core import Plan, route
+
+ROUTER_CANDIDATES = [
+    {"mode": "exact", "density": 8},
+    {"mode": "tuned", "density": 8, "sptc_min_nnz": 4},
+    {"mode": "tuned", "density": 8, "sptc_min_nnz": 8},
+    {"mode": "tuned", "density": 8, "sptc_min_nnz": 16},
+    {"mode": "exact", "density": 6},
+]
+KERNEL_CANDIDATES = [{"engine": e, "slab": s} for e, s in itertools.product(["valu", "mfma"], [64, 128])]
+
+
+@dataclass
+class TuneResult:
+    best: dict
+    ms: float
+    table: list = field(default_factory=list)     # (candidate, ms)
+
+    def router_kwargs(self) -> dict:
+        return {k: v for k, v in self.best.items() if k not in ("engine", "slab")}
+
+
+def autotune(rowptr, col, val, ncols: int, m: int, B: torch.Tensor, *, dtype=torch.float16,
+             router_candidates=None, kernel_candidates=None, warmup: int = 3, iters: int = 10) -> TuneResult:
+    """Times every (router, kernel) candidate on C = A·B and returns the fastest."""
+    router_candidates = ROUTER_CANDIDATES if router_candidates is None else router_candidates
+    kernel_candidates = KERNEL_CANDIDATES if kernel_candidates is None else kernel_candidates
+    C = torch.empty((m, B.shape[1]), device=B.device, dtype=torch.float32)
+    table = []
+    for rc in router_candidates:
+        kw = dict(rc)
+        mode = kw.pop("mode")
+        meta = route(rowptr, col, val, ncols, mode=mode, **kw)
+        for kc in kernel_candidates:
+            plan = Plan.from_meta(meta, m, ncols, dtype, engine=kc["engine"], slab=kc["slab"])
+            ms = plan.timed(B, C, warmup=warmup, iters=iters)
+            table.append(({**rc, **kc}, ms))
+            del plan
+        del meta
+    best, ms = min(table, key=lambda t: t[1])
+    return TuneResult(best, ms, table)
