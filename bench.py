@@ -15,8 +15,12 @@ Prints ONE JSON line (rank 0):
   cpu_baseline  the oracle's routed multiply (decode + FP64 CSR product) timed on
              the host over a bounded row sample, 1 core
   fwd_bwd    (default run; --no-extras skips it) the same R-MAT graph at K=256 in
-             fp16 and bf16: plain forward vs forward + dB = A^T·dY (routed plan
-             of A^T, built once) + dvals = SDDMM(dY, B), with errors vs FP64
+             fp16 and bf16: plain forward vs forward + backward (dB = A^T·dY and
+             dvals = SDDMM(dY, B)), with errors vs FP64
+  batched    (default run; --no-extras skips it) 1024 Barabasi-Albert graphs
+             (m=3, n_i log-uniform in [20, 5000]), K=128: one autotuned launch
+             over the concatenated tile queue (eager and replayed from a
+             captured hipGraph) vs 1024 sequential per-graph launches
 """
 from __future__ import annotations
 
@@ -45,6 +49,7 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-check", action="store_true")
     p.add_argument("--no-extras", action="store_true", help="skip the fwd+bwd and batched measurements")
+    p.add_argument("--batched", action="store_true", help="measure only the batched workload")
     p.add_argument("--backward", action="store_true",
                    help="measure only forward vs forward+backward on --workload (fp16 and bf16)")
     p.add_argument("--copy-calibration", action="store_true",
@@ -135,19 +140,30 @@ def measure_fwd_bwd(csr, K, dtype_name, mode, steps, warmup, check=True):
     def fwd():
         A.matmul(B, C)
 
-    def fwd_bwd():
+    def fwd_bwd_routed():
         A.matmul(B, C)
         At.matmul(dY, dB)
         A.sddmm(dY, B, dv)
 
+    def fwd_bwd():
+        A.matmul(B, C)
+        A.spmm_t_sddmm(dY, B, dB, dv)
+
+    A.spmm_t_sddmm(dY, B, dB, dv)                   # builds the fused plan (preprocessing)
+    torch.cuda.synchronize()
     ms_f = time_steps(fwd, steps, warmup)
     ms_fb = time_steps(fwd_bwd, steps, warmup)
+    ms_fbr = time_steps(fwd_bwd_routed, steps, warmup)
+    ms_fused = time_steps(lambda: A.spmm_t_sddmm(dY, B, dB, dv), steps, warmup)
     ms_db = time_steps(lambda: At.matmul(dY, dB), steps, warmup)
     ms_sd = time_steps(lambda: A.sddmm(dY, B, dv), steps, warmup)
+    if check:
+        A.spmm_t_sddmm(dY, B, dB, dv)                # the fused results are the ones checked below
     flops = 3 * 2.0 * csr.nnz * K
     out = {"workload": f"rmat18 K={K}", "dtype": dtype_name, "mode": mode, "ms_fwd": round(ms_f, 5),
            "ms_fwd_bwd": round(ms_fb, 5), "ratio_fwd_bwd_over_fwd": round(ms_fb / ms_f, 3),
-           "ms_dB": round(ms_db, 5), "ms_sddmm": round(ms_sd, 5),
+           "backward": "fused (dB + dvals in one pass over A^T rows)", "ms_bwd_fused": round(ms_fused, 5),
+           "ms_fwd_bwd_routed": round(ms_fbr, 5), "ms_dB_routed": round(ms_db, 5), "ms_sddmm": round(ms_sd, 5),
            "gflops_fwd_bwd": round(flops / (ms_fb * 1e-3) / 1e9, 1),
            "roofline_frac_fwd_bwd": round(
                (algorithmic_bytes(csr.nnz, csr.m, csr.n, K) + algorithmic_bytes(csr.nnz, csr.n, csr.m, K)
@@ -172,6 +188,67 @@ def measure_fwd_bwd(csr, K, dtype_name, mode, steps, warmup, check=True):
     return out
 
 
+def measure_batched(steps, warmup, check=True, count=1024, K=128):
+    """Config 5: `count` BA graphs in one launch vs one launch per graph."""
+    import torch
+
+    from trspmm import graphs
+    from trspmm.batch import RoutedBatch, concat_block_diagonal
+    from trspmm.sparse import RoutedSparse
+    from trspmm.tune import autotune
+    dev = torch.device("cuda", 0)
+    t0 = time.time()
+    gs = graphs.ba_batch(count)
+    gen_s = time.time() - t0
+    rowptr, col, val, row_off, col_off, m_list, n_list = concat_block_diagonal(gs)
+    ncols, rows, nnz = int(col_off[-1]), int(row_off[-1]), int(col.size)
+    Bcat = torch.from_numpy(graphs.dense_features(ncols, K, seed=5001)).to(dev)
+    t0 = time.time()
+    tune = autotune(torch.from_numpy(rowptr), torch.from_numpy(col), torch.from_numpy(val), ncols, rows, Bcat,
+                    iters=10, warmup=3)
+    tune_s = time.time() - t0
+    batch = RoutedBatch(gs, engine=tune.best["engine"], slab=tune.best["slab"], **tune.router_kwargs())
+    C = torch.empty((rows, K), device=dev, dtype=torch.float32)
+    ms_eager = time_steps(lambda: batch.matmul(Bcat, C), steps, warmup)
+    graph = batch.capture(Bcat, C)
+    ms_graph = time_steps(graph.replay, steps, warmup)
+    # plain: one routed matrix per graph (default routing and kernels), launched one after another
+    t0 = time.time()
+    plains = [RoutedSparse(torch.from_numpy(c.rowptr), torch.from_numpy(c.col), torch.from_numpy(c.val), c.n, m=c.m)
+              for c in gs]
+    plain_prep_s = time.time() - t0
+    Cs = torch.empty((rows, K), device=dev, dtype=torch.float32)
+    Bv = [Bcat[int(col_off[i]): int(col_off[i]) + int(n_list[i])] for i in range(count)]
+    Cv = [Cs[int(row_off[i]): int(row_off[i]) + int(m_list[i])] for i in range(count)]
+
+    def sequential():
+        for A, b, c in zip(plains, Bv, Cv):
+            A.matmul(b, c)
+
+    ms_seq = time_steps(sequential, max(3, steps // 10), 1)
+    flops = 2.0 * nnz * K
+    byts = algorithmic_bytes(nnz, rows, ncols, K)
+    out = {"workload": f"{count} BA graphs m=3, n_i in [20, 5000], K={K}", "rows_padded": rows, "nnz": nnz,
+           "ms_batched_graph": round(ms_graph, 5), "ms_batched_eager": round(ms_eager, 5),
+           "ms_sequential_plain": round(ms_seq, 4), "speedup_vs_sequential": round(ms_seq / ms_graph, 2),
+           "gflops_batched": round(flops / (ms_graph * 1e-3) / 1e9, 1),
+           "roofline_frac_batched": round(byts / (ms_graph * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "autotuned": tune.best, "autotune_table_ms": [[c, round(ms, 5)] for c, ms in tune.table],
+           "routes": batch.A.plan.stats, "gen_s": round(gen_s, 1), "autotune_s": round(tune_s, 1),
+           "plain_prep_s": round(plain_prep_s, 1)}
+    if check:
+        import scipy.sparse as sp
+        A64 = sp.csr_matrix((val.astype(np.float64), col, rowptr), shape=(rows, ncols))
+        ref = np.asarray(A64 @ Bcat.float().cpu().numpy().astype(np.float64))
+        graph.replay()
+        torch.cuda.synchronize()
+        out["rel_err_batched_vs_fp64"] = float(np.linalg.norm(C.cpu().numpy() - ref) / np.linalg.norm(ref))
+        sequential()
+        torch.cuda.synchronize()
+        out["rel_err_sequential_vs_fp64"] = float(np.linalg.norm(Cs.cpu().numpy() - ref) / np.linalg.norm(ref))
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -183,6 +260,9 @@ def main():
     from trspmm import graphs
     from trspmm.sparse import RoutedSparse
 
+    if a.batched:
+        print(json.dumps(measure_batched(a.steps, a.warmup, not a.no_check)), flush=True)
+        return
     builder, K, bseed = graphs.WORKLOADS[a.workload]
     t0 = time.time()
     csr = builder()
@@ -262,6 +342,10 @@ def main():
         torch.cuda.empty_cache()
         print("[bench] fwd+bwd at K=256", file=sys.stderr, flush=True)
         line["fwd_bwd"] = [measure_fwd_bwd(csr, 256, dt, a.mode, 20, 5, not a.no_check) for dt in ("fp16", "bf16")]
+        del csr
+        torch.cuda.empty_cache()
+        print("[bench] batched graphs", file=sys.stderr, flush=True)
+        line["batched"] = measure_batched(20, 5, not a.no_check)
     print(json.dumps(line), flush=True)
 
 
