@@ -50,6 +50,8 @@ def parse():
     p.add_argument("--no-check", action="store_true")
     p.add_argument("--no-extras", action="store_true", help="skip the fwd+bwd and batched measurements")
     p.add_argument("--batched", action="store_true", help="measure only the batched workload")
+    p.add_argument("--per-route", action="store_true",
+                   help="per-route rows, nnz and times of --workload (exact and tuned routing)")
     p.add_argument("--backward", action="store_true",
                    help="measure only forward vs forward+backward on --workload (fp16 and bf16)")
     p.add_argument("--copy-calibration", action="store_true",
@@ -188,6 +190,73 @@ def measure_fwd_bwd(csr, K, dtype_name, mode, steps, warmup, check=True):
     return out
 
 
+def route_rows(meta, rows):
+    """Rows holding at least one nonzero of each route, from the routed metadata
+    (device tensors): TC sub-block occupancy bits, 2:4 per-row slots, residual parts."""
+    import torch
+    t = dict(zip(__import__("trspmm").META_NAMES, meta.tensors("cuda")))
+    W = t["s_offsets"].numel() - 1
+    dev = t["s_offsets"].device
+    out = {}
+    # TC: sub-block s of part p covers window t_window_row[p] / 16; bit (r%8)*8+q of word r/8
+    tc = torch.zeros(W, 16, dtype=torch.int32, device=dev)
+    nsub = t["t_binary"].numel() // 2
+    if nsub:
+        part_of = torch.repeat_interleave(torch.arange(t["t_window_row"].numel(), device=dev),
+                                          torch.diff(t["t_row_offset"]).long())
+        win = (t["t_window_row"][part_of] // 16).long()
+        words = t["t_binary"].view(nsub, 2)
+        r = torch.arange(16, device=dev)
+        occ = ((words[:, (r // 8)] >> ((r % 8) * 8).long()) & 0xff) != 0          # [S, 16]
+        tc.index_put_((win[:, None].expand(-1, 16).reshape(-1), r.repeat(nsub)), occ.reshape(-1).int(),
+                      accumulate=True)
+    sp = torch.zeros(W, 16, dtype=torch.int32, device=dev)
+    G = t["s_column"].shape[0]
+    if G:
+        hit = (t["s_pos"].view(G, 16, 2) >= 0).any(-1)
+        sw = t["s_window"].long()
+        r = torch.arange(16, device=dev)
+        sp.index_put_((sw[:, None].expand(-1, 16).reshape(-1), r.repeat(G)), hit.reshape(-1).int(), accumulate=True)
+    res = torch.zeros(W * 16, dtype=torch.bool, device=dev)
+    for rows_t, off_t in ((t["c_row"], t["c_row_offset"]), (t["c_row_short"], t["c_row_offset_short"])):
+        if rows_t.numel():
+            nz = torch.diff(off_t) > 0
+            res[rows_t[nz].long()] = True
+    m = rows
+    out["tc"] = int((tc.view(-1)[:m] > 0).sum())
+    out["sptc"] = int((sp.view(-1)[:m] > 0).sum())
+    out["residual"] = int(res[:m].sum())
+    return out
+
+
+def measure_routes(csr, K, mode, steps, warmup):
+    """Config 3 view of one routing: per-route nnz, rows and launch times."""
+    import torch
+
+    from trspmm import ROUTE_RESIDUAL, ROUTE_SPTC, ROUTE_TC, graphs
+    from trspmm.sparse import RoutedSparse
+    dev = torch.device("cuda", 0)
+    B = torch.from_numpy(graphs.dense_features(csr.n, K, seed=1002)).to(dev)
+    A = RoutedSparse(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val), csr.n,
+                     m=csr.m, mode=mode)
+    C = torch.empty((csr.m, K), device=dev, dtype=torch.float32)
+    st = A.plan.stats
+    ms = {name: time_steps(lambda mk=mk: A.matmul(B, C, routes=mk), steps, warmup)
+          for name, mk in (("all", 7), ("tc", ROUTE_TC), ("sptc", ROUTE_SPTC), ("residual", ROUTE_RESIDUAL))}
+    nnz = {"tc": st["tc_nnz"], "residual": st["residual_nnz"]}
+    nnz["sptc"] = csr.nnz - nnz["tc"] - nnz["residual"]
+    out = {"mode": mode, "ms": {k: round(v, 5) for k, v in ms.items()}, "nnz": nnz,
+           "rows_with_route": route_rows(A.meta, csr.m), "tc_subblocks": st["tc_subblocks"],
+           "sptc_tiles": st["sptc_tiles"], "router_s": round(A.meta.seconds, 4),
+           "gflops": round(2.0 * csr.nnz * K / (ms["all"] * 1e-3) / 1e9, 1),
+           "roofline_frac": round(algorithmic_bytes(csr.nnz, csr.m, csr.n, K) / (ms["all"] * 1e-3) / 1e9
+                                  / HBM_PEAK_GBS, 4)}
+    for name in ("tc", "sptc", "residual"):
+        if ms[name] > 0 and nnz[name]:
+            out.setdefault("gflops_per_route", {})[name] = round(2.0 * nnz[name] * K / (ms[name] * 1e-3) / 1e9, 1)
+    return out
+
+
 def measure_batched(steps, warmup, check=True, count=1024, K=128):
     """Config 5: `count` BA graphs in one launch vs one launch per graph."""
     import torch
@@ -267,6 +336,17 @@ def main():
     t0 = time.time()
     csr = builder()
     gen_s = time.time() - t0
+    if a.per_route:
+        n = csr.m
+        b1 = (int(n * 0.2) // 64) * 64
+        b2 = b1 + (int(n * 0.3) // 16) * 16
+        regions = {"communities": [0, b1], "banded": [b1, b2], "tail": [b2, n]} if a.workload.startswith(
+            "three_route") else {}
+        for mode in (["exact", "tuned"] if a.mode == "exact" else [a.mode]):
+            rec = measure_routes(csr, K, mode, a.steps, a.warmup)
+            rec.update(workload=a.workload, rows=csr.m, nnz_total=csr.nnz, K=K, regions_rows=regions)
+            print(json.dumps(rec), flush=True)
+        return
     if a.backward:
         for dt in ("fp16", "bf16"):
             print(json.dumps(measure_fwd_bwd(csr, K, dt, a.mode, a.steps, a.warmup, not a.no_check)), flush=True)
