@@ -302,7 +302,8 @@ def measure_batched(steps, warmup, check=True, count=1024, K=128):
     tune = autotune(torch.from_numpy(rowptr), torch.from_numpy(col), torch.from_numpy(val), ncols, rows, Bcat,
                     iters=10, warmup=3)
     tune_s = time.time() - t0
-    batch = RoutedBatch(gs, engine=tune.best["engine"], slab=tune.best["slab"], **tune.router_kwargs())
+    batch = RoutedBatch(gs, engine=tune.best["engine"], slab=tune.best["slab"], plan_opts=tune.plan_opts(),
+                        **tune.router_kwargs())
     C = torch.empty((rows, K), device=dev, dtype=torch.float32)
     ms_eager = time_steps(lambda: batch.matmul(Bcat, C), steps, warmup)
     graph = batch.capture(Bcat, C)

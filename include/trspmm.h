@@ -126,12 +126,18 @@ typedef struct trspmm_plan trspmm_plan;
 
 /* Plan options (no reference counterpart; the reference's kernels are fixed).
  * Zero-initialised = defaults (environment TRSPMM_RESIDUAL / TRSPMM_SLAB, else
- * the vector-ALU residual engine and 64-feature slabs). */
+ * the matrix-core residual engine and 64-feature slabs; caps 64 / 64 / 2048;
+ * round-robin work items). */
 enum { TRSPMM_ENGINE_DEFAULT = 0, TRSPMM_ENGINE_VALU = 1, TRSPMM_ENGINE_MFMA = 2 };
 typedef struct trspmm_plan_opts {
     int32_t residual_engine;  /* TRSPMM_ENGINE_* */
     int32_t slab;             /* tile-kernel features per pass: 0 (default), 64 or 128 */
-    int32_t reserved[6];
+    int32_t tile_cap;         /* a window with more 2:4 tiles is split into chunks of this many (0 = 64) */
+    int32_t tc_cap;           /* the same for TC sub-blocks (0 = 64) */
+    int32_t res_cap;          /* the same for residual entries (0 = 2048) */
+    int32_t item_order;       /* 0: work items dealt round robin to the XCDs (the dispatch order);
+                                 1: each XCD takes a contiguous range of work items */
+    int32_t reserved[2];
 } trspmm_plan_opts;
 
 /* Builds the device plan (AMD lane layouts) from reference-format metadata.

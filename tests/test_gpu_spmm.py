@@ -75,9 +75,9 @@ def test_spmm_tuned_routing(cuda, case):
 
 @pytest.mark.parametrize("case", ["hub", "rmat10", "ragged", "dense"])
 @pytest.mark.parametrize("mode", ["exact", "tuned"])
-def test_spmm_matrix_core_residual_engine(cuda, case, mode, monkeypatch):
-    """The matrix-core residual engine (one-hot fragments) gives the same product."""
-    monkeypatch.setenv("TRSPMM_RESIDUAL", "mfma")
+def test_spmm_vector_alu_residual_engine(cuda, case, mode, monkeypatch):
+    """The vector-ALU residual engine (separate CSR kernel) gives the same product."""
+    monkeypatch.setenv("TRSPMM_RESIDUAL", "valu")
     err, worst, A = _run(CASES[case](), 128, mode=mode)
     assert err < TOL_FRO and worst < 1e-4, (err, worst, A.stats)
 

@@ -14,6 +14,7 @@
 
 #include <rocprim/rocprim.hpp>
 
+#include <type_traits>
 #include <vector>
 
 #include "backward.h"
@@ -85,8 +86,10 @@ __device__ inline float lo16(uint32_t w) {
 }
 
 // dot product of 8 packed 16-bit values.  The halves are taken from the 32-bit
-// words with shifts: indexing an 8 x 16-bit vector bit-cast from the load
-// compiled to eight reads of element 0 (hipcc, ROCm 7.2, gfx950).
+// words with shifts: bit-casting the load (or one of its 32-bit words) to a
+// vector of 16-bit values and using its elements compiled to repeated reads
+// of element 0 (hipcc, ROCm 7.2, gfx950), with fp32 conversions and with the
+// packed dot instructions alike.
 template <typename DT>
 __device__ inline float dot8(u32x4 a, u32x4 b) {
     float s = 0.f;
@@ -109,35 +112,35 @@ __device__ inline void rs_step(float (&part)[SDDMM_U], bool hi, int bit) {
     }
 }
 
-// Reduce-scatter of 8 per-edge partial sums over a lane group of LPR lanes
-// (j = lane within the group): at each lane-bit keep the half of the edge set
-// that bit selects and add the partner lane's copy of it; lane-bits left once
-// each lane holds one edge are plain all-reduces.  Afterwards the lane holds
-// the full sums of edges [base, base + NV) in part[0..NV), and the lanes whose
-// bits in LOWMASK are zero are the writers.
-template <int LPR>
+// Reduce-scatter of NE (2, 4 or 8) per-edge partial sums (part[0..NE)) over a
+// lane group of LPR lanes (j = lane within the group): at each lane-bit keep
+// the half of the edge set that bit selects and add the partner lane's copy
+// of it; lane-bits left once each lane holds one edge are plain all-reduces.
+// Afterwards the lane holds the full sums of edges [base, base + NV) in
+// part[0..NV), and the lanes whose bits in LOWMASK are zero are the writers.
+template <int NE, int BIT>
+__device__ inline void rs_steps(float (&part)[SDDMM_U], int j, int& base) {
+    if constexpr (BIT >= 1) {
+        if constexpr (NE > 1) {
+            const bool hi = (j & BIT) != 0;
+            rs_step<NE / 2>(part, hi, BIT);
+            base += hi ? NE / 2 : 0;
+            rs_steps<NE / 2, BIT / 2>(part, j, base);
+        } else {
+            part[0] += __shfl_xor(part[0], BIT, 64);
+            rs_steps<1, BIT / 2>(part, j, base);
+        }
+    }
+}
+
+template <int LPR, int NE = SDDMM_U>
 struct EdgeReduce {
-    static constexpr int NV = LPR >= 8 ? 1 : (LPR >= 4 ? 2 : (LPR >= 2 ? 4 : 8));
-    static constexpr int LOWMASK = LPR >= 16 ? LPR / 8 - 1 : 0;
+    static_assert(NE >= 1 && NE <= SDDMM_U && (NE & (NE - 1)) == 0, "NE must be a power of two <= 8");
+    static constexpr int NV = NE >= LPR ? NE / LPR : 1;
+    static constexpr int LOWMASK = NE >= LPR ? 0 : LPR / NE - 1;
     __device__ static inline int run(float (&part)[SDDMM_U], int j) {
         int base = 0;
-        if constexpr (LPR >= 2) {
-            const bool hi = (j & (LPR / 2)) != 0;
-            rs_step<4>(part, hi, LPR / 2);
-            base += hi ? 4 : 0;
-        }
-        if constexpr (LPR >= 4) {
-            const bool hi = (j & (LPR / 4)) != 0;
-            rs_step<2>(part, hi, LPR / 4);
-            base += hi ? 2 : 0;
-        }
-        if constexpr (LPR >= 8) {
-            const bool hi = (j & (LPR / 8)) != 0;
-            rs_step<1>(part, hi, LPR / 8);
-            base += hi ? 1 : 0;
-        }
-#pragma unroll
-        for (int b = LPR / 16; b >= 1; b >>= 1) part[0] += __shfl_xor(part[0], b, 64);
+        rs_steps<NE, LPR / 2>(part, j, base);
         return base;
     }
 };
@@ -235,6 +238,7 @@ int launch_sddmm_typed(const int32_t* rowptr, int64_t rows, const int32_t* col, 
 // wave's groups (plain stores); kind 1 = a chunk [b, e) of one long row split
 // over the groups (fp32 atomic adds into a row zeroed beforehand).
 constexpr int BWD_BLOCK = 256;
+constexpr int BWD_U = 4;                // entries per batch (two batches in flight)
 constexpr int BWD_ROW_CAP = 512;        // entries of one row handled as a whole
 constexpr int BWD_ITEM_CAP = 1024;      // entries of a kind-0 item
 constexpr int BWD_ITEM_ROWS = 16;       // rows of a kind-0 item
@@ -254,8 +258,8 @@ __device__ inline void bwd_row_range(int64_t c, int32_t e0, int32_t e1, bool ato
                                      const uint16_t* __restrict__ dY, int64_t lddy,
                                      const uint16_t* __restrict__ B, int64_t ldb, int K,
                                      float* __restrict__ dB, int64_t lddb, float* __restrict__ dvt) {
-    constexpr int U = SDDMM_U;
-    using R = EdgeReduce<LPR>;
+    constexpr int U = BWD_U;
+    using R = EdgeReduce<LPR, U>;
     u32x4 bc[NCH];
     float acc[NCH][8];
 #pragma unroll
@@ -265,32 +269,52 @@ __device__ inline void bwd_row_range(int64_t c, int32_t e0, int32_t e1, bool ato
 #pragma unroll
         for (int q = 0; q < 8; ++q) acc[k][q] = 0.f;
     }
-    for (int32_t eb = e0; eb < e1; eb += U) {
-        int32_t r[U];
-        float v[U];
+    // Batches of U entries in a two-ahead pipeline, unrolled by two so the ring
+    // slots are compile-time: batch b's dY rows were gathered during batch
+    // b - 1 from indices loaded during batch b - 2, so computing b waits only
+    // for loads issued before the current batch's.  Every load is unconditional
+    // (indices clamped to the last entry; values masked where they are used), so
+    // the count of loads in flight is the same on every path.  An odd batch
+    // count gets one masked extra batch.
+    const int nb = (e1 - e0 + U - 1) / U;
+    int32_t ridx[2][U];
+    uint16_t vraw[2][U];
+    u32x4 y[2][U][NCH];
+    auto load_idx = [&](int b, int32_t (&r)[U]) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int32_t e = eb + u < e1 ? eb + u : e1 - 1;     // clamped: unconditional loads
-            r[u] = col[e];
-            v[u] = eb + u < e1 ? lo16<DT>(val[e]) : 0.f;
+            const int32_t e = e0 + b * U + u;
+            r[u] = col[e < e1 ? e : e1 - 1];
         }
-        u32x4 y[U][NCH];
+    };
+    auto load_val = [&](int b, uint16_t (&v)[U]) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int32_t e = e0 + b * U + u;
+            v[u] = val[e < e1 ? e : e1 - 1];
+        }
+    };
+    auto gather = [&](const int32_t (&r)[U], u32x4 (&yy)[U][NCH]) {
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int k = 0; k < NCH; ++k) {
                 const int f = 8 * j + 8 * LPR * k;
-                y[u][k] = *(const u32x4*)(dY + (int64_t)r[u] * lddy + (f < K ? f : 0));
+                yy[u][k] = *(const u32x4*)(dY + (int64_t)r[u] * lddy + (f < K ? f : 0));
             }
-        float part[U];
+    };
+    auto compute = [&](int b, const u32x4 (&yy)[U][NCH], const uint16_t (&v)[U]) {
+        const int32_t eb = e0 + b * U;
+        float part[SDDMM_U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
+            const float vu = eb + u < e1 ? lo16<DT>(v[u]) : 0.f;
             part[u] = 0.f;
 #pragma unroll
             for (int k = 0; k < NCH; ++k) {
                 if (8 * j + 8 * LPR * k < K) {
-                    fma8<DT>(acc[k], v[u], y[u][k]);
-                    part[u] += dot8<DT>(y[u][k], bc[k]);
+                    fma8<DT>(acc[k], vu, yy[u][k]);
+                    part[u] += dot8<DT>(yy[u][k], bc[k]);
                 }
             }
         }
@@ -299,6 +323,23 @@ __device__ inline void bwd_row_range(int64_t c, int32_t e0, int32_t e1, bool ato
 #pragma unroll
             for (int k = 0; k < R::NV; ++k)
                 if (eb + base + k < e1) dvt[eb + base + k] = part[k];
+        }
+    };
+    if (nb > 0) {
+        load_idx(0, ridx[0]);
+        load_idx(1, ridx[1]);
+        gather(ridx[0], y[0]);
+        load_val(0, vraw[0]);
+        auto body = [&](int b, auto par_tag) {
+            constexpr int PAR = decltype(par_tag)::value;
+            load_idx(b + 2, ridx[PAR]);              // batch b + 2
+            gather(ridx[1 - PAR], y[1 - PAR]);       // batch b + 1
+            load_val(b + 1, vraw[1 - PAR]);
+            compute(b, y[PAR], vraw[PAR]);           // batch b
+        };
+        for (int b = 0; b < nb; b += 2) {
+            body(b, std::integral_constant<int, 0>{});
+            body(b + 1, std::integral_constant<int, 1>{});
         }
     }
 #pragma unroll

@@ -140,6 +140,7 @@ __device__ inline float bits16_to_f32(uint16_t b, int dtype) {
 //   tc_bits      [S*2]  occupancy: bit (r%8)*8+q of word r/8 <=> (row r, slot q) nonzero
 //   tc_valoff    [S+1]  first value of each sub-block; values stored in bit order
 //   tc_vals      [nnz_tc] 16-bit values (plan dtype)
+//   tc_dense     [S*16*8] the same values per (sub-block, row), zeros in empty slots
 //   sp_tile_off  [W+1]  2:4 tiles of window w; a tile = 4 groups x 4 columns
 //   sp_cols      [T*16] columns (-1 = empty)
 //   sp_a         [T*16*4] uint32: (tile, row r, group c) -> two 16-bit values (v0 | v1 << 16)
@@ -166,6 +167,9 @@ struct PlanDev {
     const uint64_t* tc_bits = nullptr;
     const int32_t* tc_valoff = nullptr;
     const uint16_t* tc_vals = nullptr;
+    // tc_dense [S*16] rows of 8 16-bit values (16 bytes): row r of sub-block s
+    // with zeros in its empty slots, the A-fragment row the tile kernel loads
+    const uint32_t* tc_dense = nullptr;
     const int32_t* sp_tile_off = nullptr;
     const int32_t* sp_cols = nullptr;
     const uint32_t* sp_a = nullptr;
@@ -181,6 +185,8 @@ struct PlanDev {
     int32_t res_mfma = 0;
     // features per pass of the tile kernel (64 or 128)
     int32_t slab = 64;
+    // work item -> XCD: 0 = round robin (dispatch order), 1 = contiguous item ranges per XCD
+    int32_t item_order = 0;
     const int4* titems = nullptr;
     int64_t n_titems = 0;
     const int4* ritems = nullptr;
@@ -192,7 +198,7 @@ struct PlanDev {
 // work-item caps: a window is split when it has more TC sub-blocks, 2:4
 // tiles or residual entries than these
 struct SplitCaps {
-    int32_t tc_sub = 32, tiles = 32, res_entries = 2048;
+    int32_t tc_sub = 64, tiles = 64, res_entries = 2048;
 };
 
 // Workgroup index -> window with consecutive windows kept on one XCD.

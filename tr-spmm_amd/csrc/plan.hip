@@ -271,12 +271,29 @@ inline unsigned blocks_for(int64_t n, int threads = 256) {
 
 }  // namespace
 
+// dense TC rows: (sub-block s, row r) -> its 8 slots as 16-bit values (zeros
+// where the occupancy bit is clear), the MFMA A-fragment row of that lane
+__global__ void k_tc_dense(const uint64_t* __restrict__ bits, const int32_t* __restrict__ valoff,
+                           const uint16_t* __restrict__ vals, int64_t S, uint16_t* __restrict__ dense) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 16 * S) return;
+    const int64_t sb = i / 16;
+    const int r = (int)(i % 16);
+    const uint64_t lo = bits[2 * sb], hi = bits[2 * sb + 1];
+    const uint64_t word = r < 8 ? lo : hi;
+    const int sh = (r & 7) * 8;
+    const uint32_t b = (uint32_t)(word >> sh) & 0xffu;
+    int vi = valoff[sb] + (r < 8 ? 0 : __popcll(lo)) + __popcll(word & ((1ull << sh) - 1ull));
+    for (int q = 0; q < 8; ++q) dense[i * 8 + q] = ((b >> q) & 1u) ? vals[vi++] : (uint16_t)0;
+}
+
 int Plan::finalize() {
     dev.tc_sub_off = tc_sub_off.as<int32_t>();
     dev.tc_cols = tc_cols.as<int32_t>();
     dev.tc_bits = tc_bits.as<uint64_t>();
     dev.tc_valoff = tc_valoff.as<int32_t>();
     dev.tc_vals = tc_vals.as<uint16_t>();
+    dev.tc_dense = tc_dense.as<uint32_t>();
     dev.sp_tile_off = sp_tile_off.as<int32_t>();
     dev.sp_cols = sp_cols.as<int32_t>();
     dev.sp_a = sp_a.as<uint32_t>();
@@ -362,15 +379,16 @@ static int build_residual(const trspmm_meta_view& v, int64_t ncols, int dtype, h
     return TRSPMM_OK;
 }
 
-// residual engine of a new plan: the vector-ALU CSR kernel (default; measured
-// faster on the R-MAT workloads) or one-hot fragments on the matrix cores,
-// chosen by the plan options, else by TRSPMM_RESIDUAL=valu|mfma in the
+// residual engine of a new plan: one-hot fragments on the matrix cores inside
+// the tile kernel (default; measured faster on the R-MAT workloads, it saves
+// the second kernel and its read-modify-write of C) or the vector-ALU CSR
+// kernel, chosen by the plan options, else by TRSPMM_RESIDUAL=valu|mfma in the
 // environment when the plan is built
 static int residual_on_matrix_cores(const trspmm_plan_opts& o) {
     if (o.residual_engine == TRSPMM_ENGINE_VALU) return 0;
     if (o.residual_engine == TRSPMM_ENGINE_MFMA) return 1;
     const char* e = std::getenv("TRSPMM_RESIDUAL");
-    return (e && std::strcmp(e, "mfma") == 0) ? 1 : 0;
+    return (e && std::strcmp(e, "valu") == 0) ? 0 : 1;
 }
 
 // feature slab of the tile kernel: the plan options, else TRSPMM_SLAB=64|128, else 64
@@ -387,6 +405,13 @@ int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtyp
     TRSPMM_CHECK(P.opts.slab == 0 || P.opts.slab == 64 || P.opts.slab == 128, "plan options: slab must be 0, 64 or 128");
     P.dev.res_mfma = residual_on_matrix_cores(P.opts);
     P.dev.slab = tile_slab(P.opts);
+    TRSPMM_CHECK(P.opts.tile_cap >= 0 && P.opts.tc_cap >= 0 && P.opts.res_cap >= 0,
+                 "plan options: caps must be >= 0");
+    TRSPMM_CHECK(P.opts.item_order == 0 || P.opts.item_order == 1, "plan options: item_order must be 0 or 1");
+    if (P.opts.tile_cap) P.caps.tiles = P.opts.tile_cap;
+    if (P.opts.tc_cap) P.caps.tc_sub = P.opts.tc_cap;
+    if (P.opts.res_cap) P.caps.res_entries = P.opts.res_cap;
+    P.dev.item_order = P.opts.item_order;
     TRSPMM_CHECK(dtype == TRSPMM_F16 || dtype == TRSPMM_BF16, "dtype must be TRSPMM_F16 or TRSPMM_BF16");
     TRSPMM_CHECK(m_out > 0 && ncols > 0, "m_out and ncols must be positive");
     TRSPMM_CHECK(v.n_windows >= 0 && v.n_windows * 16 >= m_out, "windows must cover the output rows");
@@ -420,6 +445,10 @@ int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtyp
         if (v.n_tval > 0)
             hipLaunchKernelGGL(k_copy_vals, dim3(blocks_for(v.n_tval)), dim3(256), 0, s, v.t_value, v.n_tval,
                                v.t_value_is_f32, dtype, P.tc_vals.as<uint16_t>());
+        TRSPMM_HIP(hipGetLastError());
+        if ((rc = P.tc_dense.alloc(2 * 16 * 8 * v.n_tsub))) return rc;
+        hipLaunchKernelGGL(k_tc_dense, dim3(blocks_for(16 * v.n_tsub)), dim3(256), 0, s, P.tc_bits.as<uint64_t>(),
+                           P.tc_valoff.as<int32_t>(), P.tc_vals.as<uint16_t>(), v.n_tsub, P.tc_dense.as<uint16_t>());
         TRSPMM_HIP(hipGetLastError());
     }
 

@@ -23,6 +23,7 @@
 // Every output element is written by exactly one work item except in split
 // windows, whose rows are zeroed and accumulated with fp32 atomics.
 #include <cstdlib>
+#include <type_traits>
 
 #include "plan.h"
 
@@ -76,37 +77,63 @@ __device__ inline s16x4 lds_tr16(const uint16_t* p) {
 // ============================================================ tile routes
 //
 // A step is one matrix instruction's worth of K: 4 TC sub-blocks (32 gathered
-// rows) or 4 2:4 tiles (64 gathered rows).  Lane l gathers the 16-byte pieces
+// rows), 4 2:4 tiles (64 gathered rows) or, with the matrix-core residual
+// engine, 32 residual entries.  Lane l gathers the 16-byte pieces
 // c = l + 64k of the step's rows (row c / (F/8), piece c % (F/8)).
+//
+// Software pipeline.  Vector loads return in issue order and a value is waited
+// for where it is first used, so every load here is issued well before its
+// first use and nothing consumes a loaded value early:
+//   * gathered pieces are kept raw with a validity mask applied only when they
+//     are parked in LDS, one sub-step after they were issued;
+//   * a step's column indices and its A fragment are loaded at the start of
+//     the previous step, ahead of that step's gathers.
 template <int F>
 struct Gather {
     static constexpr int PIECES = F / 8;
-    static constexpr int MAXK = 64 * PIECES / 64;   // pieces per lane for a 64-row step
+    static constexpr int MAXK = PIECES;              // pieces per lane for a 64-row step
     u32x4 v[MAXK];
+    uint32_t ok;                                     // bit k: piece k holds data (else parked as zeros)
+};
+
+// column indices of a step's staged rows, loaded ahead of its gathers
+template <int F>
+struct StepCols {
+    int32_t c[Gather<F>::MAXK];
+    int32_t nrows;
 };
 
 // cols[row * STRIDE] is the column of staged row `row` (STRIDE 2: the low word
-// of a residual entry record)
-template <int F, int R, int STRIDE = 1>
-__device__ inline void gather_rows(Gather<F>& g, const uint16_t* __restrict__ B, int64_t ldb, int f0, int fw,
-                                   const int32_t* __restrict__ cols, int nrows, int lane) {
+// of a residual entry record); nrows >= 1
+template <int F, int R, int STRIDE>
+__device__ inline void load_cols(StepCols<F>& sc, const int32_t* __restrict__ cols, int nrows, int lane) {
     constexpr int PIECES = F / 8;
-    // unconditional loads from clamped in-range addresses, masked afterwards
-    // (a predicated load would make the compiler wait for each one)
-    int col[R * PIECES / 64];
+    sc.nrows = nrows;
 #pragma unroll
     for (int k = 0; k < R * PIECES / 64; ++k) {
         const int row = (lane + 64 * k) / PIECES;
-        col[k] = cols[(row < nrows ? row : nrows - 1) * STRIDE];
+        sc.c[k] = cols[(row < nrows ? row : nrows - 1) * STRIDE];     // clamped: unconditional load
     }
+}
+
+// B rows addressed with 32-bit element offsets (the launcher checks that B
+// holds fewer than 2^31 elements)
+template <int F, int R>
+__device__ inline void gather_rows(Gather<F>& g, const StepCols<F>& sc, const uint16_t* __restrict__ B, uint32_t ldb,
+                                   int f0, int fw, int lane) {
+    constexpr int PIECES = F / 8;
+    uint32_t ok = 0;
 #pragma unroll
     for (int k = 0; k < R * PIECES / 64; ++k) {
         const int c = lane + 64 * k;
         const int row = c / PIECES, piece = c % PIECES;
-        const bool ok = row < nrows && col[k] >= 0 && piece * 8 < fw;
-        const u32x4 x = *(const u32x4*)(B + (int64_t)(ok ? col[k] : 0) * ldb + f0 + (ok ? piece * 8 : 0));
-        g.v[k] = ok ? x : u32x4{0u, 0u, 0u, 0u};
+        // bitwise & (no short-circuit): a branch per piece would put a wait in each
+        const bool v = (row < sc.nrows) & (sc.c[k] >= 0) & (piece * 8 < fw);
+        const uint32_t off = v ? (uint32_t)sc.c[k] * ldb + (uint32_t)(f0 + piece * 8) : 0u;
+        g.v[k] = *(const u32x4*)(B + off);
+        ok |= (uint32_t)v << k;
     }
+    g.ok = ok;
 }
 
 template <int F, int R, int LDS_STRIDE>
@@ -116,40 +143,164 @@ __device__ inline void park_rows(const Gather<F>& g, uint16_t* __restrict__ Bs, 
     for (int k = 0; k < R * PIECES / 64; ++k) {
         const int c = lane + 64 * k;
         const int row = c / PIECES, piece = c % PIECES;
+        const u32x4 x = ((g.ok >> k) & 1u) ? g.v[k] : u32x4{0u, 0u, 0u, 0u};
         u32x2* dst = (u32x2*)(Bs + row * LDS_STRIDE + piece * 8);
-        dst[0] = u32x2{g.v[k].x, g.v[k].y};
-        dst[1] = u32x2{g.v[k].z, g.v[k].w};
+        dst[0] = u32x2{x.x, x.y};
+        dst[1] = u32x2{x.z, x.w};
     }
 }
 
-// A fragment of a TC step: lane (kg, r) holds row r of sub-block sb+kg, the
-// 8 slots decoded from the occupancy bits (values stored in bit order).
-__device__ inline u16x8 tc_fragment(const PlanDev& p, int32_t my, int32_t s1, int r) {
-    u16x8 abits = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (my < s1) {
-        const uint64_t lo = p.tc_bits[2 * (int64_t)my], hi = p.tc_bits[2 * (int64_t)my + 1];
-        const uint64_t word = r < 8 ? lo : hi;
-        const int sh = (r & 7) * 8;
-        const uint32_t bits = (uint32_t)(word >> sh) & 0xffu;
-        int vi = p.tc_valoff[my] + (r < 8 ? 0 : __popcll(lo)) + __popcll(word & ((1ull << sh) - 1ull));
+// A fragment loaded ahead of its use and applied (masked) there
+struct Frag {
+    u32x4 a;
+    int32_t idx;
+    bool ok;
+};
+
+// Step loop of one route kind over the steps [0, n) of a work item, for one
+// pass of `ns` slabs (F features each, from feature fp).  KIND 0 = TC
+// sub-blocks (32 rows, dense A rows, MFMA), 1 = 2:4 tiles (64 rows, SMFMAC),
+// 2 = residual entries as one-hot fragments (32 rows, MFMA).
+//
+// Sub-step s is (step s / ns, slab s % ns).  The loop is unrolled by two so
+// that a sub-step's parity is known at compile time:
+//   * even sub-steps accumulate into acc[0] and odd ones into acc[1] (one slab
+//     per parity when ns == 2; two partial sums of the same slab when ns == 1,
+//     added by the caller);
+//   * column indices and A fragments live in a two-entry ring: sub-step s
+//     parks its pieces (gathered during s - 1), applies the fragment loaded at
+//     s - 2, issues the gathers of s + 1 from the columns loaded at s - 1 and
+//     loads the columns and fragment of s + 2.
+// Every load is unconditional (indices clamped to the last step, fragments of
+// sub-steps past the end masked to zero), so the count of loads in flight is
+// the same on every path and each wait covers exactly the data it needs.  An
+// odd sub-step count gets one masked extra sub-step.
+template <int KIND, int F, typename DT, int LDS_STRIDE>
+__device__ inline void run_steps(const PlanDev& p, const uint16_t* __restrict__ B, uint32_t ldb, int K, int fp, int ns,
+                                 int32_t first, int32_t end, int n, uint16_t* __restrict__ Bs,
+                                 const uint16_t* __restrict__ bp, int lane, f32x4 (&acc)[2][F / 16]) {
+    using X8 = typename DT::x8;
+    using X16 = typename DT::x16;
+    constexpr int NB = F / 16;
+    constexpr int R = KIND == 1 ? 64 : 32;        // staged rows per step
+    constexpr int UNITS = KIND == 2 ? 32 : 4;     // entries / sub-blocks / tiles per step
+    const int r = lane & 15, kg = lane >> 4;
+    const int sh = ns - 1;                        // step = s >> sh, slab = s & sh
+    const int nsub = n << sh;
+
+    auto cols_of = [&](int s, StepCols<F>& sc) {
+        int step = s >> sh;
+        step = step < n ? step : n - 1;
+        const int32_t u = first + UNITS * step;
+        if constexpr (KIND == 0)
+            load_cols<F, R, 1>(sc, p.tc_cols + (int64_t)u * 8, (end - u) * 8 < R ? (end - u) * 8 : R, lane);
+        else if constexpr (KIND == 1)
+            load_cols<F, R, 1>(sc, p.sp_cols + (int64_t)u * 16, (end - u) * 16 < R ? (end - u) * 16 : R, lane);
+        else
+            load_cols<F, R, 2>(sc, (const int32_t*)(p.res_rec + u), end - u < R ? end - u : R, lane);
+    };
+    // A fragment of sub-step s (masked at use): TC = row r of sub-block kg of its
+    // step; 2:4 = row r of tile kg (4 groups x 2 values) and its index nibbles
+    auto frag_of = [&](int s, Frag& fr) {
+        int step = s >> sh;
+        const bool live = s < nsub;
+        step = step < n ? step : n - 1;
+        if constexpr (KIND == 0) {
+            const int32_t my = first + 4 * step + kg, mc = my < end ? my : end - 1;
+            fr.a = *(const u32x4*)(p.tc_dense + ((int64_t)mc * 16 + r) * 4);
+            fr.ok = live && my < end;
+        } else if constexpr (KIND == 1) {
+            const int32_t my = first + 4 * step + kg, mc = my < end ? my : end - 1;
+            fr.a = *(const u32x4*)(p.sp_a + ((int64_t)mc * 16 + r) * 4);
+            fr.idx = p.sp_idx[(int64_t)mc * 16 + r];
+            fr.ok = live && my < end;
+        } else {
+            fr.ok = live;
+        }
+    };
+    auto gather = [&](Gather<F>& g, const StepCols<F>& sc, int s) {
+        const int f0 = fp + (s & sh) * F;
+        gather_rows<F, R>(g, sc, B, ldb, f0, K - f0 < F ? K - f0 : F, lane);
+    };
+
+    Gather<F> g;
+    StepCols<F> cb[2];
+    Frag fb[2] = {{{0u, 0u, 0u, 0u}, 0x4444, false}, {{0u, 0u, 0u, 0u}, 0x4444, false}};
+    cols_of(0, cb[0]);
+    frag_of(0, fb[0]);
+    gather(g, cb[0], 0);
+    cols_of(1, cb[1]);
+    frag_of(1, fb[1]);
+
+    auto body = [&](int s, auto par_tag) {
+        constexpr int PAR = decltype(par_tag)::value;
+        X8 a;
+        int idx = 0x4444;
+        if constexpr (KIND == 2) {
+            // residual entries as a 16x32 one-hot fragment: staged row k = entry k, and
+            // A[r][k] = value of entry k if its local row is r (records: col | val << 32 | lrow << 48)
+            u16x8 abits = {0, 0, 0, 0, 0, 0, 0, 0};
+            int step = s >> sh;
+            step = step < n ? step : n - 1;
+            const int32_t eb = first + 32 * step + 8 * kg;
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-            if ((bits >> q) & 1u) abits[q] = p.tc_vals[vi++];
+            for (int q = 0; q < 8; ++q) {
+                const int32_t e = eb + q < end ? eb + q : end - 1;
+                const uint64_t rec = p.res_rec[e];
+                const bool hit = s < nsub && eb + q < end && (int)((rec >> 48) & 0xffu) == r && (int32_t)rec >= 0;
+                abits[q] = hit ? (uint16_t)(rec >> 32) : (uint16_t)0;
+            }
+            a = __builtin_bit_cast(X8, abits);
+        }
+        __syncthreads();                               // previous sub-step's LDS reads are done
+        park_rows<F, R, LDS_STRIDE>(g, Bs, lane);
+        if constexpr (KIND != 2) {
+            a = __builtin_bit_cast(X8, fb[PAR].ok ? fb[PAR].a : u32x4{0u, 0u, 0u, 0u});
+            idx = fb[PAR].ok ? fb[PAR].idx : 0x4444;   // (0,1) in every group: valid, times zeros
+        }
+        gather(g, cb[1 - PAR], s + 1);                 // overlaps this sub-step's math
+        cols_of(s + 2, cb[PAR]);
+        frag_of(s + 2, fb[PAR]);
+        __syncthreads();
+        if constexpr (KIND == 1) {
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                // B operand of v_smfmac_f32_16x16x64 (measured lane layout): slot q of lane
+                // (kg, n) holds staged row 32*(q/8) + 8*kg + q%8 of feature n, where staged
+                // row 16t + 4g + p is column p of group g of the t-th tile (the A layout:
+                // lane (kg, r) = tile kg, row r, groups 0..3, nibble g = p0 | p1 << 2).
+                const s16x4 b0 = lds_tr16(bp + nb * 16), b1 = lds_tr16(bp + 4 * LDS_STRIDE + nb * 16);
+                const s16x4 b2 = lds_tr16(bp + 32 * LDS_STRIDE + nb * 16);
+                const s16x4 b3 = lds_tr16(bp + 36 * LDS_STRIDE + nb * 16);
+                const s16x16 bb = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3],
+                                   b2[0], b2[1], b2[2], b2[3], b3[0], b3[1], b3[2], b3[3]};
+                acc[PAR][nb] = mma_sparse<DT>(a, __builtin_bit_cast(X16, bb), acc[PAR][nb], idx);
+            }
+        } else {
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                // B operand of v_mfma_f32_16x16x32: lane (kg, n) holds staged rows 8kg..8kg+7, feature n
+                const s16x4 b0 = lds_tr16(bp + nb * 16), b1 = lds_tr16(bp + 4 * LDS_STRIDE + nb * 16);
+                const s16x8 bb = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+                acc[PAR][nb] = mma_dense<DT>(a, __builtin_bit_cast(X8, bb), acc[PAR][nb]);
+            }
+        }
+    };
+    for (int s = 0; s < nsub; s += 2) {
+        body(s, std::integral_constant<int, 0>{});
+        body(s + 1, std::integral_constant<int, 1>{});
     }
-    return abits;
 }
 
 template <int F, typename DT>
 __global__ __launch_bounds__(64) void k_tile_routes(PlanDev p, const uint16_t* __restrict__ B, int64_t ldb, int K,
                                                     float* __restrict__ C, int64_t ldc, uint32_t mask, int64_t grid) {
-    using X8 = typename DT::x8;
-    using X16 = typename DT::x16;
     constexpr int LDS_STRIDE = F + 4;     // keeps the 4 lane groups of an operand read on disjoint banks
     constexpr int NB = F / 16;
     __shared__ __align__(16) uint16_t Bs[64 * LDS_STRIDE];
 
     const int lane = threadIdx.x;
-    const int64_t it = xcd_window(blockIdx.x, grid);
+    const int64_t it = p.item_order ? xcd_window(blockIdx.x, grid) : (int64_t)blockIdx.x;
     if (it >= p.n_titems) return;
     const int4 item = p.titems[it];
     const int64_t w = item.x;
@@ -171,100 +322,45 @@ __global__ __launch_bounds__(64) void k_tile_routes(PlanDev p, const uint16_t* _
     if (!(mask & TRSPMM_ROUTE_RESIDUAL)) e1 = e0;
     const int r = lane & 15, kg = lane >> 4;
     const int nts = (s1 - s0 + 3) / 4, ntt = (t1 - t0 + 3) / 4, nte = (e1 - e0 + 31) / 32;
-    const int nsteps = nts + ntt + nte;
     const int64_t row0 = w * 16;
     const uint16_t* bp = Bs + (kg * 8 + (r >> 2)) * LDS_STRIDE + 4 * (r & 3);
 
-    for (int f0 = 0; f0 < K; f0 += F) {
-        const int fw = K - f0 < F ? K - f0 : F;
-        f32x4 acc[NB];
+    // a pass covers up to two slabs of F features (the step loops run over both,
+    // so every step's metadata is read once per pass); acc[0] / acc[1] hold
+    // slabs 0 / 1, or two partial sums of slab 0 when the pass has one slab
+    for (int fp = 0; fp < K; fp += 2 * F) {
+        const int ns = K - fp > F ? 2 : 1;
+        f32x4 acc[2][NB];
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-        Gather<F> g;
-        const int sp0 = nts, re0 = nts + ntt;      // first 2:4 step, first residual step
-        auto issue = [&](int step) {
-            if (step < sp0) {
-                const int32_t sb = s0 + 4 * step;
-                gather_rows<F, 32>(g, B, ldb, f0, fw, p.tc_cols + (int64_t)sb * 8, (s1 - sb) * 8 < 32 ? (s1 - sb) * 8 : 32,
-                                   lane);
-            } else if (step < re0) {
-                const int32_t t = t0 + 4 * (step - sp0);
-                gather_rows<F, 64>(g, B, ldb, f0, fw, p.sp_cols + (int64_t)t * 16, (t1 - t) * 16 < 64 ? (t1 - t) * 16 : 64,
-                                   lane);
-            } else if (step < nsteps) {
-                const int32_t eb = e0 + 32 * (step - re0);
-                gather_rows<F, 32, 2>(g, B, ldb, f0, fw, (const int32_t*)(p.res_rec + eb), e1 - eb < 32 ? e1 - eb : 32,
-                                      lane);
-            }
-        };
-        issue(0);
-        for (int step = 0; step < nsteps; ++step) {
-            __syncthreads();                       // previous step's LDS reads are done
-            if (step < sp0 || step >= re0) park_rows<F, 32, LDS_STRIDE>(g, Bs, lane);
-            else park_rows<F, 64, LDS_STRIDE>(g, Bs, lane);
-            issue(step + 1);                       // next step's gathers overlap this step's math
-            if (step < sp0 || step >= re0) {
-                u16x8 abits;
-                if (step < sp0) {
-                    abits = tc_fragment(p, s0 + 4 * step + kg, s1, r);
-                } else {
-                    // residual entries as a 16x32 one-hot fragment: staged row k = entry k, and
-                    // A[r][k] = value of entry k if its local row is r (records: col | val << 32 | lrow << 48)
-                    const int32_t eb = e0 + 32 * (step - re0) + 8 * kg;
+        for (int sl = 0; sl < 2; ++sl)
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) {
-                        const int32_t e = eb + q < e1 ? eb + q : e1 - 1;
-                        const uint64_t rec = p.res_rec[e];
-                        const bool hit = eb + q < e1 && (int)((rec >> 48) & 0xffu) == r && (int32_t)rec >= 0;
-                        abits[q] = hit ? (uint16_t)(rec >> 32) : (uint16_t)0;
-                    }
-                }
-                const X8 a = __builtin_bit_cast(X8, abits);
-                __syncthreads();
+            for (int nb = 0; nb < NB; ++nb) acc[sl][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const uint32_t ldb32 = (uint32_t)ldb;
+        if (nts > 0) run_steps<0, F, DT, LDS_STRIDE>(p, B, ldb32, K, fp, ns, s0, s1, nts, Bs, bp, lane, acc);
+        if (ntt > 0) run_steps<1, F, DT, LDS_STRIDE>(p, B, ldb32, K, fp, ns, t0, t1, ntt, Bs, bp, lane, acc);
+        if (nte > 0) run_steps<2, F, DT, LDS_STRIDE>(p, B, ldb32, K, fp, ns, e0, e1, nte, Bs, bp, lane, acc);
+        if (ns == 1) {
+            // one slab: even and odd sub-steps summed into acc[0]
 #pragma unroll
-                for (int nb = 0; nb < NB; ++nb) {
-                    // B operand of v_mfma_f32_16x16x32: lane (kg, n) holds staged rows 8kg..8kg+7, feature n
-                    const s16x4 b0 = lds_tr16(bp + nb * 16), b1 = lds_tr16(bp + 4 * LDS_STRIDE + nb * 16);
-                    const s16x8 bb = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
-                    acc[nb] = mma_dense<DT>(a, __builtin_bit_cast(X8, bb), acc[nb]);
-                }
-            } else {
-                const int32_t my = t0 + 4 * (step - sp0) + kg;
-                u32x4 av = {0u, 0u, 0u, 0u};
-                int idx = 0x4444;                  // (0,1) in every group: valid, multiplied by zeros
-                if (my < t1) {
-                    av = *(const u32x4*)(p.sp_a + ((int64_t)my * 16 + r) * 4);
-                    idx = p.sp_idx[(int64_t)my * 16 + r];
-                }
-                const X8 a = __builtin_bit_cast(X8, av);
-                __syncthreads();
-#pragma unroll
-                for (int nb = 0; nb < NB; ++nb) {
-                    // B operand of v_smfmac_f32_16x16x64 (measured lane layout): slot q of lane
-                    // (kg, n) holds staged row 32*(q/8) + 8*kg + q%8 of feature n, where staged
-                    // row 16t + 4g + p is column p of group g of the t-th tile (the A layout:
-                    // lane (kg, r) = tile kg, row r, groups 0..3, nibble g = p0 | p1 << 2).
-                    const s16x4 b0 = lds_tr16(bp + nb * 16), b1 = lds_tr16(bp + 4 * LDS_STRIDE + nb * 16);
-                    const s16x4 b2 = lds_tr16(bp + 32 * LDS_STRIDE + nb * 16);
-                    const s16x4 b3 = lds_tr16(bp + 36 * LDS_STRIDE + nb * 16);
-                    const s16x16 bb = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3],
-                                       b2[0], b2[1], b2[2], b2[3], b3[0], b3[1], b3[2], b3[3]};
-                    acc[nb] = mma_sparse<DT>(a, __builtin_bit_cast(X16, bb), acc[nb], idx);
-                }
-            }
+            for (int nb = 0; nb < NB; ++nb) acc[0][nb] += acc[1][nb];
         }
         // accumulator element i of lane (kg, n) = C[16w + 4kg + i, f0 + 16nb + n]
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            const int f = nb * 16 + r;
+        for (int sl = 0; sl < 2; ++sl) {
+            if (sl >= ns) continue;
+            const int f0 = fp + sl * F;
+            const int fw = K - f0 < F ? K - f0 : F;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int64_t row = row0 + kg * 4 + i;
-                if (row < p.m_out && f < fw) {
-                    float* dst = C + row * ldc + f0 + f;
-                    if (!split) __builtin_nontemporal_store(acc[nb][i], dst);
-                    else if (acc[nb][i] != 0.f) unsafeAtomicAdd(dst, acc[nb][i]);
+            for (int nb = 0; nb < NB; ++nb) {
+                const int f = nb * 16 + r;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int64_t row = row0 + kg * 4 + i;
+                    if (row < p.m_out && f < fw) {
+                        float* dst = C + row * ldc + f0 + f;
+                        if (!split) __builtin_nontemporal_store(acc[sl][nb][i], dst);
+                        else if (acc[sl][nb][i] != 0.f) unsafeAtomicAdd(dst, acc[sl][nb][i]);
+                    }
                 }
             }
         }
@@ -301,7 +397,7 @@ __global__ __launch_bounds__(256) void k_residual(PlanDev p, const uint16_t* __r
     float* tile = tile_all[wv];
     int32_t* rp = rp_all[wv];
     // blocks (4 items each) are what the dispatcher spreads over the XCDs
-    const int64_t it = xcd_window(blockIdx.x, grid) * 4 + wv;
+    const int64_t it = (p.item_order ? xcd_window(blockIdx.x, grid) : (int64_t)blockIdx.x) * 4 + wv;
     if (it >= p.n_ritems) return;                       // the whole wave leaves together (no block barriers below)
     const int4 item = p.ritems[it];
     const int64_t w = item.x;

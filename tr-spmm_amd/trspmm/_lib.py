@@ -69,7 +69,9 @@ class MetaView(ctypes.Structure):
 
 
 class PlanOpts(ctypes.Structure):
-    _fields_ = [("residual_engine", ctypes.c_int32), ("slab", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
+    _fields_ = [("residual_engine", ctypes.c_int32), ("slab", ctypes.c_int32), ("tile_cap", ctypes.c_int32),
+                ("tc_cap", ctypes.c_int32), ("res_cap", ctypes.c_int32), ("item_order", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 2)]
 
 
 class TrspmmError(RuntimeError):

@@ -69,13 +69,13 @@ def concat_block_diagonal(graphs):
 
 class RoutedBatch:
     def __init__(self, graphs, *, mode: str = "exact", dtype: torch.dtype = torch.float16, engine=None, slab=None,
-                 **router_kwargs):
+                 plan_opts: dict | None = None, **router_kwargs):
         rowptr, col, val, self.row_off, self.col_off, self.m_list, self.n_list = concat_block_diagonal(graphs)
         self.count = len(self.m_list)
         self.dtype = dtype
         self.A = RoutedSparse(torch.from_numpy(rowptr), torch.from_numpy(col), torch.from_numpy(val),
                               int(self.col_off[-1]), int(self.row_off[-1]), mode=mode, dtype=dtype, engine=engine,
-                              slab=slab, **router_kwargs)
+                              slab=slab, plan_opts=plan_opts, **router_kwargs)
 
     @property
     def rows(self) -> int:

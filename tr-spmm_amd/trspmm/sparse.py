@@ -38,7 +38,7 @@ class _BwdPlan:
 class RoutedSparse:
     def __init__(self, rowptr, col, val, ncols: int, m: int | None = None, *, mode: str = "exact",
                  dtype: torch.dtype = torch.float16, engine: str | None = None, slab: int | None = None,
-                 **router_kwargs):
+                 plan_opts: dict | None = None, **router_kwargs):
         rowptr = torch.as_tensor(rowptr)
         col = torch.as_tensor(col)
         val = torch.as_tensor(val)
@@ -54,7 +54,8 @@ class RoutedSparse:
         self.window = int(router_kwargs.get("window", 16))
         self.meta = route(rowptr, col, val, self.ncols, mode=mode, **router_kwargs)
         self.engine, self.slab = engine, slab
-        self.plan = Plan.from_meta(self.meta, self.m, self.ncols, dtype, engine=engine, slab=slab)
+        self.plan_opts = dict(plan_opts or {})      # tile_cap / tc_cap / res_cap / item_order
+        self.plan = Plan.from_meta(self.meta, self.m, self.ncols, dtype, engine=engine, slab=slab, **self.plan_opts)
         # device CSR (validated by the router): the SDDMM pattern and the
         # source of the transposed plan
         dev = torch.device("cuda", torch.cuda.current_device())
