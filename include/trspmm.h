@@ -217,6 +217,21 @@ void trspmm_bwd_plan_free(trspmm_bwd_plan* plan);
 int trspmm_spmm_t_sddmm(const trspmm_bwd_plan* plan, const void* dY, int64_t lddy, const void* B,
                         int64_t ldb, int64_t K, float* dB, int64_t lddb, float* dvals, void* stream);
 
+/* ------------------------------------------------------------------ matching statistics
+ * Replaces matching_utils.match_2to4 (setup_24matching op,
+ * TR-source/Block/block_matching_window_para_dr.cpp:346-419): statistics of
+ * the sliding-window 1:2 / 2:4 matching heuristic with dense-pool routing
+ * over the 16-bit row masks of each window.  Host memory, host threads
+ * (threads <= 0: up to 16).  values are not read (as in the reference). */
+typedef struct trspmm_match_stats {
+    int64_t total_rows, total_cols, total_nnz;
+    double sparsity;
+    int64_t num_row_panels, num_sptc_groups, num_tc_blocks, num_16x16_blocks, block_padding_groups;
+    int64_t fine_fallback, coarse_fallback, fake_zeros;
+} trspmm_match_stats;
+int trspmm_match_2to4(const int32_t* rowptr, const int32_t* col, int64_t rows, int64_t nnz, int window,
+                      int dense_threshold, int t_max, int threads, trspmm_match_stats* out);
+
 /* ------------------------------------------------------------------ diagnostics */
 
 /* libstdc++ std::sort order of iota(n) by key descending, computed by the

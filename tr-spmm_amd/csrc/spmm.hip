@@ -345,21 +345,50 @@ __global__ __launch_bounds__(64) void k_tile_routes(PlanDev p, const uint16_t* _
             for (int nb = 0; nb < NB; ++nb) acc[0][nb] += acc[1][nb];
         }
         // accumulator element i of lane (kg, n) = C[16w + 4kg + i, f0 + 16nb + n]
+        if (!split) {
+            // full-line stores: the tile goes through LDS (the staging buffer is free
+            // now) and every lane writes 16-byte pieces of whole rows, instead of the
+            // 64-byte row fragments of the accumulator layout
+            float* T = reinterpret_cast<float*>(Bs);       // [16][2F] fp32 = 8 KB for F = 64
+            constexpr int TW = 2 * F;
+            static_assert(16 * TW * 4 <= 64 * LDS_STRIDE * 2, "tile must fit the staging buffer");
+            __syncthreads();                                // the last sub-step's LDS reads are done
 #pragma unroll
-        for (int sl = 0; sl < 2; ++sl) {
-            if (sl >= ns) continue;
-            const int f0 = fp + sl * F;
-            const int fw = K - f0 < F ? K - f0 : F;
+            for (int sl = 0; sl < 2; ++sl) {
+                if (sl >= ns) continue;
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-                const int f = nb * 16 + r;
+                for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int64_t row = row0 + kg * 4 + i;
-                    if (row < p.m_out && f < fw) {
-                        float* dst = C + row * ldc + f0 + f;
-                        if (!split) __builtin_nontemporal_store(acc[sl][nb][i], dst);
-                        else if (acc[sl][nb][i] != 0.f) unsafeAtomicAdd(dst, acc[sl][nb][i]);
+                    for (int i = 0; i < 4; ++i) T[(kg * 4 + i) * TW + sl * F + nb * 16 + r] = acc[sl][nb][i];
+            }
+            __syncthreads();
+            const int width = K - fp < ns * F ? K - fp : ns * F;     // features of this pass (multiple of 8)
+            const int chunks = width / 4;                            // 16-byte pieces per row
+#pragma unroll
+            for (int k = 0; k < (16 * TW / 4) / 64; ++k) {
+                const int c = lane + 64 * k;
+                const int row = c / (TW / 4), piece = c % (TW / 4);
+                const int64_t grow = row0 + row;
+                if (piece < chunks && grow < p.m_out) {
+                    const f32x4 x = *(const f32x4*)(T + row * TW + 4 * piece);
+                    __builtin_nontemporal_store(x, (f32x4*)(C + grow * ldc + fp + 4 * piece));
+                }
+            }
+            __syncthreads();                                // T is staging space again for the next pass
+        } else {
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                if (sl >= ns) continue;
+                const int f0 = fp + sl * F;
+                const int fw = K - f0 < F ? K - f0 : F;
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) {
+                    const int f = nb * 16 + r;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int64_t row = row0 + kg * 4 + i;
+                        if (row < p.m_out && f < fw && acc[sl][nb][i] != 0.f)
+                            unsafeAtomicAdd(C + row * ldc + f0 + f, acc[sl][nb][i]);
                     }
                 }
             }

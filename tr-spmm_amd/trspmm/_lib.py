@@ -24,7 +24,7 @@ EXPORTS = [
     "trspmm_plan_create", "trspmm_plan_from_meta", "trspmm_plan_create_packed", "trspmm_plan_free",
     "trspmm_plan_stats", "trspmm_spmm", "trspmm_spmm_timed", "trspmm_memcpy", "trspmm_host_sort_order",
     "trspmm_csr_transpose", "trspmm_sddmm", "trspmm_bwd_plan_create", "trspmm_bwd_plan_free",
-    "trspmm_spmm_t_sddmm", "trspmm_plan_create_ex", "trspmm_plan_from_meta_ex",
+    "trspmm_spmm_t_sddmm", "trspmm_plan_create_ex", "trspmm_plan_from_meta_ex", "trspmm_match_2to4",
 ]
 
 ENGINE_DEFAULT, ENGINE_VALU, ENGINE_MFMA = 0, 1, 2
@@ -74,6 +74,13 @@ class PlanOpts(ctypes.Structure):
                 ("reserved", ctypes.c_int32 * 2)]
 
 
+class MatchStats(ctypes.Structure):
+    _fields_ = [("total_rows", _I64), ("total_cols", _I64), ("total_nnz", _I64), ("sparsity", ctypes.c_double),
+                ("num_row_panels", _I64), ("num_sptc_groups", _I64), ("num_tc_blocks", _I64),
+                ("num_16x16_blocks", _I64), ("block_padding_groups", _I64), ("fine_fallback", _I64),
+                ("coarse_fallback", _I64), ("fake_zeros", _I64)]
+
+
 class TrspmmError(RuntimeError):
     """A libtrspmm call returned an error status."""
 
@@ -119,6 +126,8 @@ def lib():
     L.trspmm_bwd_plan_create.argtypes = [_P, _P, _P, _P, _I64, _I64, _I64, ctypes.c_int, _P, ctypes.POINTER(_P)]
     L.trspmm_bwd_plan_free.argtypes = [_P]
     L.trspmm_spmm_t_sddmm.argtypes = [_P, _P, _I64, _P, _I64, _I64, _P, _I64, _P, _P]
+    L.trspmm_match_2to4.argtypes = [_P, _P, _I64, _I64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(MatchStats)]
     _LIB = L
     return L
 

@@ -8,7 +8,9 @@ The reference fixes its choices at build time (TC width 8, density 8, one
             become dense 16x8 sub-blocks)
   kernels   residual engine (vector ALU / matrix cores), tile-kernel
             features per pass (64 / 128), work-item cap (a window with more
-            2:4 tiles / TC sub-blocks is split into chunks of this many)
+            2:4 tiles / TC sub-blocks is split into chunks of this many), work
+            item order (round robin over the XCDs, or contiguous ranges per
+            XCD, which keeps a small graph's operand rows in one L2)
 The window height stays 16: the MFMA/SMFMAC tiles are 16 rows tall.  Each
 router configuration is routed once and every kernel variant gets its own
 plan; each candidate is timed with HIP events over `iters` launches after a
@@ -30,8 +32,8 @@ ROUTER_CANDIDATES = [
     {"mode": "tuned", "density": 8, "sptc_min_nnz": 16},
     {"mode": "exact", "density": 6},
 ]
-KERNEL_CANDIDATES = [{"engine": e, "slab": s, "tile_cap": c}
-                     for e, s, c in itertools.product(["valu", "mfma"], [64, 128], [32, 64, 128])]
+KERNEL_CANDIDATES = [{"engine": e, "slab": s, "tile_cap": c, "item_order": o}
+                     for e, s, c, o in itertools.product(["valu", "mfma"], [64, 128], [32, 64, 128], [0, 1])]
 
 
 @dataclass
@@ -41,11 +43,11 @@ class TuneResult:
     table: list = field(default_factory=list)     # (candidate, ms)
 
     def router_kwargs(self) -> dict:
-        return {k: v for k, v in self.best.items() if k not in ("engine", "slab", "tile_cap")}
+        return {k: v for k, v in self.best.items() if k not in ("engine", "slab", "tile_cap", "item_order")}
 
     def plan_opts(self) -> dict:
         c = self.best.get("tile_cap", 0)
-        return {"tile_cap": c, "tc_cap": c}
+        return {"tile_cap": c, "tc_cap": c, "item_order": self.best.get("item_order", 0)}
 
 
 def autotune(rowptr, col, val, ncols: int, m: int, B: torch.Tensor, *, dtype=torch.float16,
@@ -61,7 +63,8 @@ def autotune(rowptr, col, val, ncols: int, m: int, B: torch.Tensor, *, dtype=tor
         meta = route(rowptr, col, val, ncols, mode=mode, **kw)
         for kc in kernel_candidates:
             cap = kc.get("tile_cap", 0)
-            plan = Plan.from_meta(meta, m, ncols, dtype, engine=kc["engine"], slab=kc["slab"], tile_cap=cap, tc_cap=cap)
+            plan = Plan.from_meta(meta, m, ncols, dtype, engine=kc["engine"], slab=kc["slab"], tile_cap=cap, tc_cap=cap,
+                                  item_order=kc.get("item_order", 0))
             ms = plan.timed(B, C, warmup=warmup, iters=iters)
             table.append(({**rc, **kc}, ms))
             del plan
