@@ -232,6 +232,22 @@ typedef struct trspmm_match_stats {
 int trspmm_match_2to4(const int32_t* rowptr, const int32_t* col, int64_t rows, int64_t nnz, int window,
                       int dense_threshold, int t_max, int threads, trspmm_match_stats* out);
 
+/* Replaces matching_utils_hash.match_2to4_hash
+ * (TR-source/Block/block_matching_hash.cpp:104-441): exact-complement pairs
+ * through per-mask chains first, then the sliding-window search with t_max 8;
+ * no dense pool (num_tc_blocks is 0, num_sptc_groups is the reference's
+ * num_groups).  Host memory, host threads. */
+int trspmm_match_2to4_hash(const int32_t* rowptr, const int32_t* col, int64_t rows, int64_t nnz, int window,
+                           int threads, trspmm_match_stats* out);
+
+/* Replaces matching_utils_bucket.match_2to4_bucket
+ * (TR-source/Block/block_matching_bucket.cpp:72-498): pairs across
+ * top-heavy / bottom-heavy / balanced buckets of the columns; no dense pool
+ * (num_tc_blocks is 0, num_sptc_groups is the reference's num_groups).  Host
+ * memory, host threads (the reference runs it on one thread). */
+int trspmm_match_2to4_bucket(const int32_t* rowptr, const int32_t* col, int64_t rows, int64_t nnz, int window,
+                             int threads, trspmm_match_stats* out);
+
 /* ------------------------------------------------------------------ diagnostics */
 
 /* libstdc++ std::sort order of iota(n) by key descending, computed by the

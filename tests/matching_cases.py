@@ -14,3 +14,6 @@ PARAMS = [
     {"t_max": 0},
     {"dense_threshold": 16, "t_max": 16},
 ]
+
+# window heights for the variants that take only window_size
+WINDOWS = [16, 8, 20, 1, 64]

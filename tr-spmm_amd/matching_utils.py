@@ -25,25 +25,40 @@ KEYS = ["total_rows", "total_cols", "total_nnz", "sparsity", "num_row_panels", "
         "num_16x16_blocks", "block_padding_groups", "fine_fallback", "coarse_fallback", "fake_zeros"]
 
 
-def match_2to4(row_ptr, col_ind, values, window_size=16, dense_threshold=8, t_max=8, *, threads=0,
-               verbose=False):
+def _arrays(row_ptr, col_ind, values, window_size):
+    """C-contiguous int32 row_ptr / col_ind (values force-cast and dropped) and
+    the reference's argument checks."""
     rp = np.ascontiguousarray(np.asarray(row_ptr), dtype=np.int32)
     ci = np.ascontiguousarray(np.asarray(col_ind), dtype=np.int32)
     np.ascontiguousarray(np.asarray(values), dtype=np.float32)      # forcecast, as the reference binding does
     if not 1 <= int(window_size) <= 64:
         raise RuntimeError("window_size must be in [1, 64]")
-    rows = rp.size - 1
-    if rows <= 0 or ci.size == 0 or int(ci.max()) < 0:
+    if rp.size - 1 <= 0 or ci.size == 0 or int(ci.max()) < 0:
         raise RuntimeError("invalid matrix dimensions")
+    return rp, ci
+
+
+def _stats(fn, rp, ci, *args, keys=KEYS, rename=None, verbose=False):
+    """Calls a trspmm_match_2to4* entry point and returns its dict; `rename`
+    maps struct fields to the op's own key names."""
     st = _lib.MatchStats()
-    rc = lib().trspmm_match_2to4(rp.ctypes.data_as(ctypes.c_void_p), ci.ctypes.data_as(ctypes.c_void_p), rows,
-                                 ci.size, int(window_size), int(dense_threshold), int(t_max), int(threads),
-                                 ctypes.byref(st))
+    rc = fn(rp.ctypes.data_as(ctypes.c_void_p), ci.ctypes.data_as(ctypes.c_void_p), rp.size - 1, ci.size, *args,
+            ctypes.byref(st))
     if rc != _lib.OK:
         raise RuntimeError(lib().trspmm_last_error().decode())
-    out = {k: getattr(st, k) for k in KEYS}
+    rename = rename or {}
+    out = {}
+    for k in keys:
+        out[k] = getattr(st, next((f for f, name in rename.items() if name == k), k))
     out["sparsity"] = float(out["sparsity"])
     if verbose:
-        for k in KEYS:
+        for k in keys:
             print(f"{k:22s} {out[k]}")
     return out
+
+
+def match_2to4(row_ptr, col_ind, values, window_size=16, dense_threshold=8, t_max=8, *, threads=0,
+               verbose=False):
+    rp, ci = _arrays(row_ptr, col_ind, values, window_size)
+    return _stats(lib().trspmm_match_2to4, rp, ci, int(window_size), int(dense_threshold), int(t_max), int(threads),
+                  verbose=verbose)

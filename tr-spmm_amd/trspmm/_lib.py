@@ -25,6 +25,7 @@ EXPORTS = [
     "trspmm_plan_stats", "trspmm_spmm", "trspmm_spmm_timed", "trspmm_memcpy", "trspmm_host_sort_order",
     "trspmm_csr_transpose", "trspmm_sddmm", "trspmm_bwd_plan_create", "trspmm_bwd_plan_free",
     "trspmm_spmm_t_sddmm", "trspmm_plan_create_ex", "trspmm_plan_from_meta_ex", "trspmm_match_2to4",
+    "trspmm_match_2to4_hash", "trspmm_match_2to4_bucket",
 ]
 
 ENGINE_DEFAULT, ENGINE_VALU, ENGINE_MFMA = 0, 1, 2
@@ -128,6 +129,8 @@ def lib():
     L.trspmm_spmm_t_sddmm.argtypes = [_P, _P, _I64, _P, _I64, _I64, _P, _I64, _P, _P]
     L.trspmm_match_2to4.argtypes = [_P, _P, _I64, _I64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.POINTER(MatchStats)]
+    for fn in (L.trspmm_match_2to4_hash, L.trspmm_match_2to4_bucket):
+        fn.argtypes = [_P, _P, _I64, _I64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(MatchStats)]
     _LIB = L
     return L
 
