@@ -21,11 +21,15 @@
  *                            single-owner output rows instead of 4 streams + add)
  *   trspmm_spmm_timed     <- the warm-up / timed-epoch loop of the same launcher
  *                            (mGCNkernel_online.cu:1604-1704) and its ms outputs
- * Entry points with no reference counterpart (the reference has no backward,
- * no batching and no device router):
+ *   trspmm_match_2to4, _hash, _bucket, _online
+ *                         <- the setup_24matching statistics ops
+ *                            matching_utils{,_hash,_bucket,_online}.match_2to4*
+ *                            (TR-source/Block/block_matching_*.cpp / .cu)
+ * Entry points with no reference counterpart (the reference has no backward
+ * and no device router):
  *   trspmm_csr_transpose, trspmm_sddmm, trspmm_bwd_plan_*, trspmm_spmm_t_sddmm
- *   (backward pass), trspmm_batch_* (many
- *   graphs in one launch), trspmm_memcpy, trspmm_host_sort_order (diagnostics).
+ *   (backward pass), trspmm_plan_*_ex (plan options), trspmm_memcpy,
+ *   trspmm_host_sort_order (diagnostics).
  */
 #ifndef TRSPMM_H
 #define TRSPMM_H
@@ -247,6 +251,28 @@ int trspmm_match_2to4_hash(const int32_t* rowptr, const int32_t* col, int64_t ro
  * memory, host threads (the reference runs it on one thread). */
 int trspmm_match_2to4_bucket(const int32_t* rowptr, const int32_t* col, int64_t rows, int64_t nnz, int window,
                              int threads, trspmm_match_stats* out);
+
+/* Replaces matching_utils_online.match_2to4_online
+ * (TR-source/Block/block_matching_online.cu:85-649): the match_2to4 heuristic
+ * on the device, one workgroup per 16-row window, columns taken in the order
+ * of a 2048-slot table (slot = column mod 2048, linear probing; where columns
+ * share a slot, the order sequential insertion by ascending column gives),
+ * 32 - nnz padding zeros per group.  rowptr [rows+1] and col [nnz] are DEVICE
+ * pointers; counters is a device buffer of TRSPMM_ONLINE_COUNTERS unsigned
+ * 64-bit values, zeroed and then filled on `stream` (asynchronous):
+ *   [0] num_sptc_groups   [1] num_tc_blocks     [2] dense_to_tc_cols
+ *   [3] dense_to_sptc_cols [4] fine_fb          [5] coarse_fb_cols
+ *   [6] num_16x16_blocks  [7] block_padding_groups [8] fake_zeros
+ *   [9] windows with more than 2048 distinct columns (each counted above
+ *       with its 2048 smallest column ids, the columns sequential insertion
+ *       in ascending order keeps; the reference drops the columns that find
+ *       no slot, which ones depending on scheduling)
+ *   [10] windows holding a negative column index (not counted above)
+ * Row offsets are clamped to [0, nnz], so a malformed rowptr cannot read
+ * outside col. */
+#define TRSPMM_ONLINE_COUNTERS 11
+int trspmm_match_2to4_online(const int32_t* rowptr, const int32_t* col, int64_t rows, int64_t nnz,
+                             int dense_threshold, int t_max, unsigned long long* counters, void* stream);
 
 /* ------------------------------------------------------------------ diagnostics */
 

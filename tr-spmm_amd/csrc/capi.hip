@@ -10,6 +10,9 @@
 
 namespace trspmm {
 
+int match_online_device(const int32_t* d_rowptr, const int32_t* d_col, int64_t rows, int64_t nnz,
+                        int dense_threshold, int t_max, unsigned long long* d_out, hipStream_t stream);
+
 static thread_local std::string g_error;
 void set_error(const std::string& msg) { g_error = msg; }
 const char* last_error() { return g_error.c_str(); }
@@ -277,6 +280,15 @@ int trspmm_host_sort_order(const int32_t* key, int32_t n, int32_t* order) {
     lsort::sort(a.data(), n, lsort::KeyDesc{}, stack);
     for (int32_t i = 0; i < n; ++i) order[i] = (int32_t)(a[(size_t)i] & 0xffffffu);
     return TRSPMM_OK;
+}
+
+// ---------------------------------------------------------------- matching statistics (device)
+int trspmm_match_2to4_online(const int32_t* d_rowptr, const int32_t* d_col, int64_t rows, int64_t nnz,
+                             int dense_threshold, int t_max, unsigned long long* d_counters, void* stream) {
+    TRSPMM_NOTNULL(d_rowptr, "rowptr");
+    TRSPMM_NOTNULL(d_counters, "counters");
+    TRSPMM_CHECK(nnz == 0 || d_col, "col must not be null when nnz > 0");
+    return match_online_device(d_rowptr, d_col, rows, nnz, dense_threshold, t_max, d_counters, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -1,0 +1,410 @@
+// Device statistics of the reference's matching_utils_online.match_2to4_online
+// (TR-source/Block/block_matching_online.cu:85-649): the sliding-window 1:2 /
+// 2:4 matching with dense-pool routing, computed on the GPU, one 64-lane
+// workgroup per 16-row window.
+//
+// Per window (rows r0 .. r0+15):
+//   1. table: the reference puts the window's distinct columns into a
+//      2048-slot table at slot col mod 2048 with linear probing, each with
+//      the 16-bit mask of the window rows it touches, and takes them in slot
+//      order.  It inserts from all threads at once, so when two columns of a
+//      window share a slot the order depends on thread scheduling.  Here the
+//      order is fixed as the one sequential insertion in ascending column
+//      order gives (one of the reference's possible outcomes); without such a
+//      collision every column sits at its own slot and the order is the
+//      reference's, whatever the scheduling.  The columns are collected in a
+//      scattered-hash table of 4096 slots (at most half full, so probes stay
+//      short whatever the column ids), then laid out: each at its home slot
+//      when no two share one, else by a union-find "next free slot" pass
+//      over the columns in ascending order (the sequential insertion without
+//      its probe walks);
+//   2. 1:2 pairs: each unpaired column i takes, among columns i+1 .. i+32 not
+//      yet paired, the first with no common row, else the first with the
+//      fewest common rows if <= t_max; else it goes to the pool (fine);
+//   3. 2:4 groups: each unused pair takes the first unused pair among the next
+//      1024 whose rows do not conflict ((both_i & any_j) | (any_i & both_j)
+//      == 0); a group adds 32 - nnz padding zeros; a pair with no partner
+//      sends both columns to the pool (coarse 2);
+//   4. the pool, in order: blocks of dense_threshold columns (TC), the rest
+//      two at a time in padded 2:4 groups (32 - nnz zeros);
+//   5. 2:4 groups packed four to a 16x16 block (ceil), shortfall = padding.
+// A window with more than 2048 distinct columns overflows the table: the
+// reference then drops the columns that find no slot, which ones depending on
+// scheduling; here the window keeps its 2048 smallest column ids (what
+// sequential insertion in ascending order keeps) and is counted in the
+// overflow counter.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.h"
+
+namespace trspmm {
+
+namespace {
+
+constexpr int OM_ROWS = 16;           // window height (fixed, as in the reference kernel)
+constexpr int OM_SLOTS = 2048;        // the reference's table slots per window
+constexpr int OM_DEDUP = 4096;        // collection table slots (at most half full)
+constexpr int OM_PAIR_SEARCH = 32;    // columns looked ahead for a 1:2 partner
+constexpr int OM_GROUP_SEARCH = 1024; // pairs looked ahead for a 2:4 partner
+constexpr int OM_WAVE = 64;
+constexpr uint16_t OM_FREE = 0xFFFF;
+
+struct OnlineLds {
+    union {
+        struct {                          // collection: column id (-1 free) and row mask
+            int32_t id[OM_DEDUP];
+            uint32_t mask[OM_DEDUP];
+        } d;
+        struct {                          // layout in the reference's table, then its order
+            uint32_t home_cnt[OM_SLOTS];  // columns whose home is this slot
+            uint16_t owner[OM_SLOTS];     // column (index into keys) in this slot, OM_FREE if none
+            uint16_t parent[OM_SLOTS];    // union-find: next slot that may be free
+            uint16_t cmask[OM_SLOTS];     // row masks in table order
+        } t;
+    } a;
+    union {
+        uint64_t keys[OM_SLOTS];          // collected columns: (column id << 32) | row mask
+        struct {
+            uint16_t g1[OM_SLOTS / 2], g2[OM_SLOTS / 2];   // pair: rows both / either columns touch
+            uint16_t c1[OM_SLOTS / 2], c2[OM_SLOTS / 2];   // pair: column positions in table order
+            uint16_t pool[OM_SLOTS];
+            uint8_t used[OM_SLOTS / 2];
+            uint8_t matched[OM_SLOTS];
+        } p;
+    } b;
+};
+
+struct OnlineCtl {
+    int32_t rp[OM_ROWS + 1];      // the window's row offsets, clamped to [0, nnz]
+    int32_t occ;                  // distinct columns collected
+    int32_t stop;                 // the collection reached its cap
+    int32_t collide;              // two collected columns share a home slot
+};
+
+__device__ inline int lane_id() { return (int)threadIdx.x; }
+
+// exclusive prefix sum over the wave; *total = sum of all lanes
+__device__ inline int wave_exclusive_scan(int v, int* total) {
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < OM_WAVE; d <<= 1) {
+        const int y = __shfl_up(x, d, OM_WAVE);
+        if (lane_id() >= d) x += y;
+    }
+    *total = __shfl(x, OM_WAVE - 1, OM_WAVE);
+    return x - v;
+}
+
+__device__ inline int wave_sum(int v) {
+#pragma unroll
+    for (int d = OM_WAVE / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, OM_WAVE);
+    return v;
+}
+
+__device__ inline int wave_max(int v) {
+#pragma unroll
+    for (int d = OM_WAVE / 2; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d, OM_WAVE));
+    return v;
+}
+
+// lowest lane whose predicate holds, or -1
+__device__ inline int first_lane(bool pred) {
+    const unsigned long long m = __ballot(pred);
+    return m ? (int)__builtin_ctzll(m) : -1;
+}
+
+__device__ inline int dedup_slot(int c) { return (int)(((uint32_t)c * 2654435761u) >> 20); }   // 12 bits
+
+// Collects the window's distinct columns in [0, limit] (and, with MASKS,
+// their row masks) into the collection table; stops once `cap` are in.
+// Returns the count (>= cap if it stopped), on every lane.
+template <bool MASKS>
+__device__ int collect(OnlineLds& s, OnlineCtl& ctl, const int32_t* __restrict__ col, int win_rows, int limit,
+                       int cap) {
+    __syncthreads();                       // earlier users of the shared memory are done
+    for (int t = lane_id(); t < OM_DEDUP; t += OM_WAVE) {
+        s.a.d.id[t] = -1;
+        if (MASKS) s.a.d.mask[t] = 0;
+    }
+    if (lane_id() == 0) {
+        ctl.occ = 0;
+        ctl.stop = 0;
+    }
+    __syncthreads();
+    auto stopped = [&]() { return __hip_atomic_load(&ctl.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0; };
+    for (int r = 0; r < win_rows && !stopped(); ++r) {
+        const int b = ctl.rp[r], end = max(b, ctl.rp[r + 1]);
+        for (int e = b + lane_id(); e < end; e += OM_WAVE) {
+            if (stopped()) break;
+            const int c = col[e];
+            if (c < 0 || c > limit) continue;
+            // at most cap + 63 <= OM_SLOTS + 64 columns are ever placed: a free slot exists
+            for (int slot = dedup_slot(c), probe = 0; probe < OM_DEDUP; ++probe, slot = (slot + 1) & (OM_DEDUP - 1)) {
+                const int prev = atomicCAS(&s.a.d.id[slot], -1, c);
+                if (prev == -1 || prev == c) {
+                    if (MASKS) atomicOr(&s.a.d.mask[slot], 1u << r);
+                    if (prev == -1 && atomicAdd(&ctl.occ, 1) + 1 >= cap)
+                        __hip_atomic_store(&ctl.stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    break;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    return ctl.occ;
+}
+
+__global__ __launch_bounds__(OM_WAVE) void k_match_online(const int32_t* __restrict__ rowptr,
+                                                          const int32_t* __restrict__ col, int64_t rows,
+                                                          int64_t nnz, int dense_threshold, int t_max,
+                                                          unsigned long long* __restrict__ out) {
+    __shared__ OnlineLds s;
+    __shared__ OnlineCtl ctl;
+    const int lane = lane_id();
+    const int64_t r0 = (int64_t)blockIdx.x * OM_ROWS;
+    const int win_rows = (int)min<int64_t>(OM_ROWS, rows - r0);
+
+    // ---- 1. the window's columns
+    if (lane <= OM_ROWS) {
+        const int64_t r = r0 + min(lane, win_rows);
+        ctl.rp[lane] = (int32_t)min<int64_t>(max<int64_t>(rowptr[r], 0), nnz);   // clamped: reads stay inside col
+    }
+    if (lane == 0) ctl.collide = 0;
+    __syncthreads();
+    // largest column id and any negative one, in one read of the window
+    const int e0 = ctl.rp[0], e1 = max(e0, ctl.rp[win_rows]);
+    int hi = -1, neg = 0;
+    for (int e = e0 + lane; e < e1; e += OM_WAVE) {
+        const int c = col[e];
+        hi = max(hi, c);
+        neg |= c < 0;
+    }
+    hi = wave_max(hi);
+    neg = wave_max(neg);
+    if (neg) {
+        if (lane == 0) atomicAdd(&out[10], 1ull);
+        return;
+    }
+    if (hi < 0) return;                                    // empty window
+    int n = collect<true>(s, ctl, col, win_rows, hi, OM_SLOTS + 1);
+    if (n > OM_SLOTS) {
+        // more distinct columns than the reference's table holds: keep the
+        // OM_SLOTS smallest column ids (what inserting in ascending order
+        // keeps).  Binary search for the largest kept id T: count(T) =
+        // OM_SLOTS, count(T - 1) < OM_SLOTS.
+        if (lane == 0) atomicAdd(&out[9], 1ull);
+        int lo = -1;                                       // count(lo) < OM_SLOTS <= count(hi)
+        while (hi - lo > 1) {
+            const int mid = lo + (hi - lo) / 2;
+            if (collect<false>(s, ctl, col, win_rows, mid, OM_SLOTS) >= OM_SLOTS) hi = mid;
+            else lo = mid;
+        }
+        n = collect<true>(s, ctl, col, win_rows, hi, OM_DEDUP + 1);   // exactly OM_SLOTS columns
+    }
+
+    // collected columns -> keys[0, n)
+    {
+        constexpr int PER = OM_DEDUP / OM_WAVE;
+        const int base = lane * PER;
+        int cnt = 0, total = 0;
+        for (int t = 0; t < PER; ++t) cnt += (s.a.d.id[base + t] != -1) ? 1 : 0;
+        int at = wave_exclusive_scan(cnt, &total);
+        for (int t = 0; t < PER; ++t)
+            if (s.a.d.id[base + t] != -1)
+                s.b.keys[at++] = ((uint64_t)(uint32_t)s.a.d.id[base + t] << 32) | s.a.d.mask[base + t];
+    }
+    __syncthreads();
+
+    // ---- layout in the reference's table
+    for (int t = lane; t < OM_SLOTS; t += OM_WAVE) {
+        s.a.t.home_cnt[t] = 0;
+        s.a.t.owner[t] = OM_FREE;
+        s.a.t.parent[t] = (uint16_t)t;
+    }
+    __syncthreads();
+    int clash = 0;
+    for (int k = lane; k < n; k += OM_WAVE)
+        clash |= atomicAdd(&s.a.t.home_cnt[(int)(s.b.keys[k] >> 32) & (OM_SLOTS - 1)], 1u) != 0;
+    clash = wave_max(clash);
+    if (!clash) {
+        for (int k = lane; k < n; k += OM_WAVE) s.a.t.owner[(int)(s.b.keys[k] >> 32) & (OM_SLOTS - 1)] = (uint16_t)k;
+    } else {
+        // ascending column order: bitonic sort of the keys
+        int P = 2;
+        while (P < n) P <<= 1;
+        for (int t = n + lane; t < P; t += OM_WAVE) s.b.keys[t] = ~0ull;
+        __syncthreads();
+        for (int k = 2; k <= P; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int t = lane; t < P / 2; t += OM_WAVE) {
+                    const int i = (t / j) * 2 * j + (t % j);
+                    const int q = i + j;
+                    const uint64_t x = s.b.keys[i], y = s.b.keys[q];
+                    if ((x > y) == ((i & k) == 0)) {
+                        s.b.keys[i] = y;
+                        s.b.keys[q] = x;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // each column takes the first free slot from its home on (cyclic):
+        // find() follows parent links with path halving; a filled slot links
+        // to the next one.  k < n <= OM_SLOTS columns are placed when column
+        // k is, so a free slot (a root) exists and find() ends.
+        if (lane == 0) {
+            for (int k = 0; k < n; ++k) {
+                int x = (int)(s.b.keys[k] >> 32) & (OM_SLOTS - 1);
+                while (s.a.t.parent[x] != x) {
+                    const int up = s.a.t.parent[x];
+                    s.a.t.parent[x] = s.a.t.parent[up];
+                    x = up;
+                }
+                s.a.t.owner[x] = (uint16_t)k;
+                s.a.t.parent[x] = (uint16_t)((x + 1) & (OM_SLOTS - 1));
+            }
+        }
+    }
+    __syncthreads();
+    // row masks in table order -> cmask[0, n)
+    {
+        constexpr int PER = OM_SLOTS / OM_WAVE;
+        const int base = lane * PER;
+        int cnt = 0, total = 0;
+        for (int t = 0; t < PER; ++t) cnt += (s.a.t.owner[base + t] != OM_FREE) ? 1 : 0;
+        int at = wave_exclusive_scan(cnt, &total);
+        for (int t = 0; t < PER; ++t) {
+            const int k = s.a.t.owner[base + t];
+            if (k != OM_FREE) s.a.t.cmask[at++] = (uint16_t)(s.b.keys[k] & 0xFFFFu);
+        }
+    }
+    __syncthreads();
+    const uint16_t* cmask = s.a.t.cmask;
+
+    // ---- 2. 1:2 pairs
+    for (int t = lane; t < n; t += OM_WAVE) s.b.p.matched[t] = 0;
+    __syncthreads();
+    int np = 0, nd = 0, fine = 0;
+    for (int i = 0; i < n; ++i) {
+        if (s.b.p.matched[i]) continue;          // same LDS byte on every lane: uniform
+        const uint32_t mi = cmask[i];
+        const int j = min(i + 1 + lane, n - 1);
+        const bool valid = lane < OM_PAIR_SEARCH && i + 1 + lane < n && !s.b.p.matched[j];
+        const int cost = valid ? __popc(mi & cmask[j]) : 64;
+        int best = -1;
+        for (int c = 0; c <= min(t_max, OM_ROWS); ++c) {
+            const int l = first_lane(cost == c);
+            if (l >= 0) {
+                best = i + 1 + l;
+                break;
+            }
+        }
+        if (lane == 0) {
+            s.b.p.matched[i] = 1;
+            if (best >= 0) {
+                s.b.p.matched[best] = 1;
+                s.b.p.g1[np] = (uint16_t)(mi & cmask[best]);
+                s.b.p.g2[np] = (uint16_t)(mi | cmask[best]);
+                s.b.p.c1[np] = (uint16_t)i;
+                s.b.p.c2[np] = (uint16_t)best;
+            } else {
+                s.b.p.pool[nd] = (uint16_t)i;
+            }
+        }
+        if (best >= 0) {
+            ++np;
+        } else {
+            ++nd;
+            ++fine;
+        }
+        __syncthreads();
+    }
+
+    // ---- 3. 2:4 groups
+    for (int t = lane; t < np; t += OM_WAVE) s.b.p.used[t] = 0;
+    __syncthreads();
+    int groups = 0, coarse = 0, fake = 0;
+    for (int i = 0; i < np; ++i) {
+        if (s.b.p.used[i]) continue;
+        const uint32_t a1 = s.b.p.g1[i], a2 = s.b.p.g2[i];
+        int best = -1;
+        for (int step = 0; step < OM_GROUP_SEARCH; step += OM_WAVE) {
+            const int j = i + 1 + step + lane;
+            bool ok = j < np;
+            if (ok) ok = !s.b.p.used[j] && ((a1 & s.b.p.g2[j]) | (a2 & s.b.p.g1[j])) == 0;
+            const int l = first_lane(ok);
+            if (l >= 0) {
+                best = i + 1 + step + l;
+                break;
+            }
+            if (i + 1 + step + OM_WAVE >= np) break;
+        }
+        if (best >= 0) {
+            ++groups;
+            fake += 32 - (__popc(cmask[s.b.p.c1[i]]) + __popc(cmask[s.b.p.c2[i]]) + __popc(cmask[s.b.p.c1[best]]) +
+                          __popc(cmask[s.b.p.c2[best]]));
+        } else {
+            coarse += 2;
+        }
+        __syncthreads();   // every lane has read used[] before lane 0 writes
+        if (lane == 0) {
+            s.b.p.used[i] = 1;
+            if (best >= 0) {
+                s.b.p.used[best] = 1;
+            } else {
+                s.b.p.pool[nd] = s.b.p.c1[i];
+                s.b.p.pool[nd + 1] = s.b.p.c2[i];
+            }
+        }
+        if (best < 0) nd += 2;
+        __syncthreads();
+    }
+
+    // ---- 4. the pool
+    const int tc = nd / dense_threshold;
+    const int processed = tc * dense_threshold;
+    const int residue = nd - processed;
+    const int rgroups = (residue + 1) / 2;
+    int rnnz = 0;
+    for (int k = processed + lane; k < nd; k += OM_WAVE) rnnz += __popc(cmask[s.b.p.pool[k]]);
+    rnnz = wave_sum(rnnz);
+    fake += 32 * rgroups - rnnz;
+    const int sptc = groups + rgroups;
+    const int blocks = (sptc + 3) / 4;
+
+    // ---- 5. totals
+    if (lane == 0) {
+        const unsigned long long v[9] = {(unsigned long long)sptc, (unsigned long long)tc,
+                                         (unsigned long long)processed, (unsigned long long)residue,
+                                         (unsigned long long)fine, (unsigned long long)coarse,
+                                         (unsigned long long)blocks, (unsigned long long)(blocks * 4 - sptc),
+                                         (unsigned long long)fake};
+        for (int k = 0; k < 9; ++k)
+            if (v[k]) atomicAdd(&out[k], v[k]);
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = 0;
+}
+
+}  // namespace
+
+int match_online_device(const int32_t* d_rowptr, const int32_t* d_col, int64_t rows, int64_t nnz,
+                        int dense_threshold, int t_max, unsigned long long* d_out, hipStream_t stream) {
+    TRSPMM_CHECK(rows > 0 && rows < (int64_t)1 << 31, "match_2to4_online: rows must be in [1, 2^31)");
+    TRSPMM_CHECK(nnz >= 0 && nnz < (int64_t)1 << 31, "match_2to4_online: nnz must be in [0, 2^31)");
+    TRSPMM_CHECK(dense_threshold >= 1, "match_2to4_online: dense_threshold must be >= 1");
+    const int64_t windows = (rows + OM_ROWS - 1) / OM_ROWS;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, d_out, TRSPMM_ONLINE_COUNTERS);
+    TRSPMM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_match_online, dim3((unsigned)windows), dim3(OM_WAVE), 0, stream, d_rowptr, d_col, rows, nnz,
+                       dense_threshold, t_max, d_out);
+    TRSPMM_HIP(hipGetLastError());
+    return TRSPMM_OK;
+}
+
+}  // namespace trspmm
