@@ -54,6 +54,8 @@ def parse():
                    help="per-route rows, nnz and times of --workload (exact and tuned routing)")
     p.add_argument("--backward", action="store_true",
                    help="measure only forward vs forward+backward on --workload (fp16 and bf16)")
+    p.add_argument("--matching", action="store_true",
+                   help="time the four setup_24matching statistics ops on --workload's matrix")
     p.add_argument("--copy-calibration", action="store_true",
                    help="also run a plain 256 MiB device copy (the reference kernel for scaling "
                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -345,6 +347,61 @@ def measure_batched(steps, warmup, check=True, count=1024, K=128):
     return out
 
 
+def measure_matching(csr, workload, steps, warmup):
+    """Wall time of each matching statistics op on the workload's CSR: the
+    host ops over their thread pool (best of `steps`, min 3), the device op
+    with HIP events around the kernels alone, inputs already on the device."""
+    import ctypes
+    import warnings
+
+    import torch
+
+    import matching_utils
+    import matching_utils_bucket
+    import matching_utils_hash
+    import matching_utils_online
+    from trspmm import _lib
+
+    rp, col, val = csr.rowptr, csr.col, csr.val
+    out = {"workload": workload, "rows": int(rp.size - 1), "nnz": int(col.size), "host_threads": 16, "ms": {}}
+    reps = max(3, min(steps, 10))
+    for name, fn in (("match_2to4", matching_utils.match_2to4), ("match_2to4_hash", matching_utils_hash.match_2to4_hash),
+                     ("match_2to4_bucket", matching_utils_bucket.match_2to4_bucket)):
+        best = float("inf")
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn(rp, col, val, threads=16)
+            best = min(best, time.perf_counter() - t)
+        out["ms"][name] = round(best * 1e3, 3)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)     # hub windows with > 2048 columns: counted below
+        stats = matching_utils_online.match_2to4_online(rp, col, val)
+    drp = torch.from_numpy(rp).cuda()
+    dcol = torch.from_numpy(col).cuda()
+    cnt = torch.empty(11, dtype=torch.int64, device="cuda")
+    L = _lib.lib()
+
+    def call():
+        _lib.check(L.trspmm_match_2to4_online(ctypes.c_void_p(drp.data_ptr()), ctypes.c_void_p(dcol.data_ptr()),
+                                              rp.size - 1, col.size, 8, 8, ctypes.c_void_p(cnt.data_ptr()),
+                                              _lib.current_stream()), "trspmm_match_2to4_online")
+    for _ in range(max(1, warmup)):
+        call()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        call()
+    e1.record()
+    torch.cuda.synchronize()
+    out["ms"]["match_2to4_online"] = round(e0.elapsed_time(e1) / reps, 3)
+    c = cnt.cpu().tolist()
+    out["online_num_sptc_groups"] = stats["num_sptc_groups"]
+    out["online_windows_over_2048_columns"] = c[9]
+    assert c[0] == stats["num_sptc_groups"], "repeated online launches disagree"
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -373,6 +430,9 @@ def main():
             rec = measure_routes(csr, K, mode, a.steps, a.warmup)
             rec.update(workload=a.workload, rows=csr.m, nnz_total=csr.nnz, K=K, regions_rows=regions)
             print(json.dumps(rec), flush=True)
+        return
+    if a.matching:
+        print(json.dumps(measure_matching(csr, a.workload, a.steps, a.warmup)), flush=True)
         return
     if a.backward:
         for dt in ("fp16", "bf16"):
