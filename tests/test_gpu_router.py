@@ -40,6 +40,14 @@ def test_router_variants(cuda, case, vi):
     _compare(CASES[case](), VARIANTS[vi], "compat")
 
 
+@pytest.mark.parametrize("mode", ["compat", "exact"])
+def test_router_more_nonzeros_than_one_scan_block(cuda, mode):
+    """442 884 nonzeros: the per-nonzero prefix sums take the multi-pass scan
+    (more than 2^18 entries)."""
+    from trspmm import graphs as g
+    _compare(g.rmat_gcn(14, seed=4), {}, mode)
+
+
 def test_router_window8(cuda):
     csr = CASES["er512"]()
     _compare(csr, {"window": 8}, "compat")

@@ -1,7 +1,9 @@
 // Device prefix sums and bounded searches used by the router and the plan
-// builder.  One workgroup scans the whole array: every caller scans per-window
-// or per-part counts (at most a few million entries), where a single pass of
-// 1024 threads is a few tens of microseconds and needs no scratch memory.
+// builder.  Arrays up to kScanSingle entries (per-window and per-part counts)
+// are scanned by one workgroup, a few tens of microseconds with no scratch
+// memory; longer ones (per-nonzero flags) in three passes over chunks of
+// kScanChunk entries: chunk sums, a one-workgroup scan of the sums, and each
+// chunk's scan from its offset.
 #pragma once
 
 #include "common.h"
@@ -36,10 +38,79 @@ __global__ __launch_bounds__(1024) void k_exclusive_scan(const T* __restrict__ i
     if (tid == 1023) out[n] = part[1023];
 }
 
+constexpr int64_t kScanSingle = 1 << 18;
+constexpr int64_t kScanChunk = 1 << 14;
+
+// sums[b] = sum of chunk b of `in` (null = zeros)
+template <typename T>
+__global__ __launch_bounds__(1024) void k_scan_chunk_sums(const T* __restrict__ in, int64_t n, T* __restrict__ sums) {
+    __shared__ T part[1024];
+    const int64_t b = (int64_t)blockIdx.x * kScanChunk, e = b + kScanChunk < n ? b + kScanChunk : n;
+    T v = 0;
+    if (in)
+        for (int64_t i = b + threadIdx.x; i < e; i += 1024) v += in[i];
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) part[threadIdx.x] += part[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = part[0];
+}
+
+// out[b .. e) = exclusive scan of chunk blockIdx.x, starting from offs[blockIdx.x];
+// out[n] = offs[gridDim.x], the total
+template <typename T>
+__global__ __launch_bounds__(1024) void k_scan_chunks(const T* __restrict__ in, T* __restrict__ out, int64_t n,
+                                                      const T* __restrict__ offs) {
+    constexpr int PER = (int)(kScanChunk / 1024);
+    __shared__ T part[1024];
+    const int tid = threadIdx.x;
+    const int64_t b = (int64_t)blockIdx.x * kScanChunk + (int64_t)tid * PER;
+    T v[PER];
+    T s = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        v[k] = (in && b + k < n) ? in[b + k] : T(0);
+        s += v[k];
+    }
+    part[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const T x = tid >= off ? part[tid - off] : T(0);
+        __syncthreads();
+        part[tid] += x;
+        __syncthreads();
+    }
+    T run = offs[blockIdx.x] + (tid ? part[tid - 1] : T(0));
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        if (b + k < n) out[b + k] = run;
+        run += v[k];
+    }
+    if (blockIdx.x == 0 && tid == 0) out[n] = offs[gridDim.x];     // offs has one entry per chunk + the total
+}
+
 template <typename T>
 inline int exclusive_scan(const T* in, T* out, int64_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_exclusive_scan<T>, dim3(1), dim3(1024), 0, s, in, out, n);
+    if (n <= kScanSingle) {
+        hipLaunchKernelGGL(k_exclusive_scan<T>, dim3(1), dim3(1024), 0, s, in, out, n);
+        TRSPMM_HIP(hipGetLastError());
+        return TRSPMM_OK;
+    }
+    const int64_t G = (n + kScanChunk - 1) / kScanChunk;
+    DevBuf sums;
+    int rc = sums.alloc(sizeof(T) * (size_t)(2 * G + 1));
+    if (rc) return rc;
+    T* chunk_sums = sums.as<T>();
+    T* chunk_offs = chunk_sums + G;
+    hipLaunchKernelGGL(k_scan_chunk_sums<T>, dim3((unsigned)G), dim3(1024), 0, s, in, n, chunk_sums);
     TRSPMM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_exclusive_scan<T>, dim3(1), dim3(1024), 0, s, chunk_sums, chunk_offs, G);
+    TRSPMM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_scan_chunks<T>, dim3((unsigned)G), dim3(1024), 0, s, in, out, n, chunk_offs);
+    TRSPMM_HIP(hipGetLastError());
+    TRSPMM_HIP(hipStreamSynchronize(s));      // the chunk sums are released on return
     return TRSPMM_OK;
 }
 
