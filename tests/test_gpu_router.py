@@ -48,6 +48,26 @@ def test_router_more_nonzeros_than_one_scan_block(cuda, mode):
     _compare(g.rmat_gcn(14, seed=4), {}, mode)
 
 
+@pytest.mark.parametrize("cols_per_row", [300, 5000])
+def test_router_hub_windows_sorted_by_the_wave(cuda, cols_per_row):
+    """Windows whose 2:4 units (up to ~40 000, more than LDS holds at 5000
+    columns per row) are ordered by the wave-parallel std::sort emulation;
+    the 16 rows of window 0 draw from 2^20 columns, the other windows are
+    sparse."""
+    from trspmm import graphs as g
+    rng = np.random.default_rng(cols_per_row)
+    rows, cols = [], []
+    for r in range(16):
+        c = rng.choice(1 << 20, size=cols_per_row, replace=False)
+        rows.append(np.full(c.size, r))
+        cols.append(c)
+    rows.append(rng.integers(16, 64, 400))
+    cols.append(rng.integers(0, 1 << 20, 400))
+    csr = g.csr_from_pairs(np.concatenate(rows), np.concatenate(cols), None, 64, 1 << 20)
+    for mode in ("compat", "exact"):
+        _compare(csr, {}, mode)
+
+
 def test_router_window8(cuda):
     csr = CASES["er512"]()
     _compare(csr, {"window": 8}, "compat")

@@ -27,6 +27,7 @@
 //      residual flags -> per-row residual parts, mma.sp tile packing.
 #include <chrono>
 #include <cstring>   // rocPRIM's texture iterator uses memset without including it
+#include <algorithm>
 #include <vector>
 
 #include <rocprim/rocprim.hpp>
@@ -40,6 +41,7 @@ namespace trspmm {
 namespace {
 
 constexpr int kWave = 64;
+constexpr int kWaveSortMin = 256;   // windows with more 2:4 units sort them with the whole wave
 
 __device__ inline uint64_t lanemask_lt() {
     const int l = threadIdx.x & 63;
@@ -153,25 +155,195 @@ struct RunArrays {
 
 __device__ inline int run_count(const RunArrays& R, int32_t run) { return R.start[run + 1] - R.start[run]; }
 
-// One wave per window.  A launch serves one size class: windows whose unit
-// bound ceil(columns/2) lies in [cls_lo, cls_hi]; the others return at once.
-// `lds_cap` = unit capacity of the dynamic LDS (0: units live in global memory).
+// ---------------------------------------------------------------- wave-parallel std::sort emulation
+//
+// The same permutation as lsort::sort(a, n, lsort::KeyDesc{}) -- libstdc++'s
+// introsort under a comparator that leaves ties unbroken -- computed by one
+// wave.  The introsort loop visits the same ranges with the same pivots; each
+// Hoare partition is computed in closed form.  With kp the pivot's key, the
+// left scan stops at the positions of [first+1, last) whose key is <= kp
+// (l_1 < l_2 < ...) and the right scan at those of [first, last) whose key is
+// >= kp (r_1 > r_2 > ...; `first` holds the pivot).  Between two stops every
+// position still holds its original value, so the partition swaps l_k with
+// r_k for k = 1..S, S = max over x of min(#l <= x, #r > x), and returns
+// min(l_{S+1}, r_S).  The final insertion pass moves an element only past
+// strictly larger keys, so it is a stable sort by key: a counting sort
+// through `tmp`.  A range whose depth budget runs out is heap-sorted by lane
+// 0, as lsort::sort does.  `tmp` needs n entries of global memory.
+struct WaveSortLds {
+    int32_t base[256];     // per-key output offsets of the final pass
+    int32_t kp, cut_l;
+};
+
+__device__ inline int32_t wave_max_i32(int32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+    return v;
+}
+
+__device__ int32_t wave_partition(uint32_t* a, int32_t first, int32_t last, int32_t* __restrict__ tmp,
+                                  WaveSortLds& sl) {
+    const int lane = threadIdx.x;
+    if (lane == 0) {
+        // median of (first + 1, mid, last - 1) moved to first (lsort::partition_pivot)
+        const lsort::KeyDesc less{};
+        const int32_t mid = first + (last - first) / 2, x = first + 1, y = mid, z = last - 1;
+        int32_t m;
+        if (less(a[x], a[y])) m = less(a[y], a[z]) ? y : (less(a[x], a[z]) ? z : x);
+        else if (less(a[x], a[z])) m = x;
+        else if (less(a[y], a[z])) m = z;
+        else m = y;
+        lsort::swap_at(a, first, m);
+        sl.kp = (int32_t)(a[first] >> 24);
+        sl.cut_l = INT32_MAX;
+    }
+    __syncthreads();
+    const uint32_t kp = (uint32_t)sl.kp;
+    const uint64_t le = lanemask_lt() | (1ull << lane);
+    // #r over the range
+    int32_t TR = 0;
+    for (int32_t b = first; b < last; b += kWave) {
+        const int32_t p = b + lane;
+        TR += __popcll(ballot(p < last && (a[p] >> 24) >= kp));
+    }
+    // S = max_x min(#l <= x, #r > x)
+    int32_t PL = 0, PR = 0, S = 0;
+    for (int32_t b = first; b < last; b += kWave) {
+        const int32_t p = b + lane;
+        const bool on = p < last;
+        const uint32_t k = on ? a[p] >> 24 : 0u;
+        const uint64_t mL = ballot(on && p > first && k <= kp), mR = ballot(on && k >= kp);
+        const int32_t pl = PL + __popcll(mL & le), pr = PR + __popcll(mR & le);
+        S = max(S, wave_max_i32(on ? min(pl, TR - pr) : 0));
+        PL += __popcll(mL);
+        PR += __popcll(mR);
+    }
+    // r_k (k <= S) -> tmp[k - 1]; l_{S+1} -> cut_l (nothing is swapped yet)
+    PL = 0;
+    PR = 0;
+    for (int32_t b = first; b < last; b += kWave) {
+        const int32_t p = b + lane;
+        const bool on = p < last;
+        const uint32_t k = on ? a[p] >> 24 : 0u;
+        const bool isL = on && p > first && k <= kp, isR = on && k >= kp;
+        const uint64_t mL = ballot(isL), mR = ballot(isR);
+        const int32_t pl = PL + __popcll(mL & le), rank = TR - (PR + __popcll(mR & le)) + 1;
+        if (isR && rank <= S) tmp[rank - 1] = p;
+        if (isL && pl == S + 1) sl.cut_l = p;
+        PL += __popcll(mL);
+        PR += __popcll(mR);
+    }
+    __syncthreads();
+    // swap l_k with r_k, k <= S.  Every swapped r lies above l_S, so the scan
+    // ends after the chunk holding l_S, before reaching a swapped position
+    // from an earlier chunk; within a chunk all reads precede the writes.
+    PL = 0;
+    for (int32_t b = first; b < last && PL < S; b += kWave) {
+        const int32_t p = b + lane;
+        const bool on = p < last;
+        const uint32_t v = on ? a[p] : 0u;
+        const bool isL = on && p > first && (v >> 24) <= kp;
+        const uint64_t mL = ballot(isL);
+        const int32_t pl = PL + __popcll(mL & le);
+        if (isL && pl <= S) {
+            const int32_t j = tmp[pl - 1];
+            a[p] = a[j];
+            a[j] = v;
+        }
+        PL += __popcll(mL);
+    }
+    const int32_t r_s = S > 0 ? tmp[S - 1] : INT32_MAX;
+    int32_t cut = min(sl.cut_l, r_s);
+    __syncthreads();
+    return cut > last ? last : cut;     // a left stop always exists (libstdc++'s median-of-3 invariant)
+}
+
+// stable sort of a[0, n) by key descending, through tmp
+__device__ void wave_stable_by_key(uint32_t* a, int32_t n, uint32_t* __restrict__ tmp, WaveSortLds& sl) {
+    const int lane = threadIdx.x;
+    const uint64_t lt = lanemask_lt();
+    for (int t = lane; t < 256; t += kWave) sl.base[t] = 0;
+    __syncthreads();
+    for (int32_t p = lane; p < n; p += kWave) atomicAdd(&sl.base[a[p] >> 24], 1);
+    __syncthreads();
+    if (lane == 0) {
+        int32_t acc = 0;
+        for (int k = 255; k >= 0; --k) {
+            const int32_t c = sl.base[k];
+            sl.base[k] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+    for (int32_t b = 0; b < n; b += kWave) {
+        const int32_t p = b + lane;
+        const bool on = p < n;
+        const uint32_t v = on ? a[p] : 0u;
+        const uint32_t k = v >> 24;
+        uint64_t rem = ballot(on);
+        while (rem) {
+            const int leader = __ffsll((long long)rem) - 1;
+            const uint32_t kk = (uint32_t)__shfl((int)k, leader);
+            const uint64_t grp = ballot(on && k == kk);
+            const int32_t at = sl.base[kk];          // read by every lane before lane 0 writes it
+            if (on && k == kk) tmp[at + __popcll(grp & lt)] = v;
+            if (lane == 0) sl.base[kk] = at + __popcll(grp);
+            rem &= ~grp;
+        }
+    }
+    __syncthreads();
+    for (int32_t p = lane; p < n; p += kWave) a[p] = tmp[p];
+    __syncthreads();
+}
+
+__device__ void wave_sort_desc(uint32_t* a, int32_t n, int32_t* __restrict__ tmp, int32_t* stack, WaveSortLds& sl) {
+    const int lane = threadIdx.x;
+    if (lane == 0) {
+        stack[0] = 0;
+        stack[1] = n;
+        stack[2] = 2 * lsort::floor_log2(n);
+    }
+    __syncthreads();
+    int sp = 1;
+    while (sp > 0) {
+        --sp;
+        int32_t first = stack[3 * sp], last = stack[3 * sp + 1], depth = stack[3 * sp + 2];
+        __syncthreads();                          // every lane has the entry before it is reused
+        while (last - first > lsort::kThreshold) {
+            if (depth == 0) {
+                if (lane == 0) lsort::heap_sort(a + first, last - first, lsort::KeyDesc{});
+                __syncthreads();
+                break;
+            }
+            --depth;
+            const int32_t cut = wave_partition(a, first, last, tmp, sl);
+            if (lane == 0) {
+                stack[3 * sp] = cut;
+                stack[3 * sp + 1] = last;
+                stack[3 * sp + 2] = depth;
+            }
+            __syncthreads();
+            ++sp;
+            last = cut;
+        }
+    }
+    wave_stable_by_key(a, n, (uint32_t*)tmp, sl);
+}
+
+// One wave per window; the launch serves the windows of one size class,
+// listed in `wlist` (largest first).  `lds_cap` = unit capacity of the
+// dynamic LDS (0: units live in global memory).
 __global__ __launch_bounds__(64) void k_window_decide(RunArrays R, WinOut O, int64_t W, int window, int density,
                                                       float tc_min_util, int skip_tc, int unit_min_nnz,
-                                                      int cls_lo, int cls_hi, int lds_cap) {
+                                                      const int32_t* __restrict__ wlist, int lds_cap) {
     extern __shared__ uint32_t dyn[];
     __shared__ int32_t hist[17], base[17], fill[17];
     __shared__ int32_t s_ncand;
     __shared__ int32_t sort_stack[3 * 66];
-    const int64_t w = blockIdx.x;
-    if (w >= W) return;
+    __shared__ WaveSortLds wsl;
+    const int64_t w = wlist[blockIdx.x];
     const int lane = threadIdx.x;
     const int32_t rb = R.win_rb[w], re = R.win_rb[w + 1];
     const int32_t nc = re - rb;
-
-    // size class: units <= ceil(nc / 2) (TC columns only reduce it)
-    const int32_t umax = (nc + 1) / 2;
-    if (umax < cls_lo || umax > cls_hi) return;
 
     // a. histogram of run counts (1..16)
     if (lane < 17) {
@@ -263,9 +435,11 @@ __global__ __launch_bounds__(64) void k_window_decide(RunArrays R, WinOut O, int
     for (int32_t k = lane; k < nu; k += kWave) used[k] = 0;
     __syncthreads();
 
-    // f. libstdc++ order + pairing, one lane
+    // f. libstdc++ order (one wave for large windows, else one lane) + pairing, one lane
+    const bool wave_sorted = nu > kWaveSortMin;
+    if (wave_sorted) wave_sort_desc(units, nu, R.pair_a + rb, sort_stack, wsl);
     if (lane == 0) {
-        lsort::sort(units, nu, lsort::KeyDesc{}, sort_stack);
+        if (!wave_sorted) lsort::sort(units, nu, lsort::KeyDesc{}, sort_stack);
         int32_t low = 0, np = 0;
         for (int32_t i = 0; i < nu; ++i) {
             const int32_t v = (int32_t)(units[i] & 0xffffffu);
@@ -815,16 +989,58 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
     WinOut O{n_tcsub.as<int32_t>(), n_tcval.as<int32_t>(), n_pairs.as<int32_t>(), n_groups.as<int32_t>(),
              leftover.as<int32_t>(), fallback.as<int32_t>(), n_rest.as<int32_t>()};
     if (W > 0) {
-        // size classes by units-per-window: small windows share a CU, large ones get a big LDS
-        // slice, the rare giant window sorts in global memory.
-        struct Cls { int lo, hi, cap; };
-        const Cls classes[3] = {{0, 1024, 1024}, {1025, 12288, 12288}, {12289, 0x7fffffff, 0}};
-        for (const Cls& c : classes) {
-            const size_t lds = c.cap > 0 ? (size_t)c.cap * 5 + 16 : 0;
-            hipLaunchKernelGGL(k_window_decide, dim3((unsigned)W), dim3(64), lds, s, R, O, W, window, cfg.density,
-                               cfg.tc_min_util, cfg.skip_tc, cfg.unit_min_nnz, c.lo, c.hi, c.cap);
+        // size classes by units-per-window: small windows share a CU, large ones take as much
+        // LDS as a workgroup may hold (160 KB on gfx950), a window beyond that sorts in global
+        // memory.  A unit takes 4 bytes (key) + 1 (used flag).
+        int big_cap = 12288;
+        {
+            int dev = 0, max_lds = 0;
+            hipFuncAttributes fa{};
+            if (hipGetDevice(&dev) == hipSuccess &&
+                hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+                hipFuncGetAttributes(&fa, (const void*)k_window_decide) == hipSuccess) {
+                const int cap = (int)((max_lds - (int64_t)fa.sharedSizeBytes - 64) / 5);
+                if (cap > big_cap &&
+                    hipFuncSetAttribute((const void*)k_window_decide, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        cap * 5 + 16) == hipSuccess)
+                    big_cap = cap;
+            }
+            (void)hipGetLastError();      // a refused query leaves the 64 KB classes
+        }
+        // windows by size class (units <= ceil(columns / 2); TC columns only reduce it), each
+        // class listed largest first so the longest windows start first
+        std::vector<int32_t> rb_h((size_t)W + 1);
+        TRSPMM_HIP(hipMemcpyAsync(rb_h.data(), win_rb.as<int32_t>(), 4 * (W + 1), hipMemcpyDeviceToHost, s));
+        TRSPMM_HIP(hipStreamSynchronize(s));
+        const int caps[4] = {1024, std::min(6144, big_cap), big_cap, 0};
+        std::vector<int32_t> lists[4];
+        for (int64_t w = 0; w < W; ++w) {
+            const int32_t umax = (rb_h[w + 1] - rb_h[w] + 1) / 2;
+            const int c = umax <= caps[0] ? 0 : umax <= caps[1] ? 1 : umax <= caps[2] ? 2 : 3;
+            lists[c].push_back((int32_t)w);
+        }
+        std::vector<int32_t> all;
+        size_t at[5] = {0, 0, 0, 0, 0};
+        for (int c = 0; c < 4; ++c) {
+            if (c > 0)
+                std::stable_sort(lists[c].begin(), lists[c].end(), [&](int32_t a, int32_t b) {
+                    return rb_h[a + 1] - rb_h[a] > rb_h[b + 1] - rb_h[b];
+                });
+            all.insert(all.end(), lists[c].begin(), lists[c].end());
+            at[c + 1] = all.size();
+        }
+        DevBuf wl;
+        if ((rc = wl.alloc(4 * all.size()))) return rc;
+        TRSPMM_HIP(hipMemcpyAsync(wl.p, all.data(), 4 * all.size(), hipMemcpyHostToDevice, s));
+        for (int c = 0; c < 4; ++c) {
+            const size_t n = at[c + 1] - at[c];
+            if (n == 0) continue;
+            const size_t lds = caps[c] > 0 ? (size_t)caps[c] * 5 + 16 : 0;
+            hipLaunchKernelGGL(k_window_decide, dim3((unsigned)n), dim3(64), lds, s, R, O, W, window, cfg.density,
+                               cfg.tc_min_util, cfg.skip_tc, cfg.unit_min_nnz, wl.as<int32_t>() + at[c], caps[c]);
             TRSPMM_HIP(hipGetLastError());
         }
+        TRSPMM_HIP(hipStreamSynchronize(s));    // the window list is released on return from this block
     }
 
     // 4. window prefix sums -> output sizes
