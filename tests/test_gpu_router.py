@@ -84,3 +84,28 @@ def test_router_rejects_unsorted(cuda):
     col = torch.tensor([5, 3], dtype=torch.int32)
     with pytest.raises(RuntimeError, match="ascending"):
         route(rp, col, torch.ones(2), 8)
+
+
+@pytest.mark.parametrize("cols, msg", [
+    ([1, 9, 3], "out of"),                 # 9 >= ncols comes first
+    ([5, 3, 9], "ascending"),              # the descent comes before the out-of-range column
+    ([2, 4, 8], "out of"),
+], ids=["range-first", "order-first", "range-only"])
+def test_router_reports_the_first_bad_entry_of_a_row(cuda, cols, msg):
+    """One long row (checked by a whole wave): the first offending entry
+    decides the message, as a scan along the row would."""
+    n = 200
+    head = list(range(10, 10 + n))           # a valid prefix longer than a wave
+    c = np.array(head + [x + 300 for x in cols], np.int32)
+    c[n:] = np.array(cols, np.int32) + np.array([300, 300, 300])
+    ncols = 300 + 8                           # columns >= 308 are out of range
+    bad = c.copy()
+    rp = torch.tensor([0, bad.size], dtype=torch.int32)
+    with pytest.raises(RuntimeError, match=msg):
+        route(rp, torch.from_numpy(bad), torch.ones(bad.size), ncols)
+
+
+def test_router_rejects_bad_row_pointers(cuda):
+    rp = torch.tensor([0, 3, 2], dtype=torch.int32)
+    with pytest.raises(RuntimeError, match="row pointers"):
+        route(rp, torch.tensor([1, 2], dtype=torch.int32), torch.ones(2), 8)

@@ -62,29 +62,40 @@ __device__ inline int64_t upper_bound_window(const int32_t* off, int64_t W, int3
 }
 
 // ---------------------------------------------------------------- 1. validation + keys
-__global__ void k_validate(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, int64_t rows,
-                           int64_t nnz, int64_t ncols, int* __restrict__ err) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r == 0 && (rowptr[0] != 0 || (int64_t)rowptr[rows] != nnz)) atomicOr(err, 1);
+// One wave per row (four rows per 256-thread block): hub rows of tens of
+// thousands of nonzeros would otherwise serialise on one thread.
+__device__ inline int64_t wave_row() { return (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); }
+
+__device__ inline int64_t wave_min_i64(int64_t v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const int64_t o = __shfl_xor(v, off);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+// err bit 1: bad row pointers; 2: a column outside [0, ncols); 4: columns not
+// strictly ascending.  Within a row the first offending entry decides between
+// 2 and 4 (range checked first), as a sequential scan would.
+__global__ __launch_bounds__(256) void k_validate(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                                  int64_t rows, int64_t nnz, int64_t ncols, int* __restrict__ err) {
+    const int64_t r = wave_row();
+    const int lane = threadIdx.x & 63;
+    if (r == 0 && lane == 0 && (rowptr[0] != 0 || (int64_t)rowptr[rows] != nnz)) atomicOr(err, 1);
     if (r >= rows) return;
     const int32_t b = rowptr[r], e = rowptr[r + 1];
     if (b > e || b < 0 || (int64_t)e > nnz) {
-        atomicOr(err, 1);
+        if (lane == 0) atomicOr(err, 1);
         return;
     }
-    int32_t prev = -1;
-    for (int32_t i = b; i < e; ++i) {
+    int64_t first = INT64_MAX;                     // 4 * position + (2: range, 3: order), lowest position first
+    for (int32_t i = b + lane; i < e && first == INT64_MAX; i += kWave) {
         const int32_t c = col[i];
-        if (c < 0 || (int64_t)c >= ncols) {
-            atomicOr(err, 2);
-            return;
-        }
-        if (c <= prev) {
-            atomicOr(err, 4);
-            return;
-        }
-        prev = c;
+        if (c < 0 || (int64_t)c >= ncols) first = 4 * (int64_t)i + 2;
+        else if (i > b && c <= col[i - 1]) first = 4 * (int64_t)i + 3;
     }
+    first = wave_min_i64(first);
+    if (lane == 0 && first != INT64_MAX) atomicOr(err, (first & 3) == 2 ? 2 : 4);
 }
 
 __global__ void k_keys(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, int64_t rows, int64_t nnz,
@@ -739,12 +750,15 @@ struct ResCounts {
     int32_t *nl, *ns, *el, *es;   // per row: long parts, short parts, long entries, short entries
 };
 
-__global__ void k_res_count(const int32_t* __restrict__ rowptr, const uint8_t* __restrict__ flag, int64_t rows,
-                            int part_c, int short_c, ResCounts RC) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_res_count(const int32_t* __restrict__ rowptr, const uint8_t* __restrict__ flag,
+                                                   int64_t rows, int part_c, int short_c, ResCounts RC) {
+    const int64_t r = wave_row();
+    const int lane = threadIdx.x & 63;
     if (r >= rows) return;
     int32_t n = 0;
-    for (int32_t e = rowptr[r]; e < rowptr[r + 1]; ++e) n += flag[e];
+    for (int32_t e = rowptr[r] + lane; e < rowptr[r + 1]; e += kWave) n += flag[e];
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
+    if (lane != 0) return;
     int32_t nl = 0, ns = 0, el = 0, es = 0;
     if (n > 0) {
         if (n <= short_c) { ns = 1; es = n; }
@@ -773,11 +787,12 @@ struct ResOut {
     float* cs_value;
 };
 
-__global__ void k_res_emit(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                           const float* __restrict__ val, const uint8_t* __restrict__ flag,
-                           const int32_t* __restrict__ n_tcsub, int64_t rows, int window, int part_c, int short_c,
-                           ResOut O) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_res_emit(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                                  const float* __restrict__ val, const uint8_t* __restrict__ flag,
+                                                  const int32_t* __restrict__ n_tcsub, int64_t rows, int window,
+                                                  int part_c, int short_c, ResOut O) {
+    const int64_t r = wave_row();
+    const int lane = threadIdx.x & 63;
     if (r >= rows) return;
     const int32_t nl = O.pl[r + 1] - O.pl[r], ns = O.ps[r + 1] - O.ps[r];
     if (nl == 0 && ns == 0) return;
@@ -786,29 +801,36 @@ __global__ void k_res_emit(const int32_t* __restrict__ rowptr, const int32_t* __
     const int32_t n = el + (O.qs[r + 1] - O.qs[r]);
     const bool split = n > part_c;
     // long parts
-    for (int32_t j = 0; j < nl; ++j) {
+    for (int32_t j = lane; j < nl; j += kWave) {
         const int32_t p = O.pl[r] + j;
         O.c_row_offset[p] = O.ql[r] + j * part_c;
         O.c_row[p] = (int32_t)r;
         O.c_atomic[p] = split ? 1 : base_atomic;
     }
-    if (ns) {
+    if (ns && lane == 0) {
         const int32_t p = O.ps[r];
         O.cs_row_offset[p] = O.qs[r];
         O.cs_row[p] = (int32_t)r;
         O.cs_atomic[p] = split ? 1 : base_atomic;
     }
-    int32_t k = 0;
-    for (int32_t e = rowptr[r]; e < rowptr[r + 1]; ++e) {
-        if (!flag[e]) continue;
-        if (k < el) {
-            O.c_column[O.ql[r] + k] = col[e];
-            O.c_value[O.ql[r] + k] = val[e];
-        } else {
-            O.cs_column[O.qs[r] + k - el] = col[e];
-            O.cs_value[O.qs[r] + k - el] = val[e];
+    // the row's flagged entries in CSR order: the first el to the long parts, the rest short
+    const uint64_t lt = lanemask_lt();
+    int32_t k0 = 0;
+    for (int32_t b = rowptr[r]; b < rowptr[r + 1]; b += kWave) {
+        const int32_t e = b + lane;
+        const bool on = e < rowptr[r + 1] && flag[e];
+        const uint64_t m = ballot(on);
+        if (on) {
+            const int32_t k = k0 + __popcll(m & lt);
+            if (k < el) {
+                O.c_column[O.ql[r] + k] = col[e];
+                O.c_value[O.ql[r] + k] = val[e];
+            } else {
+                O.cs_column[O.qs[r] + k - el] = col[e];
+                O.cs_value[O.qs[r] + k - el] = val[e];
+            }
         }
-        ++k;
+        k0 += __popcll(m);
     }
 }
 
@@ -918,7 +940,7 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
     if ((rc = err.alloc(4))) return rc;
     TRSPMM_HIP(hipMemsetAsync(err.p, 0, 4, s));
     if (rows > 0)
-        hipLaunchKernelGGL(k_validate, dim3(nblk(rows)), dim3(256), 0, s, rowptr, col, rows, nnz, ncols,
+        hipLaunchKernelGGL(k_validate, dim3(nblk(rows, 4)), dim3(256), 0, s, rowptr, col, rows, nnz, ncols,
                            err.as<int>());
     int herr = 0;
     TRSPMM_HIP(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, s));
@@ -1155,7 +1177,7 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
         return rc;
     ResCounts RCn{nl.as<int32_t>(), ns.as<int32_t>(), el.as<int32_t>(), es.as<int32_t>()};
     if (rows > 0) {
-        hipLaunchKernelGGL(k_res_count, dim3(nblk(rows)), dim3(256), 0, s, rowptr, res_flag.as<uint8_t>(), rows,
+        hipLaunchKernelGGL(k_res_count, dim3(nblk(rows, 4)), dim3(256), 0, s, rowptr, res_flag.as<uint8_t>(), rows,
                            cfg.part_c, cfg.short_c, RCn);
         TRSPMM_HIP(hipGetLastError());
     }
@@ -1181,7 +1203,7 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
               (float*)M.data[11], (int32_t*)M.data[12], (int32_t*)M.data[13], (int32_t*)M.data[14],
               (int32_t*)M.data[15], (float*)M.data[16]};
     if (rows > 0) {
-        hipLaunchKernelGGL(k_res_emit, dim3(nblk(rows)), dim3(256), 0, s, rowptr, col, val, res_flag.as<uint8_t>(),
+        hipLaunchKernelGGL(k_res_emit, dim3(nblk(rows, 4)), dim3(256), 0, s, rowptr, col, val, res_flag.as<uint8_t>(),
                            n_tcsub.as<int32_t>(), rows, window, cfg.part_c, cfg.short_c, RO);
         TRSPMM_HIP(hipGetLastError());
     }
