@@ -21,6 +21,8 @@
  *                            single-owner output rows instead of 4 streams + add)
  *   trspmm_spmm_timed     <- the warm-up / timed-epoch loop of the same launcher
  *                            (mGCNkernel_online.cu:1604-1704) and its ms outputs
+ *   trspmm_tca_order, trspmm_minhash
+ *                         <- apply_tca_reorder, spmm/tr_csr_fp16/tr_tca_reorder.py:44-357
  *   trspmm_match_2to4, _hash, _bucket, _online
  *                         <- the setup_24matching statistics ops
  *                            matching_utils{,_hash,_bucket,_online}.match_2to4*
@@ -273,6 +275,27 @@ int trspmm_match_2to4_bucket(const int32_t* rowptr, const int32_t* col, int64_t 
 #define TRSPMM_ONLINE_COUNTERS 11
 int trspmm_match_2to4_online(const int32_t* rowptr, const int32_t* col, int64_t rows, int64_t nnz,
                              int dense_threshold, int t_max, unsigned long long* counters, void* stream);
+
+/* ------------------------------------------------------------------ TCA row reordering
+ * Replaces the row reordering of the reference's caller
+ * (spmm/tr_csr_fp16/tr_tca_reorder.py:44-357, apply_tca_reorder): MinHash-LSH
+ * candidate pairs, exact Jaccard, two-level greedy clustering (cluster caps
+ * `thres`, then `cluster_thres` over the clusters).  Not bit-reproducible
+ * with the reference (which hashes through datasketch's SHA-1 MinHash).
+ *
+ * trspmm_minhash: MinHash signatures on the device, sig[row * num_perm + k]
+ * (0xffffffff for an empty row); rowptr / col / sig are device pointers.
+ * trspmm_minhash_coefficients: the 2 * num_perm hash coefficients (a_k, b_k)
+ * the signatures use (host), so a checker can recompute them.
+ * trspmm_tca_order: host arrays; order[new_row] = old row.  `sig` (host,
+ * rows * num_perm) may be null: the signatures are then computed on the
+ * current device.  Synchronous. */
+int trspmm_minhash(const int32_t* rowptr, const int32_t* col, int64_t rows, int num_perm, uint64_t seed,
+                   uint32_t* sig, void* stream);
+int trspmm_minhash_coefficients(int num_perm, uint64_t seed, uint64_t* out);
+int trspmm_tca_order(const int32_t* rowptr, const int32_t* col, int64_t rows, int64_t nnz, const uint32_t* sig,
+                     int num_perm, int thres, int cluster_thres, double lsh_threshold, uint64_t seed,
+                     int32_t* order);
 
 /* ------------------------------------------------------------------ diagnostics */
 

@@ -12,6 +12,11 @@ namespace trspmm {
 
 int match_online_device(const int32_t* d_rowptr, const int32_t* d_col, int64_t rows, int64_t nnz,
                         int dense_threshold, int t_max, unsigned long long* d_out, hipStream_t stream);
+int minhash_host_coefficients(int P, uint64_t seed, uint64_t* out);
+int minhash_device(const int32_t* d_rowptr, const int32_t* d_col, int64_t rows, int P, uint64_t seed,
+                   uint32_t* d_sig, hipStream_t s);
+int tca_order(const int32_t* rowptr, const int32_t* col, int64_t rows, int64_t nnz, const uint32_t* sig, int P,
+              int thres, int cluster_thres, double lsh_threshold, uint64_t seed, int32_t* order);
 
 static thread_local std::string g_error;
 void set_error(const std::string& msg) { g_error = msg; }
@@ -289,6 +294,34 @@ int trspmm_match_2to4_online(const int32_t* d_rowptr, const int32_t* d_col, int6
     TRSPMM_NOTNULL(d_counters, "counters");
     TRSPMM_CHECK(nnz == 0 || d_col, "col must not be null when nnz > 0");
     return match_online_device(d_rowptr, d_col, rows, nnz, dense_threshold, t_max, d_counters, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------- TCA row reordering
+int trspmm_minhash(const int32_t* d_rowptr, const int32_t* d_col, int64_t rows, int num_perm, uint64_t seed,
+                   uint32_t* d_sig, void* stream) {
+    TRSPMM_NOTNULL(d_rowptr, "rowptr");
+    TRSPMM_CHECK(rows == 0 || d_sig, "sig must not be null");
+    return minhash_device(d_rowptr, d_col, rows, num_perm, seed, d_sig, (hipStream_t)stream);
+}
+
+int trspmm_minhash_coefficients(int num_perm, uint64_t seed, uint64_t* out) {
+    TRSPMM_NOTNULL(out, "out");
+    TRSPMM_CHECK(num_perm >= 1 && num_perm <= 1024, "num_perm must be in [1, 1024]");
+    return minhash_host_coefficients(num_perm, seed, out);
+}
+
+int trspmm_tca_order(const int32_t* rowptr, const int32_t* col, int64_t rows, int64_t nnz, const uint32_t* sig,
+                     int num_perm, int thres, int cluster_thres, double lsh_threshold, uint64_t seed,
+                     int32_t* order) {
+    TRSPMM_NOTNULL(rowptr, "rowptr");
+    TRSPMM_NOTNULL(order, "order");
+    TRSPMM_CHECK(nnz == 0 || col, "col must not be null when nnz > 0");
+    try {
+        return tca_order(rowptr, col, rows, nnz, sig, num_perm, thres, cluster_thres, lsh_threshold, seed, order);
+    } catch (const std::bad_alloc&) {
+        set_error("tca_order: out of host memory");
+        return TRSPMM_ENOMEM;
+    }
 }
 
 }  // extern "C"
