@@ -52,7 +52,7 @@ def _star(n=1500, seed=23):
     return g.csr_from_pairs(rows, cols, rng.uniform(-1, 1, rows.size), n, n)
 
 
-MATS = dict(CASES, rect=_rect, star=_star)
+MATS = dict(CASES, rect=_rect, star=_star, star800=lambda: _star(800, seed=5))   # star800: a whole long row
 
 
 @pytest.mark.parametrize("case", sorted(MATS))
