@@ -46,6 +46,7 @@ namespace {
 constexpr int OM_ROWS = 16;           // window height (fixed, as in the reference kernel)
 constexpr int OM_SLOTS = 2048;        // the reference's table slots per window
 constexpr int OM_DEDUP = 4096;        // collection table slots (at most half full)
+constexpr int OM_COUNT = 8192;        // counting table slots (ids only; at most half full)
 constexpr int OM_PAIR_SEARCH = 32;    // columns looked ahead for a 1:2 partner
 constexpr int OM_GROUP_SEARCH = 1024; // pairs looked ahead for a 2:4 partner
 constexpr int OM_WAVE = 64;
@@ -57,6 +58,7 @@ struct OnlineLds {
             int32_t id[OM_DEDUP];
             uint32_t mask[OM_DEDUP];
         } d;
+        int32_t cnt[OM_COUNT];            // counting passes: ids only, twice the slots
         struct {                          // layout in the reference's table, then its order
             uint32_t home_cnt[OM_SLOTS];  // columns whose home is this slot
             uint16_t owner[OM_SLOTS];     // column (index into keys) in this slot, OM_FREE if none
@@ -65,6 +67,7 @@ struct OnlineLds {
         } t;
     } a;
     union {
+        uint32_t ids[2 * OM_SLOTS];       // cut-off search: the distinct ids below a cut, sorted
         uint64_t keys[OM_SLOTS];          // collected columns: (column id << 32) | row mask
         struct {
             uint16_t g1[OM_SLOTS / 2], g2[OM_SLOTS / 2];   // pair: rows both / either columns touch
@@ -116,16 +119,20 @@ __device__ inline int first_lane(bool pred) {
 }
 
 __device__ inline int dedup_slot(int c) { return (int)(((uint32_t)c * 2654435761u) >> 20); }   // 12 bits
+__device__ inline int count_slot(int c) { return (int)(((uint32_t)c * 2654435761u) >> 19); }   // 13 bits
 
-// Collects the window's distinct columns in [0, limit] (and, with MASKS,
-// their row masks) into the collection table; stops once `cap` are in.
-// Returns the count (>= cap if it stopped), on every lane.
+// Collects the window's distinct columns in [0, limit] into a table: with
+// MASKS the collection table (ids and row masks, OM_DEDUP slots), else the
+// counting table (ids only, OM_COUNT slots in the same memory).  Stops once
+// `cap` are in.  Returns the count (>= cap if it stopped), on every lane.
 template <bool MASKS>
 __device__ int collect(OnlineLds& s, OnlineCtl& ctl, const int32_t* __restrict__ col, int win_rows, int limit,
                        int cap) {
+    constexpr int SLOTS = MASKS ? OM_DEDUP : OM_COUNT;
+    int32_t* ids = MASKS ? s.a.d.id : s.a.cnt;
     __syncthreads();                       // earlier users of the shared memory are done
-    for (int t = lane_id(); t < OM_DEDUP; t += OM_WAVE) {
-        s.a.d.id[t] = -1;
+    for (int t = lane_id(); t < SLOTS; t += OM_WAVE) {
+        ids[t] = -1;
         if (MASKS) s.a.d.mask[t] = 0;
     }
     if (lane_id() == 0) {
@@ -140,9 +147,10 @@ __device__ int collect(OnlineLds& s, OnlineCtl& ctl, const int32_t* __restrict__
             if (stopped()) break;
             const int c = col[e];
             if (c < 0 || c > limit) continue;
-            // at most cap + 63 <= OM_SLOTS + 64 columns are ever placed: a free slot exists
-            for (int slot = dedup_slot(c), probe = 0; probe < OM_DEDUP; ++probe, slot = (slot + 1) & (OM_DEDUP - 1)) {
-                const int prev = atomicCAS(&s.a.d.id[slot], -1, c);
+            // at most cap + 63 <= SLOTS / 2 + 64 columns are ever placed: a free slot exists
+            for (int slot = MASKS ? dedup_slot(c) : count_slot(c), probe = 0; probe < SLOTS;
+                 ++probe, slot = (slot + 1) & (SLOTS - 1)) {
+                const int prev = atomicCAS(&ids[slot], -1, c);
                 if (prev == -1 || prev == c) {
                     if (MASKS) atomicOr(&s.a.d.mask[slot], 1u << r);
                     if (prev == -1 && atomicAdd(&ctl.occ, 1) + 1 >= cap)
@@ -192,15 +200,51 @@ __global__ __launch_bounds__(OM_WAVE) void k_match_online(const int32_t* __restr
     if (n > OM_SLOTS) {
         // more distinct columns than the reference's table holds: keep the
         // OM_SLOTS smallest column ids (what inserting in ascending order
-        // keeps).  Binary search for the largest kept id T: count(T) =
-        // OM_SLOTS, count(T - 1) < OM_SLOTS.
+        // keeps).  Bisect for a cut whose distinct count lies in
+        // [OM_SLOTS, OM_COUNT / 2] -- the counting table then holds every
+        // distinct id below it -- and take the OM_SLOTS-th smallest of those.
+        // count(lo) < OM_SLOTS and count(hi) > OM_COUNT / 2 cannot hold with
+        // hi = lo + 1 (one more id adds at most one column), so the loop ends.
         if (lane == 0) atomicAdd(&out[9], 1ull);
-        int lo = -1;                                       // count(lo) < OM_SLOTS <= count(hi)
-        while (hi - lo > 1) {
-            const int mid = lo + (hi - lo) / 2;
-            if (collect<false>(s, ctl, col, win_rows, mid, OM_SLOTS) >= OM_SLOTS) hi = mid;
-            else lo = mid;
+        constexpr int BAND = OM_COUNT / 2;
+        int lo = -1, cut = hi, m = 0;                      // count(lo) < OM_SLOTS
+        for (;;) {
+            m = collect<false>(s, ctl, col, win_rows, cut, BAND + 1);
+            if (m >= OM_SLOTS && m <= BAND) break;
+            if (m < OM_SLOTS) lo = cut;
+            else hi = cut;
+            cut = lo + (hi - lo) / 2;
         }
+        // the m ids -> ids[0, m), sorted ascending
+        {
+            constexpr int PER = OM_COUNT / OM_WAVE;
+            const int base = lane * PER;
+            int c = 0, total = 0;
+#pragma unroll 4
+            for (int t = 0; t < PER; ++t) c += (s.a.cnt[base + t] != -1) ? 1 : 0;
+            int at = wave_exclusive_scan(c, &total);
+#pragma unroll 4      // re-read rather than keep PER values live across the scan
+            for (int t = 0; t < PER; ++t)
+                if (s.a.cnt[base + t] != -1) s.b.ids[at++] = (uint32_t)s.a.cnt[base + t];
+        }
+        int P = 2;
+        while (P < m) P <<= 1;
+        for (int t = m + lane; t < P; t += OM_WAVE) s.b.ids[t] = 0xffffffffu;
+        __syncthreads();
+        for (int k = 2; k <= P; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int t = lane; t < P / 2; t += OM_WAVE) {
+                    const int i = (t / j) * 2 * j + (t % j), q = i + j;
+                    const uint32_t x = s.b.ids[i], y = s.b.ids[q];
+                    if ((x > y) == ((i & k) == 0)) {
+                        s.b.ids[i] = y;
+                        s.b.ids[q] = x;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        hi = (int)s.b.ids[OM_SLOTS - 1];
         n = collect<true>(s, ctl, col, win_rows, hi, OM_DEDUP + 1);   // exactly OM_SLOTS columns
     }
 
@@ -209,8 +253,10 @@ __global__ __launch_bounds__(OM_WAVE) void k_match_online(const int32_t* __restr
         constexpr int PER = OM_DEDUP / OM_WAVE;
         const int base = lane * PER;
         int cnt = 0, total = 0;
+#pragma unroll 4
         for (int t = 0; t < PER; ++t) cnt += (s.a.d.id[base + t] != -1) ? 1 : 0;
         int at = wave_exclusive_scan(cnt, &total);
+#pragma unroll 4
         for (int t = 0; t < PER; ++t)
             if (s.a.d.id[base + t] != -1)
                 s.b.keys[at++] = ((uint64_t)(uint32_t)s.a.d.id[base + t] << 32) | s.a.d.mask[base + t];
@@ -273,8 +319,10 @@ __global__ __launch_bounds__(OM_WAVE) void k_match_online(const int32_t* __restr
         constexpr int PER = OM_SLOTS / OM_WAVE;
         const int base = lane * PER;
         int cnt = 0, total = 0;
+#pragma unroll 4
         for (int t = 0; t < PER; ++t) cnt += (s.a.t.owner[base + t] != OM_FREE) ? 1 : 0;
         int at = wave_exclusive_scan(cnt, &total);
+#pragma unroll 4
         for (int t = 0; t < PER; ++t) {
             const int k = s.a.t.owner[base + t];
             if (k != OM_FREE) s.a.t.cmask[at++] = (uint16_t)(s.b.keys[k] & 0xFFFFu);
