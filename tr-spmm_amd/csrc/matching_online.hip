@@ -47,6 +47,7 @@ constexpr int OM_ROWS = 16;           // window height (fixed, as in the referen
 constexpr int OM_SLOTS = 2048;        // the reference's table slots per window
 constexpr int OM_DEDUP = 4096;        // collection table slots (at most half full)
 constexpr int OM_COUNT = 8192;        // counting table slots (ids only; at most half full)
+constexpr int OM_BATCH = 4;           // column loads in flight per lane in a collection pass
 constexpr int OM_PAIR_SEARCH = 32;    // columns looked ahead for a 1:2 partner
 constexpr int OM_GROUP_SEARCH = 1024; // pairs looked ahead for a 2:4 partner
 constexpr int OM_WAVE = 64;
@@ -141,13 +142,28 @@ __device__ int collect(OnlineLds& s, OnlineCtl& ctl, const int32_t* __restrict__
     }
     __syncthreads();
     auto stopped = [&]() { return __hip_atomic_load(&ctl.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0; };
-    for (int r = 0; r < win_rows && !stopped(); ++r) {
-        const int b = ctl.rp[r], end = max(b, ctl.rp[r + 1]);
-        for (int e = b + lane_id(); e < end; e += OM_WAVE) {
-            if (stopped()) break;
-            const int c = col[e];
+    // one flat pass over the window's nonzeros, OM_BATCH loads in flight per lane;
+    // an entry's row is the number of row starts at or below it
+    int32_t rp[OM_ROWS + 1];
+#pragma unroll
+    for (int k = 0; k <= OM_ROWS; ++k) rp[k] = ctl.rp[k];
+    const int e_end = max(rp[0], rp[win_rows]);
+    for (int e0 = rp[0]; e0 < e_end && !stopped(); e0 += OM_BATCH * OM_WAVE) {
+        int cs[OM_BATCH];
+#pragma unroll
+        for (int u = 0; u < OM_BATCH; ++u) {
+            const int e = e0 + u * OM_WAVE + lane_id();
+            cs[u] = e < e_end ? col[e] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < OM_BATCH; ++u) {
+            const int c = cs[u];
             if (c < 0 || c > limit) continue;
-            // at most cap + 63 <= SLOTS / 2 + 64 columns are ever placed: a free slot exists
+            const int e = e0 + u * OM_WAVE + lane_id();
+            int r = 0;
+#pragma unroll
+            for (int k = 1; k < OM_ROWS; ++k) r += (k < win_rows && e >= rp[k]) ? 1 : 0;
+            // at most cap + OM_BATCH * 64 - 1 <= SLOTS / 2 + 256 columns are ever placed: a free slot exists
             for (int slot = MASKS ? dedup_slot(c) : count_slot(c), probe = 0; probe < SLOTS;
                  ++probe, slot = (slot + 1) & (SLOTS - 1)) {
                 const int prev = atomicCAS(&ids[slot], -1, c);
@@ -184,10 +200,16 @@ __global__ __launch_bounds__(OM_WAVE) void k_match_online(const int32_t* __restr
     // largest column id and any negative one, in one read of the window
     const int e0 = ctl.rp[0], e1 = max(e0, ctl.rp[win_rows]);
     int hi = -1, neg = 0;
-    for (int e = e0 + lane; e < e1; e += OM_WAVE) {
-        const int c = col[e];
-        hi = max(hi, c);
-        neg |= c < 0;
+    for (int b = e0; b < e1; b += OM_BATCH * OM_WAVE) {
+        int cs[OM_BATCH];
+#pragma unroll
+        for (int u = 0; u < OM_BATCH; ++u) {
+            const int e = b + u * OM_WAVE + lane;
+            cs[u] = e < e1 ? col[e] : -1;
+            neg |= (e < e1) & (cs[u] < 0);
+        }
+#pragma unroll
+        for (int u = 0; u < OM_BATCH; ++u) hi = max(hi, cs[u]);
     }
     hi = wave_max(hi);
     neg = wave_max(neg);
