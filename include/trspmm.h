@@ -141,8 +141,12 @@ typedef struct trspmm_plan_opts {
     int32_t tile_cap;         /* a window with more 2:4 tiles is split into chunks of this many (0 = 64) */
     int32_t tc_cap;           /* the same for TC sub-blocks (0 = 64) */
     int32_t res_cap;          /* the same for residual entries (0 = 2048) */
-    int32_t item_order;       /* 0: work items dealt round robin to the XCDs (the dispatch order);
-                                 1: each XCD takes a contiguous range of work items */
+    int32_t item_order;       /* 0: one workgroup per work item, dealt round robin to the XCDs
+                                    (the dispatch order);
+                                 1: one workgroup per work item, each XCD takes a contiguous range;
+                                 2: persistent tile kernel, a chip-sized grid whose workgroups
+                                    take work items from the plan's queue counter (launches of
+                                    one such plan must not overlap on concurrent streams) */
     int32_t reserved[2];
 } trspmm_plan_opts;
 

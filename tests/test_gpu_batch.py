@@ -55,11 +55,14 @@ def test_batched_equals_per_graph(cuda, mode, dtype):
             assert torch.count_nonzero(outs[i]) == 0
 
 
-def test_batched_graph_replay(cuda):
+@pytest.mark.parametrize("item_order", [0, 2])
+def test_batched_graph_replay(cuda, item_order):
     """Captured launch replayed with new operand contents and eager work in
-    between; every replay equals the eager result."""
+    between; every replay equals the eager result.  item_order 2 (persistent
+    workgroups on the plan's queue counters) checks that each launch leaves
+    the queue clean for the next replay."""
     gs = _batch_graphs()
-    batch = RoutedBatch(gs)
+    batch = RoutedBatch(gs, plan_opts={"item_order": item_order})
     K = 128
     Bcat = torch.empty(batch.ncols, K, device=cuda, dtype=torch.float16)
     C = torch.empty(batch.rows, K, device=cuda, dtype=torch.float32)
@@ -82,7 +85,7 @@ def test_autotune_picks_fastest_and_is_correct(cuda):
     B = torch.randn(c.n, K, generator=torch.Generator().manual_seed(2)).half().to(cuda)
     res = autotune(torch.from_numpy(c.rowptr), torch.from_numpy(c.col), torch.from_numpy(c.val), c.n, c.m, B,
                    iters=3, warmup=1)
-    assert len(res.table) == 120
+    assert len(res.table) == 180
     assert res.ms == min(ms for _, ms in res.table)
     A = RoutedSparse(torch.from_numpy(c.rowptr), torch.from_numpy(c.col), torch.from_numpy(c.val), c.n, m=c.m,
                      engine=res.best["engine"], slab=res.best["slab"], plan_opts=res.plan_opts(),
@@ -106,12 +109,12 @@ def test_plan_options(cuda, engine, slab):
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-5
 
 
-@pytest.mark.parametrize("item_order", [0, 1])
+@pytest.mark.parametrize("item_order", [0, 1, 2])
 @pytest.mark.parametrize("cap", [1, 3, 64])
 @pytest.mark.parametrize("engine", ["valu", "mfma"])
 def test_work_item_options(cuda, item_order, cap, engine):
     """Small caps split every window into chunks that add atomically into
-    zeroed rows; both item orders; the product is unchanged."""
+    zeroed rows; every item order; the product is unchanged."""
     c = CASES["hub"]()
     B = torch.randn(c.n, 128, generator=torch.Generator().manual_seed(4)).half().to(cuda)
     A = RoutedSparse(torch.from_numpy(c.rowptr), torch.from_numpy(c.col), torch.from_numpy(c.val), c.n, m=c.m,
@@ -121,3 +124,22 @@ def test_work_item_options(cuda, item_order, cap, engine):
     vq = torch.from_numpy(c.val).half().double().numpy()
     ref = oracle.csr_spmm64(c.rowptr, c.col, vq, B.cpu().double().numpy(), c.m)
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-5
+
+
+def test_persistent_queue_many_items_per_workgroup(cuda):
+    """More work items than the chip holds workgroups (5000 windows, 80000
+    rows), so each persistent workgroup takes many items from the queue; five
+    launches in a row on one plan each equal the one-workgroup-per-item
+    launch bit for bit (6 random columns per row: no window reaches a split
+    cap, so every row has one writer and no atomics reorder sums)."""
+    rng = np.random.default_rng(21)
+    n = 80000
+    c = g.csr_from_pairs(np.repeat(np.arange(n), 6), rng.integers(0, n, 6 * n), rng.standard_normal(6 * n), n, n)
+    B = torch.randn(c.n, 64, generator=torch.Generator().manual_seed(5)).half().to(cuda)
+    args = (torch.from_numpy(c.rowptr), torch.from_numpy(c.col), torch.from_numpy(c.val), c.n)
+    A0 = RoutedSparse(*args, m=c.m, plan_opts={"item_order": 0})
+    A2 = RoutedSparse(*args, m=c.m, plan_opts={"item_order": 2})
+    want = A0.matmul(B)
+    for rep in range(5):
+        print(f"[persistent] launch {rep}", flush=True)
+        assert torch.equal(A2.matmul(B), want), rep

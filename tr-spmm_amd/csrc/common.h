@@ -185,8 +185,15 @@ struct PlanDev {
     int32_t res_mfma = 0;
     // features per pass of the tile kernel (64 or 128)
     int32_t slab = 64;
-    // work item -> XCD: 0 = round robin (dispatch order), 1 = contiguous item ranges per XCD
+    // work item -> workgroup of the tile kernel: 0 = one workgroup per item in
+    // dispatch order (round robin over the XCDs), 1 = one per item with
+    // contiguous item ranges per XCD, 2 = persistent workgroups pulling items
+    // from the queue counters
     int32_t item_order = 0;
+    // persistent tile queue: queue[x * kQueueStride] = tickets taken from
+    // XCD queue x, queue[8 * kQueueStride] = workgroups that have left; all
+    // zero between launches
+    int32_t* queue = nullptr;
     const int4* titems = nullptr;
     int64_t n_titems = 0;
     const int4* ritems = nullptr;
@@ -208,6 +215,10 @@ struct SplitCaps {
 // L2.  The map is a bijection on [0, grid); callers skip results >= windows.
 constexpr int64_t kXcds = 8;
 inline int64_t xcd_grid(int64_t windows) { return ((windows + kXcds - 1) / kXcds) * kXcds; }
+// persistent tile queue (PlanDev::queue): 8 per-XCD counters and a done
+// counter, int32 each, 256 bytes apart
+constexpr int kQueueStride = 64;
+constexpr int kQueueInts = (int)(kXcds + 1) * kQueueStride;
 __device__ inline int64_t xcd_window(int64_t bid, int64_t grid) {
     const int64_t per = grid / kXcds;
     return (bid % kXcds) * per + bid / kXcds;

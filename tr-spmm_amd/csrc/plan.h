@@ -10,7 +10,7 @@ struct Plan {
     DevBuf tc_sub_off, tc_cols, tc_bits, tc_valoff, tc_vals, tc_dense;
     DevBuf sp_tile_off, sp_cols, sp_a, sp_idx;
     DevBuf res_rowptr, res_cols, res_vals, res_rec;
-    DevBuf titems, ritems, split_windows;
+    DevBuf titems, ritems, split_windows, queue;
     SplitCaps caps;
     trspmm_plan_opts opts{};   // requested engine / slab (0 = default)
     int finalize();

@@ -305,6 +305,7 @@ int Plan::finalize() {
     dev.titems = titems.as<int4>();
     dev.ritems = ritems.as<int4>();
     dev.split_windows = split_windows.as<int32_t>();
+    dev.queue = queue.as<int32_t>();
     return TRSPMM_OK;
 }
 
@@ -333,8 +334,9 @@ int Plan::build_items(hipStream_t s) {
     TRSPMM_HIP(hipMemcpyAsync(&tot[2], soff.as<int32_t>() + W, 4, hipMemcpyDeviceToHost, s));
     TRSPMM_HIP(hipStreamSynchronize(s));
     if ((rc = titems.alloc(16 * (int64_t)(tot[0] ? tot[0] : 1))) || (rc = ritems.alloc(16 * (int64_t)(tot[1] ? tot[1] : 1))) ||
-        (rc = split_windows.alloc(4 * (int64_t)(tot[2] ? tot[2] : 1))))
+        (rc = split_windows.alloc(4 * (int64_t)(tot[2] ? tot[2] : 1))) || (rc = queue.alloc(4 * kQueueInts)))
         return rc;
+    TRSPMM_HIP(hipMemsetAsync(queue.as<int32_t>(), 0, 4 * kQueueInts, s));      // the persistent tile queue starts empty
     hipLaunchKernelGGL(k_fill_items, dim3(blocks_for(W)), dim3(256), 0, s, tc_sub_off.as<int32_t>(),
                        sp_tile_off.as<int32_t>(), res_rowptr.as<int32_t>(), W, caps, (int)dev.res_mfma, toff.as<int32_t>(),
                        roff.as<int32_t>(), soff.as<int32_t>(), titems.as<int4>(), ritems.as<int4>(),
@@ -407,7 +409,7 @@ int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtyp
     P.dev.slab = tile_slab(P.opts);
     TRSPMM_CHECK(P.opts.tile_cap >= 0 && P.opts.tc_cap >= 0 && P.opts.res_cap >= 0,
                  "plan options: caps must be >= 0");
-    TRSPMM_CHECK(P.opts.item_order == 0 || P.opts.item_order == 1, "plan options: item_order must be 0 or 1");
+    TRSPMM_CHECK(P.opts.item_order >= 0 && P.opts.item_order <= 2, "plan options: item_order must be 0, 1 or 2");
     if (P.opts.tile_cap) P.caps.tiles = P.opts.tile_cap;
     if (P.opts.tc_cap) P.caps.tc_sub = P.opts.tc_cap;
     if (P.opts.res_cap) P.caps.res_entries = P.opts.res_cap;

@@ -22,6 +22,8 @@
 //                 tile kernel wrote).
 // Every output element is written by exactly one work item except in split
 // windows, whose rows are zeroed and accumulated with fp32 atomics.
+#include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <type_traits>
 
@@ -292,16 +294,19 @@ __device__ inline void run_steps(const PlanDev& p, const uint16_t* __restrict__ 
     }
 }
 
+// one work item of the tile kernel: the item's routes over all K features,
+// its 16-row tile written at the end (Bs is free again on return).  With a
+// ticket (the persistent queue's next index, an atomic's result still in
+// flight), the ticket is made uniform before the first pass's stores: every
+// load of the pass has been consumed there, so the wait for it costs nothing,
+// while waiting after the stores would wait for them too (one in-order
+// counter covers loads and stores)
 template <int F, typename DT>
-__global__ __launch_bounds__(64) void k_tile_routes(PlanDev p, const uint16_t* __restrict__ B, int64_t ldb, int K,
-                                                    float* __restrict__ C, int64_t ldc, uint32_t mask, int64_t grid) {
+__device__ __forceinline__ void tile_item(const PlanDev& p, int64_t it, const uint16_t* __restrict__ B, int64_t ldb,
+                                          int K, float* __restrict__ C, int64_t ldc, uint32_t mask, uint16_t* Bs,
+                                          int lane, int32_t* ticket) {
     constexpr int LDS_STRIDE = F + 4;     // keeps the 4 lane groups of an operand read on disjoint banks
     constexpr int NB = F / 16;
-    __shared__ __align__(16) uint16_t Bs[64 * LDS_STRIDE];
-
-    const int lane = threadIdx.x;
-    const int64_t it = p.item_order ? xcd_window(blockIdx.x, grid) : (int64_t)blockIdx.x;
-    if (it >= p.n_titems) return;
     const int4 item = p.titems[it];
     const int64_t w = item.x;
     const bool split = item.y != 0;
@@ -344,6 +349,7 @@ __global__ __launch_bounds__(64) void k_tile_routes(PlanDev p, const uint16_t* _
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) acc[0][nb] += acc[1][nb];
         }
+        if (ticket != nullptr && fp == 0) *ticket = __builtin_amdgcn_readfirstlane(*ticket);
         // accumulator element i of lane (kg, n) = C[16w + 4kg + i, f0 + 16nb + n]
         if (!split) {
             // full-line stores: the tile goes through LDS (the staging buffer is free
@@ -396,6 +402,73 @@ __global__ __launch_bounds__(64) void k_tile_routes(PlanDev p, const uint16_t* _
     }
 }
 
+// Work item -> workgroup (PlanDev::item_order):
+//   0  one workgroup per item, dispatch order (the dispatcher deals
+//      consecutive workgroups to the 8 XCDs round robin);
+//   1  one workgroup per item, contiguous item ranges per XCD (xcd_window);
+//   2  persistent: a grid sized to the chip's resident capacity (a multiple
+//      of 8).  The items are dealt to 8 queues, queue x holding items
+//      x, x + 8, x + 16, ... (the items XCD x runs under order 0); its
+//      counter sits on a 256-byte line of its own.  Workgroup b starts on
+//      queue b % 8, the queue of the XCD it was dispatched to, takes items
+//      with an atomic add (fetching the next index while it works on the
+//      current one), and when that queue is empty moves on to the other
+//      seven in turn, so an XCD that finishes early takes over the others'
+//      tails.  One counter for all items measured slower (about 10 ns per
+//      item serialised on one address), and so did tickets of 2 or 4 items
+//      (worse balance).  The last workgroup to leave (the done
+//      counter) puts every counter back to zero, so the next launch on the
+//      stream, or a graph replay, starts from a clean queue without a
+//      memset.  Launches of one plan on concurrent streams would share the
+//      counters and must not overlap.
+//      Order 0 measured faster on every workload timed (R-MAT scale 18,
+//      K = 128: 0.255 against 0.313 ms; 1024 batched BA graphs: 0.380 against
+//      0.403 ms), so it stays the default and 2 is an autotuner candidate.
+//      A likely cause: a wave that ends frees its slot while its stores
+//      drain, whereas a persistent wave's next loads share one in-order
+//      counter with those stores.
+template <int F, typename DT>
+__global__ __launch_bounds__(64) void k_tile_routes(PlanDev p, const uint16_t* __restrict__ B, int64_t ldb, int K,
+                                                    float* __restrict__ C, int64_t ldc, uint32_t mask, int64_t grid) {
+    __shared__ __align__(16) uint16_t Bs[64 * (F + 4)];
+    const int lane = threadIdx.x;
+    if (p.item_order != 2) {
+        const int64_t it = p.item_order == 1 ? xcd_window(blockIdx.x, grid) : (int64_t)blockIdx.x;
+        if (it < p.n_titems) tile_item<F, DT>(p, it, B, ldb, K, C, ldc, mask, Bs, lane, nullptr);
+        return;
+    }
+    // queue q's next ticket, uniform over the wave; ticket t of queue q is its item q + 8t
+    auto take = [&](int q) -> int32_t {
+        int32_t t = 0;
+        if (lane == 0) t = atomicAdd(p.queue + q * kQueueStride, 1);
+        return __builtin_amdgcn_readfirstlane(t);
+    };
+    int q = blockIdx.x % kXcds;
+    int32_t t = take(q);
+    for (int visited = 1;;) {
+        const int64_t it = q + kXcds * (int64_t)t;
+        if (it >= p.n_titems) {                         // queue q is empty: the next one, or leave after all 8
+            if (visited == kXcds) break;
+            ++visited;
+            q = (q + 1) % kXcds;
+            t = take(q);
+            continue;
+        }
+        t = 0;
+        if (lane == 0) t = atomicAdd(p.queue + q * kQueueStride, 1);      // next ticket, in flight during the item
+        tile_item<F, DT>(p, it, B, ldb, K, C, ldc, mask, Bs, lane, &t);
+        __syncthreads();                                // the item's last LDS reads are done before the next's writes
+    }
+    if (lane == 0) {
+        __threadfence();
+        int32_t* done = p.queue + kXcds * kQueueStride;
+        if (atomicAdd(done, 1) == (int32_t)gridDim.x - 1) {
+            for (int x = 0; x < kXcds; ++x) atomicExch(p.queue + x * kQueueStride, 0);
+            atomicExch(done, 0);
+        }
+    }
+}
+
 // ============================================================ residual route
 //
 // One wave64 per work item.  The item's entries (the 16 rows of a window, or
@@ -426,7 +499,7 @@ __global__ __launch_bounds__(256) void k_residual(PlanDev p, const uint16_t* __r
     float* tile = tile_all[wv];
     int32_t* rp = rp_all[wv];
     // blocks (4 items each) are what the dispatcher spreads over the XCDs
-    const int64_t it = (p.item_order ? xcd_window(blockIdx.x, grid) : (int64_t)blockIdx.x) * 4 + wv;
+    const int64_t it = (p.item_order == 1 ? xcd_window(blockIdx.x, grid) : (int64_t)blockIdx.x) * 4 + wv;
     if (it >= p.n_ritems) return;                       // the whole wave leaves together (no block barriers below)
     const int4 item = p.ritems[it];
     const int64_t w = item.x;
@@ -569,6 +642,29 @@ __global__ __launch_bounds__(256) void k_zero_rows(const int32_t* __restrict__ s
     }
 }
 
+// workgroups of a 64-thread kernel the current device holds at once (CUs x
+// resident workgroups per CU), remembered per device; -1 after a failed query
+template <typename Kernel>
+int64_t resident_blocks(Kernel kernel) {
+    static std::atomic<int64_t> known[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    if (dev >= 0 && dev < 64) {
+        const int64_t v = known[dev].load(std::memory_order_relaxed);
+        if (v > 0) return v;
+    }
+    int per_cu = 0, cus = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess || per_cu <= 0 || cus <= 0) {
+        set_error(std::string("occupancy query for the persistent tile kernel failed: ") + hipGetErrorString(e));
+        return -1;
+    }
+    const int64_t v = (int64_t)per_cu * cus;
+    if (dev >= 0 && dev < 64) known[dev].store(v, std::memory_order_relaxed);
+    return v;
+}
+
 template <typename DT>
 int launch_typed(const PlanDev& p, const void* B, int64_t ldb, int64_t K, float* C, int64_t ldc, uint32_t mask,
                  hipStream_t s) {
@@ -578,7 +674,16 @@ int launch_typed(const PlanDev& p, const void* B, int64_t ldb, int64_t K, float*
         TRSPMM_HIP(hipGetLastError());
     }
     if (p.n_titems > 0) {
-        const int64_t grid = xcd_grid(p.n_titems);
+        int64_t grid = xcd_grid(p.n_titems);
+        if (p.item_order == 2) {
+            TRSPMM_CHECK(p.queue != nullptr, "plan has no work queue");
+            TRSPMM_CHECK(p.n_titems < 0x7fff0000, "too many work items for the persistent queue");
+            const int64_t resident = p.slab == 128 ? resident_blocks(k_tile_routes<128, DT>)
+                                                   : resident_blocks(k_tile_routes<64, DT>);
+            if (resident < 0) return TRSPMM_EHIP;
+            // a multiple of 8, so every XCD's queue starts with workgroups of its own
+            grid = std::min<int64_t>(xcd_grid(p.n_titems), std::max<int64_t>(kXcds, resident / kXcds * kXcds));
+        }
         TRSPMM_CHECK(grid <= 0x7fffffff, "too many work items");
         if (p.slab == 128)
             hipLaunchKernelGGL((k_tile_routes<128, DT>), dim3((unsigned)grid), dim3(64), 0, s, p, (const uint16_t*)B,
