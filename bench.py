@@ -258,7 +258,7 @@ def route_rows(meta, rows):
 
 
 def measure_routes(csr, K, mode, steps, warmup):
-    """Config 3 view of one routing: per-route nnz, rows and launch times."""
+    """Per-route view of one routing (the three-route workload): per-route nnz, rows and launch times."""
     import torch
 
     from trspmm import ROUTE_RESIDUAL, ROUTE_SPTC, ROUTE_TC, graphs
@@ -286,7 +286,7 @@ def measure_routes(csr, K, mode, steps, warmup):
 
 
 def measure_batched(steps, warmup, check=True, count=1024, K=128):
-    """Config 5: `count` BA graphs in one launch vs one launch per graph."""
+    """`count` Barabasi-Albert graphs (m = 3) in one launch vs one launch per graph."""
     import torch
 
     from trspmm import graphs

@@ -2,7 +2,7 @@
 
 The reference's permutation comes from datasketch's SHA-1 MinHash, which is
 not available here, so the reordering is checked by what must hold for any
-correct TCA pass (SURVEY of the reference: "pinned by invariants only"):
+correct TCA pass:
 a valid permutation, every row carried over intact with ascending columns,
 nnz preserved; and by its purpose, on a matrix with planted clusters whose
 rows were shuffled: after reordering, 16-row windows hold far fewer
