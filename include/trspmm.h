@@ -275,7 +275,8 @@ int trspmm_match_2to4_bucket(const int32_t* rowptr, const int32_t* col, int64_t 
  *       no slot, which ones depending on scheduling)
  *   [10] windows holding a negative column index (not counted above)
  * Row offsets are clamped to [0, nnz], so a malformed rowptr cannot read
- * outside col. */
+ * outside col.  Takes 4 * ceil(rows / 16) + 256 bytes of stream-ordered
+ * scratch (hipMallocAsync / hipFreeAsync on `stream`) for the window order. */
 #define TRSPMM_ONLINE_COUNTERS 11
 int trspmm_match_2to4_online(const int32_t* rowptr, const int32_t* col, int64_t rows, int64_t nnz,
                              int dense_threshold, int t_max, unsigned long long* counters, void* stream);

@@ -1,7 +1,7 @@
 // Device statistics of the reference's matching_utils_online.match_2to4_online
 // (TR-source/Block/block_matching_online.cu:85-649): the sliding-window 1:2 /
 // 2:4 matching with dense-pool routing, computed on the GPU, one 64-lane
-// workgroup per 16-row window.
+// workgroup per 16-row window, the windows launched heaviest first.
 //
 // Per window (rows r0 .. r0+15):
 //   1. table: the reference puts the window's distinct columns into a
@@ -15,9 +15,9 @@
 //      reference's, whatever the scheduling.  The columns are collected in a
 //      scattered-hash table of 4096 slots (at most half full, so probes stay
 //      short whatever the column ids), then laid out: each at its home slot
-//      when no two share one, else by a union-find "next free slot" pass
-//      over the columns in ascending order (the sequential insertion without
-//      its probe walks);
+//      when no two share one, else by inserting them in ascending order, the
+//      wave finding each one's first free slot in a register bitmap of the
+//      table with one ballot;
 //   2. 1:2 pairs: each unpaired column i takes, among columns i+1 .. i+32 not
 //      yet paired, the first with no common row, else the first with the
 //      fewest common rows if <= t_max; else it goes to the pool (fine);
@@ -63,7 +63,6 @@ struct OnlineLds {
         struct {                          // layout in the reference's table, then its order
             uint32_t home_cnt[OM_SLOTS];  // columns whose home is this slot
             uint16_t owner[OM_SLOTS];     // column (index into keys) in this slot, OM_FREE if none
-            uint16_t parent[OM_SLOTS];    // union-find: next slot that may be free
             uint16_t cmask[OM_SLOTS];     // row masks in table order
         } t;
     } a;
@@ -75,7 +74,6 @@ struct OnlineLds {
             uint16_t c1[OM_SLOTS / 2], c2[OM_SLOTS / 2];   // pair: column positions in table order
             uint16_t pool[OM_SLOTS];
             uint8_t used[OM_SLOTS / 2];
-            uint8_t matched[OM_SLOTS];
         } p;
     } b;
 };
@@ -117,6 +115,61 @@ __device__ inline int wave_max(int v) {
 __device__ inline int first_lane(bool pred) {
     const unsigned long long m = __ballot(pred);
     return m ? (int)__builtin_ctzll(m) : -1;
+}
+
+// Window order.  A window's cost grows with its nonzeros (collection and
+// cut-off passes) and distinct columns (layout, pairing, grouping), up to a
+// few ms for R-MAT hub windows against tens of us for most.  Workgroup b takes
+// order[b]: the windows sorted heaviest first by floor(log2(nnz + 1)), so the
+// hub windows start first and, consecutive in the order, go to different
+// XCDs (the dispatcher sends workgroup b to XCD b % 8).  Taking window b
+// instead left one XCD running R-MAT's hub windows (mostly multiples of 8)
+// long after the others had finished.
+constexpr int OM_CLASSES = 32;
+
+__device__ inline int window_class(const int32_t* __restrict__ rowptr, int64_t rows, int64_t nnz, int64_t w) {
+    const int64_t r0 = w * OM_ROWS, r1 = min<int64_t>(r0 + OM_ROWS, rows);
+    const int64_t a = min<int64_t>(max<int64_t>(rowptr[r0], 0), nnz), b = min<int64_t>(max<int64_t>(rowptr[r1], 0), nnz);
+    return __clz((int)(b > a ? b - a : 0) + 1);           // 1 (heaviest) .. 31 (empty or one nonzero)
+}
+
+// cnt[c] += windows of class c
+__global__ __launch_bounds__(256) void k_window_classes(const int32_t* __restrict__ rowptr, int64_t rows, int64_t nnz,
+                                                        int64_t windows, int32_t* __restrict__ cnt) {
+    __shared__ int32_t c[OM_CLASSES];
+    if (threadIdx.x < OM_CLASSES) c[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < windows) atomicAdd(&c[window_class(rowptr, rows, nnz, w)], 1);
+    __syncthreads();
+    if (threadIdx.x < OM_CLASSES && c[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], c[threadIdx.x]);
+}
+
+// order[start(c) + k] = the k-th window of class c to arrive (fill[] counts arrivals)
+__global__ __launch_bounds__(256) void k_window_order(const int32_t* __restrict__ rowptr, int64_t rows, int64_t nnz,
+                                                      int64_t windows, const int32_t* __restrict__ cnt,
+                                                      int32_t* __restrict__ fill, int32_t* __restrict__ order) {
+    __shared__ int32_t start[OM_CLASSES], local[OM_CLASSES], base[OM_CLASSES];
+    if (threadIdx.x == 0) {
+        int32_t run = 0;
+        for (int k = 0; k < OM_CLASSES; ++k) {
+            start[k] = run;
+            run += cnt[k];
+        }
+    }
+    if (threadIdx.x < OM_CLASSES) local[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int cls = 0, rank = 0;
+    if (w < windows) {
+        cls = window_class(rowptr, rows, nnz, w);
+        rank = atomicAdd(&local[cls], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < OM_CLASSES && local[threadIdx.x])
+        base[threadIdx.x] = start[threadIdx.x] + atomicAdd(&fill[threadIdx.x], local[threadIdx.x]);
+    __syncthreads();
+    if (w < windows) order[base[cls] + rank] = (int32_t)w;
 }
 
 __device__ inline int dedup_slot(int c) { return (int)(((uint32_t)c * 2654435761u) >> 20); }   // 12 bits
@@ -183,11 +236,12 @@ __device__ int collect(OnlineLds& s, OnlineCtl& ctl, const int32_t* __restrict__
 __global__ __launch_bounds__(OM_WAVE) void k_match_online(const int32_t* __restrict__ rowptr,
                                                           const int32_t* __restrict__ col, int64_t rows,
                                                           int64_t nnz, int dense_threshold, int t_max,
+                                                          const int32_t* __restrict__ order,
                                                           unsigned long long* __restrict__ out) {
     __shared__ OnlineLds s;
     __shared__ OnlineCtl ctl;
     const int lane = lane_id();
-    const int64_t r0 = (int64_t)blockIdx.x * OM_ROWS;
+    const int64_t r0 = (int64_t)order[blockIdx.x] * OM_ROWS;
     const int win_rows = (int)min<int64_t>(OM_ROWS, rows - r0);
 
     // ---- 1. the window's columns
@@ -289,7 +343,6 @@ __global__ __launch_bounds__(OM_WAVE) void k_match_online(const int32_t* __restr
     for (int t = lane; t < OM_SLOTS; t += OM_WAVE) {
         s.a.t.home_cnt[t] = 0;
         s.a.t.owner[t] = OM_FREE;
-        s.a.t.parent[t] = (uint16_t)t;
     }
     __syncthreads();
     int clash = 0;
@@ -318,20 +371,32 @@ __global__ __launch_bounds__(OM_WAVE) void k_match_online(const int32_t* __restr
                 __syncthreads();
             }
         }
-        // each column takes the first free slot from its home on (cyclic):
-        // find() follows parent links with path halving; a filled slot links
-        // to the next one.  k < n <= OM_SLOTS columns are placed when column
-        // k is, so a free slot (a root) exists and find() ends.
-        if (lane == 0) {
-            for (int k = 0; k < n; ++k) {
-                int x = (int)(s.b.keys[k] >> 32) & (OM_SLOTS - 1);
-                while (s.a.t.parent[x] != x) {
-                    const int up = s.a.t.parent[x];
-                    s.a.t.parent[x] = s.a.t.parent[up];
-                    x = up;
+        // each column, in ascending order, takes the first free slot from its
+        // home on (cyclic).  The free slots are a bitmap in registers: lane L
+        // (L < 32) holds slots 64L .. 64L + 63.  One ballot finds the first
+        // lane at or after the home's word with a free slot there, the lowest
+        // set bit of its word is the slot; with none, the search wraps to the
+        // first free slot of the table (k < n <= OM_SLOTS columns are placed
+        // when column k is, so one exists).
+        uint64_t free_w = lane < OM_SLOTS / 64 ? ~0ull : 0ull;
+        for (int k0 = 0; k0 < n; k0 += OM_WAVE) {
+            const int home_l = k0 + lane < n ? (int)(s.b.keys[k0 + lane] >> 32) & (OM_SLOTS - 1) : 0;
+            const int cnt = min(OM_WAVE, n - k0);
+            for (int j = 0; j < cnt; ++j) {
+                const int h = __builtin_amdgcn_readlane(home_l, j);
+                const int w = h >> 6;
+                uint64_t cand = lane > w ? free_w : (lane == w ? free_w & (~0ull << (h & 63)) : 0ull);
+                unsigned long long m = __ballot(cand != 0);
+                if (m == 0) {
+                    cand = free_w;
+                    m = __ballot(cand != 0);
                 }
-                s.a.t.owner[x] = (uint16_t)k;
-                s.a.t.parent[x] = (uint16_t)((x + 1) & (OM_SLOTS - 1));
+                const int f = (int)__builtin_ctzll(m);
+                const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)cand, f);
+                const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(cand >> 32), f);
+                const int bit = lo ? __builtin_ctz(lo) : 32 + __builtin_ctz(hi);
+                if (lane == f) free_w &= ~(1ull << bit);
+                if (lane == 0) s.a.t.owner[f * 64 + bit] = (uint16_t)(k0 + j);
             }
         }
     }
@@ -354,43 +419,48 @@ __global__ __launch_bounds__(OM_WAVE) void k_match_online(const int32_t* __restr
     const uint16_t* cmask = s.a.t.cmask;
 
     // ---- 2. 1:2 pairs
-    for (int t = lane; t < n; t += OM_WAVE) s.b.p.matched[t] = 0;
-    __syncthreads();
+    // Column i's candidates i + 1 .. i + 32 sit on lanes 0 .. 31 (lane l holds
+    // cmask[i + 1 + l], loaded one column ahead).  Which of the next 64
+    // columns are already paired is a wave-uniform bit mask that shifts with
+    // i (a partner lies at most 32 ahead), so no step waits on shared memory.
     int np = 0, nd = 0, fine = 0;
+    uint64_t taken = 0;                                    // bit d: column i + d is paired
+    uint32_t cand = cmask[min(1 + lane, n - 1)];
+    uint32_t mi = cmask[0];
     for (int i = 0; i < n; ++i) {
-        if (s.b.p.matched[i]) continue;          // same LDS byte on every lane: uniform
-        const uint32_t mi = cmask[i];
-        const int j = min(i + 1 + lane, n - 1);
-        const bool valid = lane < OM_PAIR_SEARCH && i + 1 + lane < n && !s.b.p.matched[j];
-        const int cost = valid ? __popc(mi & cmask[j]) : 64;
-        int best = -1;
-        for (int c = 0; c <= min(t_max, OM_ROWS); ++c) {
-            const int l = first_lane(cost == c);
-            if (l >= 0) {
-                best = i + 1 + l;
-                break;
+        const uint32_t cand_next = cmask[min(i + 2 + lane, n - 1)];
+        if (!(taken & 1)) {
+            const bool valid = (lane < OM_PAIR_SEARCH) & (i + 1 + lane < n) & !((taken >> (1 + (lane & 31))) & 1);
+            const int cost = valid ? __popc(mi & cand) : 64;
+            int best = -1;
+            for (int c = 0; c <= min(t_max, OM_ROWS); ++c) {
+                const int l = first_lane(cost == c);
+                if (l >= 0) {
+                    best = l;
+                    break;
+                }
             }
-        }
-        if (lane == 0) {
-            s.b.p.matched[i] = 1;
             if (best >= 0) {
-                s.b.p.matched[best] = 1;
-                s.b.p.g1[np] = (uint16_t)(mi & cmask[best]);
-                s.b.p.g2[np] = (uint16_t)(mi | cmask[best]);
-                s.b.p.c1[np] = (uint16_t)i;
-                s.b.p.c2[np] = (uint16_t)best;
+                const uint32_t mb = __builtin_amdgcn_readlane(cand, best);
+                if (lane == 0) {
+                    s.b.p.g1[np] = (uint16_t)(mi & mb);
+                    s.b.p.g2[np] = (uint16_t)(mi | mb);
+                    s.b.p.c1[np] = (uint16_t)i;
+                    s.b.p.c2[np] = (uint16_t)(i + 1 + best);
+                }
+                taken |= 2ull << best;
+                ++np;
             } else {
-                s.b.p.pool[nd] = (uint16_t)i;
+                if (lane == 0) s.b.p.pool[nd] = (uint16_t)i;
+                ++nd;
+                ++fine;
             }
         }
-        if (best >= 0) {
-            ++np;
-        } else {
-            ++nd;
-            ++fine;
-        }
-        __syncthreads();
+        taken >>= 1;
+        mi = __builtin_amdgcn_readlane(cand, 0);           // cmask[i + 1]
+        cand = cand_next;
     }
+    __syncthreads();
 
     // ---- 3. 2:4 groups
     for (int t = lane; t < np; t += OM_WAVE) s.b.p.used[t] = 0;
@@ -469,11 +539,23 @@ int match_online_device(const int32_t* d_rowptr, const int32_t* d_col, int64_t r
     TRSPMM_CHECK(nnz >= 0 && nnz < (int64_t)1 << 31, "match_2to4_online: nnz must be in [0, 2^31)");
     TRSPMM_CHECK(dense_threshold >= 1, "match_2to4_online: dense_threshold must be >= 1");
     const int64_t windows = (rows + OM_ROWS - 1) / OM_ROWS;
+    // stream-ordered scratch: class counts, class fill counters, the window order
+    int32_t* scratch = nullptr;
+    TRSPMM_HIP(hipMallocAsync((void**)&scratch, 4 * (2 * OM_CLASSES + windows), stream));
+    int32_t* cnt = scratch;
+    int32_t* fill = scratch + OM_CLASSES;
+    int32_t* order = scratch + 2 * OM_CLASSES;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, d_out, TRSPMM_ONLINE_COUNTERS);
-    TRSPMM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, (unsigned long long*)scratch, OM_CLASSES);
+    const unsigned blocks = (unsigned)((windows + 255) / 256);
+    hipLaunchKernelGGL(k_window_classes, dim3(blocks), dim3(256), 0, stream, d_rowptr, rows, nnz, windows, cnt);
+    hipLaunchKernelGGL(k_window_order, dim3(blocks), dim3(256), 0, stream, d_rowptr, rows, nnz, windows, cnt, fill,
+                       order);
     hipLaunchKernelGGL(k_match_online, dim3((unsigned)windows), dim3(OM_WAVE), 0, stream, d_rowptr, d_col, rows, nnz,
-                       dense_threshold, t_max, d_out);
-    TRSPMM_HIP(hipGetLastError());
+                       dense_threshold, t_max, order, d_out);
+    const hipError_t launched = hipGetLastError();
+    TRSPMM_HIP(hipFreeAsync(scratch, stream));
+    TRSPMM_HIP(launched);
     return TRSPMM_OK;
 }
 
