@@ -133,7 +133,7 @@ typedef struct trspmm_plan trspmm_plan;
 /* Plan options (no reference counterpart; the reference's kernels are fixed).
  * Zero-initialised = defaults (environment TRSPMM_RESIDUAL / TRSPMM_SLAB, else
  * the matrix-core residual engine and 64-feature slabs; caps 64 / 64 / 2048;
- * round-robin work items). */
+ * heaviest-first work items dealt round robin). */
 enum { TRSPMM_ENGINE_DEFAULT = 0, TRSPMM_ENGINE_VALU = 1, TRSPMM_ENGINE_MFMA = 2 };
 typedef struct trspmm_plan_opts {
     int32_t residual_engine;  /* TRSPMM_ENGINE_* */
@@ -141,12 +141,16 @@ typedef struct trspmm_plan_opts {
     int32_t tile_cap;         /* a window with more 2:4 tiles is split into chunks of this many (0 = 64) */
     int32_t tc_cap;           /* the same for TC sub-blocks (0 = 64) */
     int32_t res_cap;          /* the same for residual entries (0 = 2048) */
-    int32_t item_order;       /* 0: one workgroup per work item, dealt round robin to the XCDs
-                                    (the dispatch order);
-                                 1: one workgroup per work item, each XCD takes a contiguous range;
+    int32_t item_order;       /* 0: one workgroup per work item, the items sorted heaviest first
+                                    (by gathered rows, in classes of powers of two) and dealt
+                                    round robin to the XCDs (the dispatch order);
+                                 1: one workgroup per work item in window order, each XCD takes
+                                    a contiguous range;
                                  2: persistent tile kernel, a chip-sized grid whose workgroups
-                                    take work items from the plan's queue counter (launches of
-                                    one such plan must not overlap on concurrent streams) */
+                                    take the heaviest-first items from the plan's queue counters
+                                    (launches of one such plan must not overlap on concurrent
+                                    streams);
+                                 3: as 0, with the items in window order */
     int32_t reserved[2];
 } trspmm_plan_opts;
 

@@ -109,7 +109,7 @@ def test_plan_options(cuda, engine, slab):
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-5
 
 
-@pytest.mark.parametrize("item_order", [0, 1, 2])
+@pytest.mark.parametrize("item_order", [0, 1, 2, 3])
 @pytest.mark.parametrize("cap", [1, 3, 64])
 @pytest.mark.parametrize("engine", ["valu", "mfma"])
 def test_work_item_options(cuda, item_order, cap, engine):
@@ -139,7 +139,9 @@ def test_persistent_queue_many_items_per_workgroup(cuda):
     args = (torch.from_numpy(c.rowptr), torch.from_numpy(c.col), torch.from_numpy(c.val), c.n)
     A0 = RoutedSparse(*args, m=c.m, plan_opts={"item_order": 0})
     A2 = RoutedSparse(*args, m=c.m, plan_opts={"item_order": 2})
-    want = A0.matmul(B)
+    A3 = RoutedSparse(*args, m=c.m, plan_opts={"item_order": 3})
+    want = A3.matmul(B)                                # window order: heaviest-first sorting changes nothing
+    assert torch.equal(A0.matmul(B), want)
     for rep in range(5):
         print(f"[persistent] launch {rep}", flush=True)
         assert torch.equal(A2.matmul(B), want), rep

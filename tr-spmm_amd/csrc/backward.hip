@@ -538,6 +538,30 @@ int bwd_plan_build(const int32_t* t_rowptr, const int32_t* t_col, const float* t
         items.push_back(int4{0, (int32_t)r, (int32_t)r1, 0});
         r = r1;
     }
+    // Heaviest items first, by the class floor(log2(cost)) with cost = entries
+    // + 16 per row (each row loads its B row and stores its dB row), stably so
+    // that a class keeps row order.  Blocks go to the 8 XCDs round robin, so the
+    // hub rows' chunks, which sit together at low row ids on power-law graphs,
+    // start first and spread over all XCDs.
+    {
+        std::vector<uint8_t> cls(items.size());
+        int64_t count[64] = {};
+        for (size_t k = 0; k < items.size(); ++k) {
+            const int4 x = items[k];
+            const uint64_t cost = x.x == 1 ? (uint64_t)(x.w - x.z) + 16
+                                           : (uint64_t)(rp[x.z] - rp[x.y]) + 16 * (uint64_t)(x.z - x.y);
+            cls[k] = (uint8_t)__builtin_clzll(cost);     // smaller = heavier
+            ++count[cls[k]];
+        }
+        int64_t start[64], run = 0;
+        for (int c = 0; c < 64; ++c) {
+            start[c] = run;
+            run += count[c];
+        }
+        std::vector<int4> sorted(items.size());
+        for (size_t k = 0; k < items.size(); ++k) sorted[start[cls[k]]++] = items[k];
+        items.swap(sorted);
+    }
     P.n_items = (int64_t)items.size();
     P.n_zero = (int64_t)zero.size();
     int rc;

@@ -185,10 +185,11 @@ struct PlanDev {
     int32_t res_mfma = 0;
     // features per pass of the tile kernel (64 or 128)
     int32_t slab = 64;
-    // work item -> workgroup of the tile kernel: 0 = one workgroup per item in
-    // dispatch order (round robin over the XCDs), 1 = one per item with
-    // contiguous item ranges per XCD, 2 = persistent workgroups pulling items
-    // from the queue counters
+    // work item -> workgroup of the tile kernel: 0 = one workgroup per item,
+    // items heaviest first, dealt round robin over the XCDs; 1 = one per item,
+    // window order, contiguous item ranges per XCD; 2 = persistent workgroups
+    // pulling the heaviest-first items from the queue counters; 3 = as 0 with
+    // the items in window order
     int32_t item_order = 0;
     // persistent tile queue: queue[x * kQueueStride] = tickets taken from
     // XCD queue x, queue[8 * kQueueStride] = workgroups that have left; all

@@ -14,9 +14,12 @@
 //   * values converted once to the plan's 16-bit type (fp16 or bf16),
 //   * columns outside [0, ncols) replaced by -1 (contribute nothing), the
 //     bounds rule the reference kernels apply (mGCNkernel_online.cu:296-306),
-//   * work-item lists for the tile-route and residual-route kernels.
+//   * work-item lists for the tile-route and residual-route kernels, the
+//     tile items ordered heaviest first (except for item_order 1 and 3).
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "plan.h"
 #include "scan.h"
@@ -309,6 +312,54 @@ int Plan::finalize() {
     return TRSPMM_OK;
 }
 
+// Tile work items heaviest first.  An item's cost is its gathered B rows
+// (32 per step of 4 TC sub-blocks, 64 per step of 4 2:4 tiles, 32 per step of
+// 32 residual entries) plus its store; the items are sorted by the class
+// floor(log2(cost)), stably, so items of one class keep window order (and the
+// column locality of neighbouring windows).  The dispatcher deals consecutive
+// workgroups to the 8 XCDs round robin, so the heavy items start first and
+// spread over all XCDs; in window order, R-MAT's hub windows (mostly
+// multiples of 8) shared one XCD.
+static int sort_items_heaviest_first(Plan& P, int64_t n, hipStream_t s) {
+    const int64_t W = P.dev.windows;
+    std::vector<int4> it((size_t)n), out((size_t)n);
+    std::vector<int32_t> so((size_t)W + 1), to((size_t)W + 1), ro((size_t)(16 * W + 1));
+    TRSPMM_HIP(hipMemcpyAsync(it.data(), P.titems.as<int4>(), 16 * n, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipMemcpyAsync(so.data(), P.tc_sub_off.as<int32_t>(), 4 * (W + 1), hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipMemcpyAsync(to.data(), P.sp_tile_off.as<int32_t>(), 4 * (W + 1), hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipMemcpyAsync(ro.data(), P.res_rowptr.as<int32_t>(), 4 * (16 * W + 1), hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipStreamSynchronize(s));
+    std::vector<uint8_t> cls((size_t)n);
+    int64_t count[64] = {};
+    for (int64_t k = 0; k < n; ++k) {
+        const int4 x = it[k];
+        int64_t tc = 0, sp = 0, res = 0;
+        if (x.y == 0) {
+            tc = so[x.x + 1] - so[x.x];
+            sp = to[x.x + 1] - to[x.x];
+            res = P.dev.res_mfma ? ro[16 * (int64_t)x.x + 16] - ro[16 * (int64_t)x.x] : 0;
+        } else if (x.y == 1) {
+            tc = x.w - x.z;
+        } else if (x.y == 2) {
+            sp = x.w - x.z;
+        } else {
+            res = x.w - x.z;
+        }
+        const uint64_t cost = 32 * ((tc + 3) / 4) + 64 * ((sp + 3) / 4) + 32 * ((res + 31) / 32) + 16;
+        cls[k] = (uint8_t)__builtin_clzll(cost);                 // smaller = heavier
+        ++count[cls[k]];
+    }
+    int64_t start[64];
+    for (int64_t c = 0, run = 0; c < 64; ++c) {
+        start[c] = run;
+        run += count[c];
+    }
+    for (int64_t k = 0; k < n; ++k) out[start[cls[k]]++] = it[k];
+    TRSPMM_HIP(hipMemcpyAsync(P.titems.as<int4>(), out.data(), 16 * n, hipMemcpyHostToDevice, s));
+    TRSPMM_HIP(hipStreamSynchronize(s));
+    return TRSPMM_OK;
+}
+
 int Plan::build_items(hipStream_t s) {
     const int64_t W = dev.windows;
     if (W == 0) {
@@ -343,6 +394,8 @@ int Plan::build_items(hipStream_t s) {
                        split_windows.as<int32_t>());
     TRSPMM_HIP(hipGetLastError());
     TRSPMM_HIP(hipStreamSynchronize(s));
+    if ((opts.item_order == 0 || opts.item_order == 2) && tot[0] > 1 && (rc = sort_items_heaviest_first(*this, tot[0], s)))
+        return rc;
     dev.n_titems = tot[0];
     dev.n_ritems = tot[1];
     dev.n_split = tot[2];
@@ -409,7 +462,7 @@ int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtyp
     P.dev.slab = tile_slab(P.opts);
     TRSPMM_CHECK(P.opts.tile_cap >= 0 && P.opts.tc_cap >= 0 && P.opts.res_cap >= 0,
                  "plan options: caps must be >= 0");
-    TRSPMM_CHECK(P.opts.item_order >= 0 && P.opts.item_order <= 2, "plan options: item_order must be 0, 1 or 2");
+    TRSPMM_CHECK(P.opts.item_order >= 0 && P.opts.item_order <= 3, "plan options: item_order must be 0, 1, 2 or 3");
     if (P.opts.tile_cap) P.caps.tiles = P.opts.tile_cap;
     if (P.opts.tc_cap) P.caps.tc_sub = P.opts.tc_cap;
     if (P.opts.res_cap) P.caps.res_entries = P.opts.res_cap;

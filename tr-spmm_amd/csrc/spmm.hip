@@ -403,12 +403,16 @@ __device__ __forceinline__ void tile_item(const PlanDev& p, int64_t it, const ui
 }
 
 // Work item -> workgroup (PlanDev::item_order):
-//   0  one workgroup per item, dispatch order (the dispatcher deals
-//      consecutive workgroups to the 8 XCDs round robin);
-//   1  one workgroup per item, contiguous item ranges per XCD (xcd_window);
+//   0  one workgroup per item in the plan's item order, heaviest first (the
+//      dispatcher deals consecutive workgroups to the 8 XCDs round robin, so
+//      the heavy items start first and spread over all XCDs);
+//   3  the same with the items in window order;
+//   1  one workgroup per item, window order, contiguous item ranges per XCD
+//      (xcd_window);
 //   2  persistent: a grid sized to the chip's resident capacity (a multiple
-//      of 8).  The items are dealt to 8 queues, queue x holding items
-//      x, x + 8, x + 16, ... (the items XCD x runs under order 0); its
+//      of 8).  The items (heaviest first) are dealt to 8 queues, queue x
+//      holding items x, x + 8, x + 16, ... (the items XCD x runs under
+//      order 0); its
 //      counter sits on a 256-byte line of its own.  Workgroup b starts on
 //      queue b % 8, the queue of the XCD it was dispatched to, takes items
 //      with an atomic add (fetching the next index while it works on the
@@ -422,8 +426,8 @@ __device__ __forceinline__ void tile_item(const PlanDev& p, int64_t it, const ui
 //      memset.  Launches of one plan on concurrent streams would share the
 //      counters and must not overlap.
 //      Order 0 measured faster on every workload timed (R-MAT scale 18,
-//      K = 128: 0.255 against 0.313 ms; 1024 batched BA graphs: 0.380 against
-//      0.403 ms), so it stays the default and 2 is an autotuner candidate.
+//      K = 128: 0.241 against 0.287 ms; 1024 batched BA graphs: 0.366 against
+//      0.383 ms), so it stays the default and 2 is an autotuner candidate.
 //      A likely cause: a wave that ends frees its slot while its stores
 //      drain, whereas a persistent wave's next loads share one in-order
 //      counter with those stores.

@@ -169,9 +169,10 @@ class Plan:
         """engine: None (default), "valu" or "mfma" residual engine; slab: None, 64 or 128
         features per pass of the tile kernel; *_cap: split a window into work items of at
         most this many 2:4 tiles / TC sub-blocks / residual entries (0 = default);
-        item_order: 0 = one workgroup per item, round robin over the XCDs, 1 = contiguous item
-        ranges per XCD, 2 = persistent workgroups taking items from the plan's queue counter
-        (launches of such a plan must not overlap on concurrent streams)."""
+        item_order: 0 = one workgroup per item, items heaviest first, round robin over the
+        XCDs; 1 = window order, contiguous item ranges per XCD; 2 = persistent workgroups taking
+        the heaviest-first items from the plan's queue counters (launches of such a plan must
+        not overlap on concurrent streams); 3 = as 0 in window order."""
         opts = _lib.PlanOpts()
         opts.residual_engine = {None: _lib.ENGINE_DEFAULT, "valu": _lib.ENGINE_VALU, "mfma": _lib.ENGINE_MFMA}[engine]
         opts.slab = int(slab or 0)

@@ -33,7 +33,7 @@ ROUTER_CANDIDATES = [
     {"mode": "exact", "density": 6},
 ]
 KERNEL_CANDIDATES = [{"engine": e, "slab": s, "tile_cap": c, "item_order": o}
-                     for e, s, c, o in itertools.product(["valu", "mfma"], [64, 128], [32, 64, 128], [0, 1, 2])]
+                     for e, s, c, o in itertools.product(["valu", "mfma"], [64, 128], [32, 64, 128], [0, 2, 3])]
 
 
 @dataclass
