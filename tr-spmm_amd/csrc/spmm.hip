@@ -15,11 +15,14 @@
 //                 registers while the current step computes) and read back
 //                 transposed with ds_read_b64_tr_b16; writes the window's
 //                 16 x K tile,
-//   k_residual    one wave64 per row window: each row's residual entries are
-//                 split over the lane groups, 16-byte B loads per lane (8 per
-//                 lane in flight), fp32 FMA, cross-group butterfly reduction
-//                 with lane shuffles; stores the rows (or adds to rows the
-//                 tile kernel wrote).
+//   k_residual    (vector-ALU residual engine; the default engine runs the
+//                 residual entries inside k_tile_routes as one-hot MFMA
+//                 fragments) one wave64 per residual work item: the item's
+//                 entries streamed flat over its rows, 16-byte B loads per
+//                 lane (8 per lane in flight), fp32 FMA in registers, a row's
+//                 sums flushed into the wave's 16-row LDS tile with LDS float
+//                 atomics when the row changes; the tile is then stored (or
+//                 added to rows the tile kernel wrote).
 // Every output element is written by exactly one work item except in split
 // windows, whose rows are zeroed and accumulated with fp32 atomics.
 #include <algorithm>
