@@ -650,10 +650,12 @@ __global__ __launch_bounds__(256) void k_zero_rows(const int32_t* __restrict__ s
 }
 
 // workgroups of a 64-thread kernel the current device holds at once (CUs x
-// resident workgroups per CU), remembered per device; -1 after a failed query
-template <typename Kernel>
-int64_t resident_blocks(Kernel kernel) {
-    static std::atomic<int64_t> known[64];
+// resident workgroups per CU), remembered per kernel and device; -1 after a
+// failed query
+template <int F, typename DT>
+int64_t resident_blocks() {
+    static std::atomic<int64_t> known[64];      // one table per tile-kernel instantiation
+    const auto kernel = k_tile_routes<F, DT>;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return -1;
     if (dev >= 0 && dev < 64) {
@@ -685,8 +687,7 @@ int launch_typed(const PlanDev& p, const void* B, int64_t ldb, int64_t K, float*
         if (p.item_order == 2) {
             TRSPMM_CHECK(p.queue != nullptr, "plan has no work queue");
             TRSPMM_CHECK(p.n_titems < 0x7fff0000, "too many work items for the persistent queue");
-            const int64_t resident = p.slab == 128 ? resident_blocks(k_tile_routes<128, DT>)
-                                                   : resident_blocks(k_tile_routes<64, DT>);
+            const int64_t resident = p.slab == 128 ? resident_blocks<128, DT>() : resident_blocks<64, DT>();
             if (resident < 0) return TRSPMM_EHIP;
             // a multiple of 8, so every XCD's queue starts with workgroups of its own
             grid = std::min<int64_t>(xcd_grid(p.n_titems), std::max<int64_t>(kXcds, resident / kXcds * kXcds));
