@@ -14,6 +14,9 @@ Prints ONE JSON line (rank 0):
              (rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/) when recorded, else null
   cpu_baseline  the reference's CPU path on the host: torch.sparse.mm (fp32) with
              the host threads of one GPU's share, plus its router on a row sample
+  gpu_library_baseline  (default run; --no-extras skips it) the GPU sparse
+             library's SpMM (hipSPARSE via torch.sparse.mm, fp32) on the same
+             matrix, a reference point only
   fwd_bwd    (default run; --no-extras skips it) the same R-MAT graph at K=256 in
              fp16 and bf16: plain forward vs forward + backward (dB = A^T·dY and
              dvals = SDDMM(dY, B)), with errors vs FP64
@@ -347,6 +350,33 @@ def measure_batched(steps, warmup, check=True, count=1024, K=128):
     return out
 
 
+def gpu_library_baseline(csr, B16, steps=10):
+    """The GPU's own sparse library on the same product: torch.sparse.mm of
+    the CSR matrix (hipSPARSE SpMM, its default algorithm), in fp32 because
+    hipSPARSE SpMM has no fp16 CSR path on this image.  A reported reference
+    point beside the routed kernel, not the measured value."""
+    import torch
+    dev = torch.device("cuda", 0)
+    A = torch.sparse_csr_tensor(torch.from_numpy(csr.rowptr).to(dev), torch.from_numpy(csr.col).to(dev),
+                                torch.from_numpy(csr.val).to(dev), size=(csr.m_pad, csr.n))
+    B = torch.from_numpy(B16).to(dev).float()
+    for _ in range(2):
+        C = A @ B
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        C = A @ B
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    del A, B, C
+    torch.cuda.empty_cache()
+    return {"what": "torch.sparse.mm (hipSPARSE SpMM, default algorithm), fp32 CSR x fp32 B (no fp16 CSR SpMM "
+                    "there), same matrix, one GPU", "ms": round(ms, 4),
+            "gflops": round(2.0 * csr.nnz * B16.shape[1] / (ms * 1e-3) / 1e9, 1)}
+
+
 def measure_matching(csr, workload, steps, warmup):
     """Wall time of each matching statistics op on the workload's CSR: the
     host ops over their thread pool (best of `steps`, min 3), the device op
@@ -507,6 +537,7 @@ def main():
     if not a.no_extras and a.workload == "rmat18_k128":
         del A, C, Bt
         torch.cuda.empty_cache()
+        line["gpu_library_baseline"] = gpu_library_baseline(csr, B16)
         print("[bench] fwd+bwd at K=256", file=sys.stderr, flush=True)
         line["fwd_bwd"] = [measure_fwd_bwd(csr, 256, dt, a.mode, 20, 5, not a.no_check) for dt in ("fp16", "bf16")]
         del csr
