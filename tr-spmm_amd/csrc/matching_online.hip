@@ -70,10 +70,11 @@ struct OnlineLds {
         uint32_t ids[2 * OM_SLOTS];       // cut-off search: the distinct ids below a cut, sorted
         uint64_t keys[OM_SLOTS];          // collected columns: (column id << 32) | row mask
         struct {
-            uint16_t g1[OM_SLOTS / 2], g2[OM_SLOTS / 2];   // pair: rows both / either columns touch
+            // pair: rows both columns touch | rows either touches << 16; all
+            // ones once the pair is in a 2:4 group
+            uint32_t pk[OM_SLOTS / 2];
             uint16_t c1[OM_SLOTS / 2], c2[OM_SLOTS / 2];   // pair: column positions in table order
             uint16_t pool[OM_SLOTS];
-            uint8_t used[OM_SLOTS / 2];
         } p;
     } b;
 };
@@ -443,8 +444,7 @@ __global__ __launch_bounds__(OM_WAVE) void k_match_online(const int32_t* __restr
             if (best >= 0) {
                 const uint32_t mb = __builtin_amdgcn_readlane(cand, best);
                 if (lane == 0) {
-                    s.b.p.g1[np] = (uint16_t)(mi & mb);
-                    s.b.p.g2[np] = (uint16_t)(mi | mb);
+                    s.b.p.pk[np] = (mi & mb) | ((mi | mb) << 16);
                     s.b.p.c1[np] = (uint16_t)i;
                     s.b.p.c2[np] = (uint16_t)(i + 1 + best);
                 }
@@ -463,44 +463,49 @@ __global__ __launch_bounds__(OM_WAVE) void k_match_online(const int32_t* __restr
     __syncthreads();
 
     // ---- 3. 2:4 groups
-    for (int t = lane; t < np; t += OM_WAVE) s.b.p.used[t] = 0;
-    __syncthreads();
+    // A grouped pair's word is set to all ones: its row union (high half) is
+    // never empty, so it never passes the test again.  One wave: lane 0's
+    // writes are ordered before the next pair's reads by a wave-scope fence
+    // (shared memory serves one wave's accesses in order).  A pair's nonzeros
+    // are popc(both) + popc(either).
     int groups = 0, coarse = 0, fake = 0;
     for (int i = 0; i < np; ++i) {
-        if (s.b.p.used[i]) continue;
-        const uint32_t a1 = s.b.p.g1[i], a2 = s.b.p.g2[i];
+        const uint32_t me = s.b.p.pk[i];
+        if (me == ~0u) continue;
+        const uint32_t a1 = me & 0xFFFFu, a2 = me >> 16;
         int best = -1;
+        uint32_t other = 0;
         for (int step = 0; step < OM_GROUP_SEARCH; step += OM_WAVE) {
             const int j = i + 1 + step + lane;
-            bool ok = j < np;
-            if (ok) ok = !s.b.p.used[j] && ((a1 & s.b.p.g2[j]) | (a2 & s.b.p.g1[j])) == 0;
-            const int l = first_lane(ok);
+            const uint32_t o = j < np ? s.b.p.pk[j] : ~0u;
+            const int l = first_lane(((a1 & (o >> 16)) | (a2 & o)) == 0);
             if (l >= 0) {
                 best = i + 1 + step + l;
+                other = __builtin_amdgcn_readlane(o, l);
                 break;
             }
             if (i + 1 + step + OM_WAVE >= np) break;
         }
         if (best >= 0) {
             ++groups;
-            fake += 32 - (__popc(cmask[s.b.p.c1[i]]) + __popc(cmask[s.b.p.c2[i]]) + __popc(cmask[s.b.p.c1[best]]) +
-                          __popc(cmask[s.b.p.c2[best]]));
+            fake += 32 - (__popc(me) + __popc(other));
         } else {
             coarse += 2;
         }
-        __syncthreads();   // every lane has read used[] before lane 0 writes
         if (lane == 0) {
-            s.b.p.used[i] = 1;
             if (best >= 0) {
-                s.b.p.used[best] = 1;
+                s.b.p.pk[best] = ~0u;
             } else {
                 s.b.p.pool[nd] = s.b.p.c1[i];
                 s.b.p.pool[nd + 1] = s.b.p.c2[i];
             }
         }
         if (best < 0) nd += 2;
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+    __syncthreads();
 
     // ---- 4. the pool
     const int tc = nd / dense_threshold;
