@@ -446,34 +446,71 @@ __global__ __launch_bounds__(64) void k_window_decide(RunArrays R, WinOut O, int
     for (int32_t k = lane; k < nu; k += kWave) used[k] = 0;
     __syncthreads();
 
-    // f. libstdc++ order (one wave for large windows, else one lane) + pairing, one lane
+    // f. libstdc++ order (one wave for large windows, else one lane), then the
+    // pairing of match_sptc_units_max_degree (block_online.cpp:365-391): in that
+    // order, each unit not yet used takes the lowest-index unused unit as its
+    // partner.  The partners are therefore taken in increasing index order, and
+    // every index up to the latest partner `hw` is used (a partner, or a unit
+    // that took one).  So a unit is used at its turn exactly when its index is
+    // <= hw, and an active unit v's partner is the first index above hw, other
+    // than v, that is not an earlier active unit.  used[] marks the active
+    // units; a 64-lane ballot over used[hw + 1 ..] finds the partner, and the
+    // order is read 64 units at a time into registers.
     const bool wave_sorted = nu > kWaveSortMin;
     if (wave_sorted) wave_sort_desc(units, nu, R.pair_a + rb, sort_stack, wsl);
-    if (lane == 0) {
-        if (!wave_sorted) lsort::sort(units, nu, lsort::KeyDesc{}, sort_stack);
-        int32_t low = 0, np = 0;
-        for (int32_t i = 0; i < nu; ++i) {
-            const int32_t v = (int32_t)(units[i] & 0xffffffu);
-            if (used[v]) continue;
-            while (low < nu && used[low]) ++low;
-            int32_t p = low;
-            if (p == v) {
-                p = v + 1;
-                while (p < nu && used[p]) ++p;
-            }
-            if (p >= nu) continue;
-            used[v] = used[p] = 1;
-            R.pair_a[rb + np] = v;
-            R.pair_b[rb + np] = p;
-            ++np;
+    else if (lane == 0) lsort::sort(units, nu, lsort::KeyDesc{}, sort_stack);
+    __syncthreads();
+    // first index > from, other than `except`, not marked in used[] (nu if none)
+    auto next_free = [&](int32_t from, int32_t except) -> int32_t {
+        for (int32_t b = from + 1; b < nu; b += kWave) {
+            const int32_t k = b + lane;
+            const bool ok = (k < nu) & (k != except) && !used[k < nu ? k : 0];
+            const uint64_t m = ballot(ok);
+            if (m) return b + (int32_t)__builtin_ctzll(m);
         }
-        int32_t left = -1;
-        if (2 * np < nu)
-            for (int32_t k = 0; k < nu; ++k)
-                if (!used[k]) {
-                    left = k;
-                    break;
-                }
+        return nu;
+    };
+    // pair np is held by lane np % 64 and stored with its 63 neighbours
+    int32_t hw = -1, np = 0, my_a = 0, my_b = 0;
+    for (int32_t i0 = 0; i0 < nu; i0 += kWave) {
+        const int32_t mine = i0 + lane < nu ? (int32_t)(units[i0 + lane] & 0xffffffu) : 0;
+        const int32_t cnt = nu - i0 < kWave ? nu - i0 : kWave;
+        for (int32_t j = 0; j < cnt; ++j) {
+            const int32_t v = __builtin_amdgcn_readlane(mine, j);
+            if (v <= hw) continue;                       // already a partner
+            const int32_t p = next_free(hw, v);
+            if (p >= nu) continue;                       // no partner: v stays unused
+            if (lane == 0) used[v] = 1;
+            if (lane == (np & (kWave - 1))) {
+                my_a = v;
+                my_b = p;
+            }
+            hw = p;
+            if ((++np & (kWave - 1)) == 0) {
+                R.pair_a[rb + np - kWave + lane] = my_a;
+                R.pair_b[rb + np - kWave + lane] = my_b;
+            }
+            // lane 0's mark is seen by the next search: shared memory serves one
+            // wave in order; the global-memory flags of windows beyond the LDS
+            // capacity need the stores done and the vector L1 refreshed
+            if (in_lds) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            } else {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+        }
+    }
+    if (lane < (np & (kWave - 1))) {
+        R.pair_a[rb + (np & ~(kWave - 1)) + lane] = my_a;
+        R.pair_b[rb + (np & ~(kWave - 1)) + lane] = my_b;
+    }
+    // the lowest unused unit, if any: indices <= hw are used, above it the unmarked ones
+    const int32_t left_k = 2 * np < nu ? next_free(hw, -1) : nu;
+    if (lane == 0) {
+        const int32_t left = left_k < nu ? left_k : -1;
         int32_t left_nnz = 0;
         if (left >= 0) {
             left_nnz = run_count(R, R.rest[rb + 2 * left]);
