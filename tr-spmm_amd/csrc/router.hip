@@ -20,8 +20,10 @@
 //   2. stable radix sort by key (rocPRIM): each (window, column) becomes a run
 //      of its edges in row order; run-length encode
 //   3. one wave per window: stable counting sort of its runs by count desc,
-//      TC chunk selection, unit list, libstdc++ order emulation on one lane
-//      (in LDS when it fits), pairing
+//      TC chunk selection, unit list, libstdc++ order emulation (one lane for
+//      small windows, the whole wave beyond 256 units; in LDS when it fits),
+//      pairing (the wave: a unit is taken iff its index is at most the latest
+//      partner, and a partner search is one ballot)
 //   4. prefix sums over windows -> output sizes; allocate outputs
 //   5. emission kernels: TC sub-blocks/parts, 2:4 groups and per-row slots,
 //      residual flags -> per-row residual parts, mma.sp tile packing.
