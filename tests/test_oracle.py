@@ -192,3 +192,56 @@ def test_oracle_rejects_large_window():
     csr = CASES["ragged"]()
     with pytest.raises(RuntimeError):
         oracle.route(csr.rowptr, csr.col, csr.val, window=20)
+
+
+def _pair_sequential(order, n):
+    """block_online.cpp:365-391 with equal degrees: each unit not yet used, in
+    `order`, takes the lowest-index unused unit (other than itself)."""
+    used = [False] * n
+    pairs = []
+    for v in order:
+        if used[v]:
+            continue
+        p = next((k for k in range(n) if not used[k] and k != v), n)
+        if p == n:
+            continue
+        used[v] = used[p] = True
+        pairs.append((v, p))
+    left = next((k for k in range(n) if not used[k]), -1)
+    return pairs, left
+
+
+def _pair_high_water(order, n):
+    """The device router's form of the same pairing: every index up to the
+    latest partner hw is used, so a unit is taken at its turn iff v <= hw, and
+    an active unit's partner is the first index above hw, other than v, that
+    no earlier active unit holds."""
+    active = [False] * n
+    hw, pairs = -1, []
+    for v in order:
+        if v <= hw:
+            continue
+        p = next((k for k in range(hw + 1, n) if k != v and not active[k]), n)
+        if p == n:
+            continue
+        active[v] = True
+        pairs.append((v, p))
+        hw = p
+    left = next((k for k in range(hw + 1, n) if not active[k]), -1) if 2 * len(pairs) < n else -1
+    return pairs, left
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_router_pairing_high_water_form_equals_sequential(seed):
+    rng = np.random.default_rng(seed)
+    orders = []
+    for n in (0, 1, 2, 3, 5, 17, 64, 65, 200):
+        orders += [list(range(n)), list(range(n))[::-1], list(rng.permutation(n))]
+        # blocks of equal keys permuted among themselves (the tie orders introsort leaves)
+        perm = list(range(n))
+        for b in range(0, n, 7):
+            perm[b:b + 7] = list(rng.permutation(perm[b:b + 7]))
+        orders.append(perm)
+    for order in orders:
+        n = len(order)
+        assert _pair_high_water(order, n) == _pair_sequential(order, n), order
