@@ -101,24 +101,25 @@ struct Gather {
     uint32_t ok;                                     // bit k: piece k holds data (else parked as zeros)
 };
 
-// column indices of a step's staged rows, loaded ahead of its gathers
+// column indices of a step's staged rows, loaded ahead of its gathers: lane l
+// holds the column of row l (R <= 64 rows per step)
 template <int F>
 struct StepCols {
-    int32_t c[Gather<F>::MAXK];
+    int32_t cl;
     int32_t nrows;
 };
 
 // cols[row * STRIDE] is the column of staged row `row` (STRIDE 2: the low word
-// of a residual entry record); nrows >= 1
+// of a residual entry record); nrows >= 1.  One 4-byte load per lane: the
+// gathers' lanes take their rows' columns from it with lane shuffles.  (Each
+// lane loading the column of every row it gathers read 8 lanes' worth of the
+// same word per row, and the kernel is bound by the vector-memory data it
+// gets back.)
 template <int F, int R, int STRIDE>
 __device__ inline void load_cols(StepCols<F>& sc, const int32_t* __restrict__ cols, int nrows, int lane) {
-    constexpr int PIECES = F / 8;
+    static_assert(R <= 64, "a step stages at most 64 rows");
     sc.nrows = nrows;
-#pragma unroll
-    for (int k = 0; k < R * PIECES / 64; ++k) {
-        const int row = (lane + 64 * k) / PIECES;
-        sc.c[k] = cols[(row < nrows ? row : nrows - 1) * STRIDE];     // clamped: unconditional load
-    }
+    sc.cl = cols[(lane < nrows ? lane : nrows - 1) * STRIDE];     // clamped: unconditional load
 }
 
 // B rows addressed with 32-bit element offsets (the launcher checks that B
@@ -132,9 +133,10 @@ __device__ inline void gather_rows(Gather<F>& g, const StepCols<F>& sc, const ui
     for (int k = 0; k < R * PIECES / 64; ++k) {
         const int c = lane + 64 * k;
         const int row = c / PIECES, piece = c % PIECES;
+        const int32_t col = __shfl(sc.cl, row, 64);
         // bitwise & (no short-circuit): a branch per piece would put a wait in each
-        const bool v = (row < sc.nrows) & (sc.c[k] >= 0) & (piece * 8 < fw);
-        const uint32_t off = v ? (uint32_t)sc.c[k] * ldb + (uint32_t)(f0 + piece * 8) : 0u;
+        const bool v = (row < sc.nrows) & (col >= 0) & (piece * 8 < fw);
+        const uint32_t off = v ? (uint32_t)col * ldb + (uint32_t)(f0 + piece * 8) : 0u;
         g.v[k] = *(const u32x4*)(B + off);
         ok |= (uint32_t)v << k;
     }
