@@ -106,6 +106,7 @@ struct Gather {
 template <int F>
 struct StepCols {
     int32_t cl;
+    uint32_t hi;       // residual steps: the high word of row l's entry record (value | local row << 16)
     int32_t nrows;
 };
 
@@ -115,11 +116,23 @@ struct StepCols {
 // lane loading the column of every row it gathers read 8 lanes' worth of the
 // same word per row, and the kernel is bound by the vector-memory data it
 // gets back.)
-template <int F, int R, int STRIDE>
+template <int F, int R>
 __device__ inline void load_cols(StepCols<F>& sc, const int32_t* __restrict__ cols, int nrows, int lane) {
     static_assert(R <= 64, "a step stages at most 64 rows");
     sc.nrows = nrows;
-    sc.cl = cols[(lane < nrows ? lane : nrows - 1) * STRIDE];     // clamped: unconditional load
+    sc.cl = cols[lane < nrows ? lane : nrows - 1];     // clamped: unconditional load
+}
+
+// residual steps: lane l loads entry record l whole (col | value << 32 |
+// local row << 48); its column stages the row, and the one-hot A fragment of
+// the step is built from the records by lane shuffles
+template <int F, int R>
+__device__ inline void load_records(StepCols<F>& sc, const uint64_t* __restrict__ rec, int nrows, int lane) {
+    static_assert(R <= 64, "a step stages at most 64 rows");
+    sc.nrows = nrows;
+    const uint64_t x = rec[lane < nrows ? lane : nrows - 1];
+    sc.cl = (int32_t)(uint32_t)x;
+    sc.hi = (uint32_t)(x >> 32);
 }
 
 // B rows addressed with 32-bit element offsets (the launcher checks that B
@@ -200,11 +213,11 @@ __device__ inline void run_steps(const PlanDev& p, const uint16_t* __restrict__ 
         step = step < n ? step : n - 1;
         const int32_t u = first + UNITS * step;
         if constexpr (KIND == 0)
-            load_cols<F, R, 1>(sc, p.tc_cols + (int64_t)u * 8, (end - u) * 8 < R ? (end - u) * 8 : R, lane);
+            load_cols<F, R>(sc, p.tc_cols + (int64_t)u * 8, (end - u) * 8 < R ? (end - u) * 8 : R, lane);
         else if constexpr (KIND == 1)
-            load_cols<F, R, 1>(sc, p.sp_cols + (int64_t)u * 16, (end - u) * 16 < R ? (end - u) * 16 : R, lane);
+            load_cols<F, R>(sc, p.sp_cols + (int64_t)u * 16, (end - u) * 16 < R ? (end - u) * 16 : R, lane);
         else
-            load_cols<F, R, 2>(sc, (const int32_t*)(p.res_rec + u), end - u < R ? end - u : R, lane);
+            load_records<F, R>(sc, p.res_rec + u, end - u < R ? end - u : R, lane);
     };
     // A fragment of sub-step s (masked at use): TC = row r of sub-block kg of its
     // step; 2:4 = row r of tile kg (4 groups x 2 values) and its index nibbles
@@ -245,17 +258,18 @@ __device__ inline void run_steps(const PlanDev& p, const uint16_t* __restrict__ 
         int idx = 0x4444;
         if constexpr (KIND == 2) {
             // residual entries as a 16x32 one-hot fragment: staged row k = entry k, and
-            // A[r][k] = value of entry k if its local row is r (records: col | val << 32 | lrow << 48)
+            // A[r][k] = value of entry k if its local row is r.  Lane (kg, r) takes
+            // entries 8kg .. 8kg + 7 of the step from the records that cb[PAR] (this
+            // sub-step's, loaded two sub-steps ago) holds one per lane.
+            const StepCols<F>& cur = cb[PAR];
             u16x8 abits = {0, 0, 0, 0, 0, 0, 0, 0};
-            int step = s >> sh;
-            step = step < n ? step : n - 1;
-            const int32_t eb = first + 32 * step + 8 * kg;
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                const int32_t e = eb + q < end ? eb + q : end - 1;
-                const uint64_t rec = p.res_rec[e];
-                const bool hit = s < nsub && eb + q < end && (int)((rec >> 48) & 0xffu) == r && (int32_t)rec >= 0;
-                abits[q] = hit ? (uint16_t)(rec >> 32) : (uint16_t)0;
+                const int src = 8 * kg + q;
+                const uint32_t hw = (uint32_t)__shfl((int)cur.hi, src, 64);
+                const int32_t cq = __shfl(cur.cl, src, 64);
+                const bool hit = (s < nsub) & (src < cur.nrows) & ((int)((hw >> 16) & 0xffu) == r) & (cq >= 0);
+                abits[q] = hit ? (uint16_t)(hw & 0xffffu) : (uint16_t)0;
             }
             a = __builtin_bit_cast(X8, abits);
         }
