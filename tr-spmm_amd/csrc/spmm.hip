@@ -178,25 +178,29 @@ struct Frag {
 };
 
 // Step loop of one route kind over the steps [0, n) of a work item, for one
-// pass of `ns` slabs (F features each, from feature fp).  KIND 0 = TC
+// pass of NS slabs (F features each, from feature fp; NS is a template
+// argument so that no load sits behind a run-time condition).  KIND 0 = TC
 // sub-blocks (32 rows, dense A rows, MFMA), 1 = 2:4 tiles (64 rows, SMFMAC),
 // 2 = residual entries as one-hot fragments (32 rows, MFMA).
 //
-// Sub-step s is (step s / ns, slab s % ns).  The loop is unrolled by two so
+// Sub-step s is (step s / NS, slab s % NS).  The loop is unrolled by two so
 // that a sub-step's parity is known at compile time:
 //   * even sub-steps accumulate into acc[0] and odd ones into acc[1] (one slab
-//     per parity when ns == 2; two partial sums of the same slab when ns == 1,
+//     per parity when NS == 2; two partial sums of the same slab when NS == 1,
 //     added by the caller);
 //   * column indices and A fragments live in a two-entry ring: sub-step s
 //     parks its pieces (gathered during s - 1), applies the fragment loaded at
 //     s - 2, issues the gathers of s + 1 from the columns loaded at s - 1 and
-//     loads the columns and fragment of s + 2.
+//     loads the columns and fragment of s + 2.  With NS == 2 the two sub-steps
+//     of a step share its columns and fragment: only even sub-steps load them
+//     (into entry 0) and the odd one reuses the fragment it applied, which
+//     saves the data those loads would return (the kernel is bound by it).
 // Every load is unconditional (indices clamped to the last step, fragments of
 // sub-steps past the end masked to zero), so the count of loads in flight is
 // the same on every path and each wait covers exactly the data it needs.  An
 // odd sub-step count gets one masked extra sub-step.
-template <int KIND, int F, typename DT, int LDS_STRIDE>
-__device__ inline void run_steps(const PlanDev& p, const uint16_t* __restrict__ B, uint32_t ldb, int K, int fp, int ns,
+template <int KIND, int F, typename DT, int LDS_STRIDE, int NS>
+__device__ inline void run_steps(const PlanDev& p, const uint16_t* __restrict__ B, uint32_t ldb, int K, int fp,
                                  int32_t first, int32_t end, int n, uint16_t* __restrict__ Bs,
                                  const uint16_t* __restrict__ bp, int lane, f32x4 (&acc)[2][F / 16]) {
     using X8 = typename DT::x8;
@@ -205,7 +209,8 @@ __device__ inline void run_steps(const PlanDev& p, const uint16_t* __restrict__ 
     constexpr int R = KIND == 1 ? 64 : 32;        // staged rows per step
     constexpr int UNITS = KIND == 2 ? 32 : 4;     // entries / sub-blocks / tiles per step
     const int r = lane & 15, kg = lane >> 4;
-    const int sh = ns - 1;                        // step = s >> sh, slab = s & sh
+    constexpr int sh = NS - 1;                    // step = s >> sh, slab = s & sh
+    constexpr bool TWO = NS == 2;                 // sub-steps 2t, 2t + 1 share step t's columns and fragment
     const int nsub = n << sh;
 
     auto cols_of = [&](int s, StepCols<F>& sc) {
@@ -249,14 +254,19 @@ __device__ inline void run_steps(const PlanDev& p, const uint16_t* __restrict__ 
     cols_of(0, cb[0]);
     frag_of(0, fb[0]);
     gather(g, cb[0], 0);
-    cols_of(1, cb[1]);
-    frag_of(1, fb[1]);
+    if constexpr (!TWO) {
+        cols_of(1, cb[1]);
+        frag_of(1, fb[1]);
+    }
+    X8 a_keep{};
+    int idx_keep = 0x4444;
 
     auto body = [&](int s, auto par_tag) {
         constexpr int PAR = decltype(par_tag)::value;
-        X8 a;
-        int idx = 0x4444;
-        if constexpr (KIND == 2) {
+        constexpr bool FRESH = !TWO || PAR == 0;      // this sub-step starts a step
+        X8 a = a_keep;
+        int idx = idx_keep;
+        if constexpr (KIND == 2 && FRESH) {
             // residual entries as a 16x32 one-hot fragment: staged row k = entry k, and
             // A[r][k] = value of entry k if its local row is r.  Lane (kg, r) takes
             // entries 8kg .. 8kg + 7 of the step from the records that cb[PAR] (this
@@ -272,16 +282,21 @@ __device__ inline void run_steps(const PlanDev& p, const uint16_t* __restrict__ 
                 abits[q] = hit ? (uint16_t)(hw & 0xffffu) : (uint16_t)0;
             }
             a = __builtin_bit_cast(X8, abits);
+            a_keep = a;
         }
         __syncthreads();                               // previous sub-step's LDS reads are done
         park_rows<F, R, LDS_STRIDE>(g, Bs, lane);
-        if constexpr (KIND != 2) {
+        if constexpr (KIND != 2 && FRESH) {
             a = __builtin_bit_cast(X8, fb[PAR].ok ? fb[PAR].a : u32x4{0u, 0u, 0u, 0u});
             idx = fb[PAR].ok ? fb[PAR].idx : 0x4444;   // (0,1) in every group: valid, times zeros
+            a_keep = a;
+            idx_keep = idx;
         }
-        gather(g, cb[1 - PAR], s + 1);                 // overlaps this sub-step's math
-        cols_of(s + 2, cb[PAR]);
-        frag_of(s + 2, fb[PAR]);
+        gather(g, TWO ? cb[0] : cb[1 - PAR], s + 1);   // overlaps this sub-step's math
+        if constexpr (FRESH) {
+            cols_of(s + 2, cb[PAR]);
+            frag_of(s + 2, fb[PAR]);
+        }
         __syncthreads();
         if constexpr (KIND == 1) {
 #pragma unroll
@@ -360,9 +375,15 @@ __device__ __forceinline__ void tile_item(const PlanDev& p, int64_t it, const ui
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) acc[sl][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
         const uint32_t ldb32 = (uint32_t)ldb;
-        if (nts > 0) run_steps<0, F, DT, LDS_STRIDE>(p, B, ldb32, K, fp, ns, s0, s1, nts, Bs, bp, lane, acc);
-        if (ntt > 0) run_steps<1, F, DT, LDS_STRIDE>(p, B, ldb32, K, fp, ns, t0, t1, ntt, Bs, bp, lane, acc);
-        if (nte > 0) run_steps<2, F, DT, LDS_STRIDE>(p, B, ldb32, K, fp, ns, e0, e1, nte, Bs, bp, lane, acc);
+        if (ns == 2) {
+            if (nts > 0) run_steps<0, F, DT, LDS_STRIDE, 2>(p, B, ldb32, K, fp, s0, s1, nts, Bs, bp, lane, acc);
+            if (ntt > 0) run_steps<1, F, DT, LDS_STRIDE, 2>(p, B, ldb32, K, fp, t0, t1, ntt, Bs, bp, lane, acc);
+            if (nte > 0) run_steps<2, F, DT, LDS_STRIDE, 2>(p, B, ldb32, K, fp, e0, e1, nte, Bs, bp, lane, acc);
+        } else {
+            if (nts > 0) run_steps<0, F, DT, LDS_STRIDE, 1>(p, B, ldb32, K, fp, s0, s1, nts, Bs, bp, lane, acc);
+            if (ntt > 0) run_steps<1, F, DT, LDS_STRIDE, 1>(p, B, ldb32, K, fp, t0, t1, ntt, Bs, bp, lane, acc);
+            if (nte > 0) run_steps<2, F, DT, LDS_STRIDE, 1>(p, B, ldb32, K, fp, e0, e1, nte, Bs, bp, lane, acc);
+        }
         if (ns == 1) {
             // one slab: even and odd sub-steps summed into acc[0]
 #pragma unroll
@@ -450,9 +471,13 @@ __device__ __forceinline__ void tile_item(const PlanDev& p, int64_t it, const ui
 //      A likely cause: a wave that ends frees its slot while its stores
 //      drain, whereas a persistent wave's next loads share one in-order
 //      counter with those stores.
+// F = 64 is held to two waves per SIMD (256 VGPRs, no spills); left to
+// itself the compiler took a few more registers than that and halved the
+// waves.  F = 128 runs one wave per SIMD whatever the bound.
 template <int F, typename DT>
-__global__ __launch_bounds__(64) void k_tile_routes(PlanDev p, const uint16_t* __restrict__ B, int64_t ldb, int K,
-                                                    float* __restrict__ C, int64_t ldc, uint32_t mask, int64_t grid) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(F == 64 ? 2 : 1, F == 64 ? 2 : 1)))
+void k_tile_routes(PlanDev p, const uint16_t* __restrict__ B, int64_t ldb, int K, float* __restrict__ C,
+                   int64_t ldc, uint32_t mask, int64_t grid) {
     __shared__ __align__(16) uint16_t Bs[64 * (F + 4)];
     const int lane = threadIdx.x;
     if (p.item_order != 2) {
