@@ -280,38 +280,40 @@ __device__ inline void bwd_row_range(int64_t c, int32_t e0, int32_t e1, bool ato
     // the count of loads in flight is the same on every path.  An odd batch
     // count gets one masked extra batch.
     const int nb = (e1 - e0 + U - 1) / U;
-    int32_t ridx[2][U];
-    uint16_t vraw[2][U];
+    // A batch's U column indices and values are loaded once per lane group:
+    // lane j loads entry j % U, and the group takes them with lane shuffles
+    // (every lane loading every entry returned each index LPR times, and the
+    // vector-memory data returned is what such gathers are bound by).
+    const int gbase = (int)(threadIdx.x & 63) - j;
+    int32_t ridx[2];
+    uint32_t vraw[2];
     u32x4 y[2][U][NCH];
-    auto load_idx = [&](int b, int32_t (&r)[U]) {
+    auto load_idx = [&](int b, int32_t& r) {
+        const int32_t e = e0 + b * U + (j & (U - 1));
+        r = col[e < e1 ? e : e1 - 1];
+    };
+    auto load_val = [&](int b, uint32_t& v) {
+        const int32_t e = e0 + b * U + (j & (U - 1));
+        v = val[e < e1 ? e : e1 - 1];
+    };
+    auto gather = [&](int32_t r_lane, u32x4 (&yy)[U][NCH]) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int32_t e = e0 + b * U + u;
-            r[u] = col[e < e1 ? e : e1 - 1];
-        }
-    };
-    auto load_val = [&](int b, uint16_t (&v)[U]) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int32_t e = e0 + b * U + u;
-            v[u] = val[e < e1 ? e : e1 - 1];
-        }
-    };
-    auto gather = [&](const int32_t (&r)[U], u32x4 (&yy)[U][NCH]) {
-#pragma unroll
-        for (int u = 0; u < U; ++u)
+            const int32_t r = __shfl(r_lane, gbase + u, 64);
 #pragma unroll
             for (int k = 0; k < NCH; ++k) {
                 const int f = 8 * j + 8 * LPR * k;
-                yy[u][k] = *(const u32x4*)(dY + (int64_t)r[u] * lddy + (f < K ? f : 0));
+                yy[u][k] = *(const u32x4*)(dY + (int64_t)r * lddy + (f < K ? f : 0));
             }
+        }
     };
-    auto compute = [&](int b, const u32x4 (&yy)[U][NCH], const uint16_t (&v)[U]) {
+    auto compute = [&](int b, const u32x4 (&yy)[U][NCH], uint32_t v_lane) {
         const int32_t eb = e0 + b * U;
         float part[SDDMM_U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const float vu = eb + u < e1 ? lo16<DT>(v[u]) : 0.f;
+            const uint32_t vb = (uint32_t)__shfl((int)v_lane, gbase + u, 64);
+            const float vu = eb + u < e1 ? lo16<DT>(vb) : 0.f;
             part[u] = 0.f;
 #pragma unroll
             for (int k = 0; k < NCH; ++k) {
