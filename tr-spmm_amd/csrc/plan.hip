@@ -315,14 +315,15 @@ int Plan::finalize() {
 // Tile work items heaviest first.  An item's cost is its gathered B rows
 // (32 per step of 4 TC sub-blocks, 64 per step of 4 2:4 tiles, 32 per step of
 // 32 residual entries) plus its store; the items are sorted by the class
-// floor(log2(cost)), stably, so items of one class keep window order (and the
-// column locality of neighbouring windows).  The dispatcher deals consecutive
-// workgroups to the 8 XCDs round robin, so the heavy items start first and
-// spread over all XCDs; in window order, R-MAT's hub windows (mostly
-// multiples of 8) shared one XCD.
+// floor(log2(cost)), stably.  The dispatcher deals consecutive workgroups to
+// the 8 XCDs round robin, so the heavy items start first and spread over all
+// XCDs; in window order, R-MAT's hub windows (mostly multiples of 8) shared
+// one XCD.  Within a class the items (in window order) are cut into 8
+// contiguous chunks that are interleaved, so each XCD walks a contiguous run
+// of windows, whose columns overlap, through its own L2.
 static int sort_items_heaviest_first(Plan& P, int64_t n, hipStream_t s) {
     const int64_t W = P.dev.windows;
-    std::vector<int4> it((size_t)n), out((size_t)n);
+    std::vector<int4> it((size_t)n), out((size_t)n);     // `it` in window order
     std::vector<int32_t> so((size_t)W + 1), to((size_t)W + 1), ro((size_t)(16 * W + 1));
     TRSPMM_HIP(hipMemcpyAsync(it.data(), P.titems.as<int4>(), 16 * n, hipMemcpyDeviceToHost, s));
     TRSPMM_HIP(hipMemcpyAsync(so.data(), P.tc_sub_off.as<int32_t>(), 4 * (W + 1), hipMemcpyDeviceToHost, s));
@@ -354,7 +355,16 @@ static int sort_items_heaviest_first(Plan& P, int64_t n, hipStream_t s) {
         start[c] = run;
         run += count[c];
     }
-    for (int64_t k = 0; k < n; ++k) out[start[cls[k]]++] = it[k];
+    std::vector<int4> sorted((size_t)n);
+    for (int64_t k = 0; k < n; ++k) sorted[start[cls[k]]++] = it[k];     // start[c] ends at class c's end
+    for (int64_t c = 0, a = 0; c < 64; ++c) {
+        const int64_t m = count[c], ch = (m + 7) / 8;
+        int64_t o = a;
+        for (int64_t t = 0; t < ch; ++t)
+            for (int64_t x = 0; x < 8; ++x)
+                if (x * ch + t < m) out[o++] = sorted[a + x * ch + t];
+        a += m;
+    }
     TRSPMM_HIP(hipMemcpyAsync(P.titems.as<int4>(), out.data(), 16 * n, hipMemcpyHostToDevice, s));
     TRSPMM_HIP(hipStreamSynchronize(s));
     return TRSPMM_OK;
