@@ -218,7 +218,8 @@ __device__ int32_t wave_partition(uint32_t* a, int32_t first, int32_t last, int3
         const int32_t p = b + lane;
         TR += __popcll(ballot(p < last && (a[p] >> 24) >= kp));
     }
-    // S = max_x min(#l <= x, #r > x)
+    // S = max_x min(#l <= x, #r > x): a running maximum per lane, one wave
+    // reduction at the end
     int32_t PL = 0, PR = 0, S = 0;
     for (int32_t b = first; b < last; b += kWave) {
         const int32_t p = b + lane;
@@ -226,10 +227,11 @@ __device__ int32_t wave_partition(uint32_t* a, int32_t first, int32_t last, int3
         const uint32_t k = on ? a[p] >> 24 : 0u;
         const uint64_t mL = ballot(on && p > first && k <= kp), mR = ballot(on && k >= kp);
         const int32_t pl = PL + __popcll(mL & le), pr = PR + __popcll(mR & le);
-        S = max(S, wave_max_i32(on ? min(pl, TR - pr) : 0));
+        S = max(S, on ? min(pl, TR - pr) : 0);
         PL += __popcll(mL);
         PR += __popcll(mR);
     }
+    S = wave_max_i32(S);
     // r_k (k <= S) -> tmp[k - 1]; l_{S+1} -> cut_l (nothing is swapped yet)
     PL = 0;
     PR = 0;
