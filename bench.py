@@ -1,10 +1,14 @@
 """Routed SpMM benchmark (one MI355X).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rmat18_k128] [--mode exact]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rmat18_k128] [--mode tuned]
 
 A step is one routed multiply C = A·B over the whole matrix (one fused
 kernel launch), with A's routed plan and B already resident in HBM.
 Preprocessing (device router + plan) runs once before the timed region.
+The default routing is the tuned router (2:4 groups with fewer than 8 of 32
+slots filled go to the residual route, which runs as one-hot MFMA fragments
+in the same kernel); config.exact_routing times the same product on the
+reference's tiling in the same run.
 Prints ONE JSON line (rank 0):
   value      GFLOP/s = 2·nnz·K / (ms_per_step)
   roofline   HBM-bound: algorithmic bytes per launch
@@ -19,7 +23,9 @@ Prints ONE JSON line (rank 0):
              matrix, a reference point only
   fwd_bwd    (default run; --no-extras skips it) the same R-MAT graph at K=256 in
              fp16 and bf16: plain forward vs forward + backward (dB = A^T·dY and
-             dvals = SDDMM(dY, B)), with errors vs FP64
+             dvals = SDDMM(dY, B)), with errors vs FP64, on the reference's
+             tiling; tuned_routing repeats forward and forward + backward with
+             the tuned router (the backward is the same either way)
   batched    (default run; --no-extras skips it) 1024 Barabasi-Albert graphs
              (m=3, n_i log-uniform in [20, 5000]), K=128: one autotuned launch
              over the concatenated tile queue (eager and replayed from a
@@ -47,11 +53,15 @@ def parse():
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=10)
     p.add_argument("--workload", default="rmat18_k128")
-    p.add_argument("--mode", default="exact", choices=["exact", "compat", "tuned"])
+    p.add_argument("--mode", default="tuned", choices=["exact", "compat", "tuned"],
+                   help="routing of the measured multiply: tuned (the default: 2:4 groups with fewer than 8 of "
+                        "32 slots filled go to the residual route), exact (the reference's tiles, every nonzero "
+                        "kept) or compat (the reference's metadata bit for bit)")
     p.add_argument("--dtype", default="fp16", choices=["fp16", "bf16"])
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-check", action="store_true")
-    p.add_argument("--no-extras", action="store_true", help="skip the fwd+bwd and batched measurements")
+    p.add_argument("--no-extras", action="store_true",
+                   help="skip the exact-routing comparison, the library baseline, fwd+bwd and the batched measurements")
     p.add_argument("--batched", action="store_true", help="measure only the batched workload")
     p.add_argument("--per-route", action="store_true",
                    help="per-route rows, nnz and times of --workload (exact and tuned routing)")
@@ -151,7 +161,9 @@ def time_steps(fn, steps, warmup):
 
 def measure_fwd_bwd(csr, K, dtype_name, mode, steps, warmup, check=True):
     """Forward vs forward + backward (dB = A^T dY on A^T's routed plan, dvals =
-    SDDMM(dY, B)) on one graph; preprocessing (both plans) outside the timing."""
+    SDDMM(dY, B)) on one graph, routed in `mode`; preprocessing (both plans)
+    outside the timing.  The fused backward works on A's CSR, so the same
+    backward follows the tuned router's forward in `tuned_routing`."""
     import torch
 
     from trspmm import graphs
@@ -190,6 +202,15 @@ def measure_fwd_bwd(csr, K, dtype_name, mode, steps, warmup, check=True):
     ms_fused = time_steps(lambda: A.spmm_t_sddmm(dY, B, dB, dv), steps, warmup)
     ms_db = time_steps(lambda: At.matmul(dY, dB), steps, warmup)
     ms_sd = time_steps(lambda: A.sddmm(dY, B, dv), steps, warmup)
+    # the same product routed by the tuned router: its forward, and the step
+    # with the same fused backward (which works on A's CSR, whatever the routing)
+    A_t = RoutedSparse(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val), csr.n,
+                       m=csr.m, mode="tuned", dtype=dtype)
+    A_t.spmm_t_sddmm(dY, B, dB, dv)
+    torch.cuda.synchronize()
+    ms_f_t = time_steps(lambda: A_t.matmul(B, C), steps, warmup)
+    ms_fb_t = time_steps(lambda: (A_t.matmul(B, C), A_t.spmm_t_sddmm(dY, B, dB, dv)), steps, warmup)
+    del A_t
     if check:
         A.spmm_t_sddmm(dY, B, dB, dv)                # the fused results are the ones checked below
     flops = 3 * 2.0 * csr.nnz * K
@@ -201,7 +222,9 @@ def measure_fwd_bwd(csr, K, dtype_name, mode, steps, warmup, check=True):
            "roofline_frac_fwd_bwd": round(
                (algorithmic_bytes(csr.nnz, csr.m, csr.n, K) + algorithmic_bytes(csr.nnz, csr.n, csr.m, K)
                 + csr.nnz * 8 + 4 * (csr.m + 1) + (csr.m + csr.n) * K * 2) / (ms_fb * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-           "prep_s": round(prep, 3)}
+           "prep_s": round(prep, 3),
+           "tuned_routing": {"ms_fwd": round(ms_f_t, 5), "ms_fwd_bwd": round(ms_fb_t, 5),
+                             "ratio_fwd_bwd_over_fwd": round(ms_fb_t / ms_f_t, 3)}}
     if check:
         import scipy.sparse as sp
         Af = sp.csr_matrix((csr.val.astype(np.float64), csr.col, csr.rowptr), shape=(csr.m_pad, csr.n))
@@ -456,7 +479,7 @@ def main():
         b2 = b1 + (int(n * 0.3) // 16) * 16
         regions = {"communities": [0, b1], "banded": [b1, b2], "tail": [b2, n]} if a.workload.startswith(
             "three_route") else {}
-        for mode in (["exact", "tuned"] if a.mode == "exact" else [a.mode]):
+        for mode in (["exact", "tuned"] if a.mode != "compat" else ["compat"]):
             rec = measure_routes(csr, K, mode, a.steps, a.warmup)
             rec.update(workload=a.workload, rows=csr.m, nnz_total=csr.nnz, K=K, regions_rows=regions)
             print(json.dumps(rec), flush=True)
@@ -466,7 +489,8 @@ def main():
         return
     if a.backward:
         for dt in ("fp16", "bf16"):
-            print(json.dumps(measure_fwd_bwd(csr, K, dt, a.mode, a.steps, a.warmup, not a.no_check)), flush=True)
+            print(json.dumps(measure_fwd_bwd(csr, K, dt, "exact" if a.mode == "tuned" else a.mode, a.steps, a.warmup,
+                                             not a.no_check)), flush=True)
         return
     dtype = torch.float16 if a.dtype == "fp16" else torch.bfloat16
     B16 = graphs.dense_features(csr.n, K, seed=bseed)                    # fp16 N(0,1)
@@ -519,6 +543,16 @@ def main():
         ref = np.asarray(Af @ B16.astype(np.float64))[: csr.m]
         got = C.cpu().numpy().astype(np.float64)
         err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    exact_routing = None
+    if a.mode != "exact" and not a.no_extras:
+        # the same product on the reference's tiling (exact routing), timed the same way
+        A_x = RoutedSparse(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val),
+                           csr.n, m=csr.m, mode="exact", dtype=dtype)
+        ms_x = time_steps(lambda: A_x.matmul(Bt, C), a.steps, a.warmup)
+        exact_routing = {"ms": round(ms_x, 5), "gflops": round(flops / (ms_x * 1e-3) / 1e9, 2),
+                         "roofline_frac": round(byts / (ms_x * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "routes": A_x.plan.stats}
+        del A_x
     traffic = traffic_record(a.workload, a.mode, a.dtype)
     line = {
         "metric": "SpMM GFLOP/s (2·nnz·K) + %HBM roofline",
@@ -527,6 +561,7 @@ def main():
         "dtype": a.dtype, "data": "synthetic (seeded generator, trspmm.graphs)",
         "config": {"workload": a.workload, "rows": csr.m, "nnz": csr.nnz, "K": K, "mode": a.mode,
                    "routes": A.plan.stats, "rel_fro_err_vs_fp64": err, "router_s": round(A.meta.seconds, 4),
+                   "exact_routing": exact_routing,
                    "wall_s_timed": round(wall, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -539,7 +574,8 @@ def main():
         torch.cuda.empty_cache()
         line["gpu_library_baseline"] = gpu_library_baseline(csr, B16)
         print("[bench] fwd+bwd at K=256", file=sys.stderr, flush=True)
-        line["fwd_bwd"] = [measure_fwd_bwd(csr, 256, dt, a.mode, 20, 5, not a.no_check) for dt in ("fp16", "bf16")]
+        line["fwd_bwd"] = [measure_fwd_bwd(csr, 256, dt, "exact" if a.mode == "tuned" else a.mode, 20, 5,
+                                           not a.no_check) for dt in ("fp16", "bf16")]
         del csr
         torch.cuda.empty_cache()
         print("[bench] batched graphs", file=sys.stderr, flush=True)
