@@ -401,10 +401,29 @@ __global__ void k_invert(const int32_t* __restrict__ perm, int64_t n, int32_t* _
     if (i < n) inv[perm[i]] = (int32_t)i;
 }
 
-__global__ void k_gather_f32(const float* __restrict__ src, const int32_t* __restrict__ idx, int64_t n,
-                             float* __restrict__ dst) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[idx[i]];
+// dst[i] = src[idx[i]].  A block of 256 lanes covers a span of
+// kGatherSpan entries, each lane GATHER_PER of them 64 apart (every load and
+// store instruction coalesced, GATHER_PER random reads in flight per lane).
+// The grid is 8 * per blocks and block b takes span (b % 8) * per + b / 8, so
+// XCD x (blocks go to the 8 XCDs round robin) walks one contiguous eighth of
+// dst: entries of nearby rows of A that share a column sit next to each other
+// in src, and their lines then meet in one L2.
+constexpr int GATHER_PER = 4;
+constexpr int64_t kGatherSpan = 256 * GATHER_PER;
+
+__global__ __launch_bounds__(256) void k_gather_f32(const float* __restrict__ src, const int32_t* __restrict__ idx,
+                                                    int64_t n, int64_t per, float* __restrict__ dst) {
+    const int64_t span = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    const int64_t i0 = span * kGatherSpan + (threadIdx.x / 64) * (64 * GATHER_PER) + (threadIdx.x & 63);
+    float v[GATHER_PER];
+#pragma unroll
+    for (int k = 0; k < GATHER_PER; ++k) {
+        const int64_t i = i0 + 64 * k;
+        v[k] = i < n ? src[idx[i]] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < GATHER_PER; ++k)
+        if (i0 + 64 * k < n) dst[i0 + 64 * k] = v[k];
 }
 
 __global__ void k_to16(const float* __restrict__ src, int64_t n, int dtype, uint16_t* __restrict__ dst) {
@@ -440,8 +459,9 @@ int launch_bwd_typed(const BwdPlan& P, const void* dY, int64_t lddy, const void*
     }
 #undef TRSPMM_BWD
     if (P.nnz > 0) {
-        hipLaunchKernelGGL(k_gather_f32, dim3(nblk(P.nnz)), dim3(256), 0, s, dvt, P.invperm.as<int32_t>(), P.nnz,
-                           dvals);
+        const int64_t per = ((P.nnz + kGatherSpan - 1) / kGatherSpan + 7) / 8;
+        hipLaunchKernelGGL(k_gather_f32, dim3((unsigned)(8 * per)), dim3(256), 0, s, dvt, P.invperm.as<int32_t>(),
+                           P.nnz, per, dvals);
         TRSPMM_HIP(hipGetLastError());
     }
     return TRSPMM_OK;
