@@ -177,7 +177,7 @@ def measure_fwd_bwd(csr, K, dtype_name, mode, steps, warmup, check=True):
     dY = torch.from_numpy(graphs.dense_features(csr.m, K, seed=1003).astype(np.float32)).to(dev).to(dtype)
     t0 = time.time()
     A = RoutedSparse(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val), csr.n,
-                     m=csr.m, mode=mode, dtype=dtype)
+                     m=csr.m, mode=mode, dtype=dtype, plan_opts={"k_hint": K})
     At = A.transpose()
     torch.cuda.synchronize()
     prep = time.time() - t0
@@ -208,7 +208,7 @@ def measure_fwd_bwd(csr, K, dtype_name, mode, steps, warmup, check=True):
     # the same product routed by the tuned router: its forward, and the step
     # with the same fused backward (which works on A's CSR, whatever the routing)
     A_t = RoutedSparse(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val), csr.n,
-                       m=csr.m, mode="tuned", dtype=dtype)
+                       m=csr.m, mode="tuned", dtype=dtype, plan_opts={"k_hint": K})
     A_t.spmm_t_sddmm(dY, B, dB, dv)
     torch.cuda.synchronize()
     ms_f_t = time_steps(lambda: A_t.matmul(B, C), steps, warmup)
@@ -295,7 +295,7 @@ def measure_routes(csr, K, mode, steps, warmup):
     dev = torch.device("cuda", 0)
     B = torch.from_numpy(graphs.dense_features(csr.n, K, seed=1002)).to(dev)
     A = RoutedSparse(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val), csr.n,
-                     m=csr.m, mode=mode)
+                     m=csr.m, mode=mode, plan_opts={"k_hint": K})
     C = torch.empty((csr.m, K), device=dev, dtype=torch.float32)
     st = A.plan.stats
     ms = {name: time_steps(lambda mk=mk: A.matmul(B, C, routes=mk), steps, warmup)
@@ -504,8 +504,9 @@ def main():
     print(f"[bench] workload={a.workload} m={csr.m} nnz={csr.nnz} K={K} gen={gen_s:.1f}s", file=sys.stderr,
           flush=True)
     t0 = time.time()
+    # the plan's work items are sized for the K it runs at (plan option k_hint)
     A = RoutedSparse(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val), csr.n,
-                     m=csr.m, mode=a.mode, dtype=dtype)
+                     m=csr.m, mode=a.mode, dtype=dtype, plan_opts={"k_hint": K})
     torch.cuda.synchronize()
     prep_s = time.time() - t0
     print(f"[bench] routed on device in {A.meta.seconds:.3f}s (plan total {prep_s:.2f}s) stats={A.plan.stats}",
@@ -550,7 +551,7 @@ def main():
     if a.mode != "exact" and not a.no_extras:
         # the same product on the reference's tiling (exact routing), timed the same way
         A_x = RoutedSparse(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val),
-                           csr.n, m=csr.m, mode="exact", dtype=dtype)
+                           csr.n, m=csr.m, mode="exact", dtype=dtype, plan_opts={"k_hint": K})
         ms_x = time_steps(lambda: A_x.matmul(Bt, C), a.steps, a.warmup)
         exact_routing = {"ms": round(ms_x, 5), "gflops": round(flops / (ms_x * 1e-3) / 1e9, 2),
                          "roofline_frac": round(byts / (ms_x * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -563,7 +564,7 @@ def main():
         "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": a.dtype, "data": "synthetic (seeded generator, trspmm.graphs)",
         "config": {"workload": a.workload, "rows": csr.m, "nnz": csr.nnz, "K": K, "mode": a.mode,
-                   "routes": A.plan.stats, "rel_fro_err_vs_fp64": err, "router_s": round(A.meta.seconds, 4),
+                   "plan_opts": A.plan_opts, "routes": A.plan.stats, "rel_fro_err_vs_fp64": err, "router_s": round(A.meta.seconds, 4),
                    "exact_routing": exact_routing,
                    "wall_s_timed": round(wall, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -140,7 +140,7 @@ typedef struct trspmm_plan_opts {
     int32_t slab;             /* tile-kernel features per pass: 0 (default), 64 or 128 */
     int32_t tile_cap;         /* a window with more 2:4 tiles is split into chunks of this many (0 = 64) */
     int32_t tc_cap;           /* the same for TC sub-blocks (0 = 64) */
-    int32_t res_cap;          /* the same for residual entries (0 = 2048) */
+    int32_t res_cap;          /* the same for residual entries (0 = 2048, or sized by k_hint) */
     int32_t item_order;       /* 0: one workgroup per work item, the items sorted heaviest first
                                     (by gathered rows, in classes of powers of two) and dealt
                                     round robin to the XCDs (the dispatch order);
@@ -151,7 +151,12 @@ typedef struct trspmm_plan_opts {
                                     (launches of one such plan must not overlap on concurrent
                                     streams);
                                  3: as 0, with the items in window order */
-    int32_t reserved[2];
+    int32_t k_hint;           /* feature width K the plan will mostly run at (0 = not known).
+                                 When > 0 and res_cap is 0, the residual cap is sized for it,
+                                 so that a residual work item holds about the same work at
+                                 any K: 4096 * 128 / K' entries (at least 256),
+                                 K' = max(K, 64).  Any K still runs on the plan. */
+    int32_t reserved;
 } trspmm_plan_opts;
 
 /* Builds the device plan (AMD lane layouts) from reference-format metadata.

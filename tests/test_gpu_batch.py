@@ -129,6 +129,22 @@ def test_work_item_options(cuda, item_order, cap, engine):
         assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-5, rep
 
 
+@pytest.mark.parametrize("k_hint", [64, 128, 256, 1024])
+@pytest.mark.parametrize("K", [64, 256])
+def test_k_hint_caps(cuda, k_hint, K):
+    """Residual caps sized by k_hint (down to 512 entries per item at k_hint
+    1024, so hub windows split into many atomic chunks) at a K other than the
+    hint: the product is unchanged."""
+    c = CASES["hub"]()
+    B = torch.randn(c.n, K, generator=torch.Generator().manual_seed(6)).half().to(cuda)
+    A = RoutedSparse(torch.from_numpy(c.rowptr), torch.from_numpy(c.col), torch.from_numpy(c.val), c.n, m=c.m,
+                     mode="tuned", plan_opts={"k_hint": k_hint})
+    vq = torch.from_numpy(c.val).half().double().numpy()
+    ref = oracle.csr_spmm64(c.rowptr, c.col, vq, B.cpu().double().numpy(), c.m)
+    got = A.matmul(B).cpu().numpy()
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-5
+
+
 def test_persistent_queue_many_items_per_workgroup(cuda):
     """More work items than the chip holds workgroups (5000 windows, 80000
     rows), so each persistent workgroup takes many items from the queue; five

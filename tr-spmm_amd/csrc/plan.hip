@@ -473,6 +473,16 @@ int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtyp
     TRSPMM_CHECK(P.opts.tile_cap >= 0 && P.opts.tc_cap >= 0 && P.opts.res_cap >= 0,
                  "plan options: caps must be >= 0");
     TRSPMM_CHECK(P.opts.item_order >= 0 && P.opts.item_order <= 3, "plan options: item_order must be 0, 1, 2 or 3");
+    TRSPMM_CHECK(P.opts.k_hint >= 0 && P.opts.k_hint <= (1 << 24), "plan options: k_hint must be in [0, 2^24]");
+    if (P.opts.k_hint > 0) {
+        // Residual work items sized for K: a residual entry's gather grows with K,
+        // so the entry cap shrinks as 128 / K (R-MAT scale 18: 4096 at K = 128 was
+        // 4 % faster than 2048, and 2048 at K = 256 faster than 1024 or 4096);
+        // below K = 128 it grows at most twofold.  The 2:4 cap stays: 32 tiles at
+        // K = 256 made the exact tiling 20 % slower.
+        const int64_t k = std::max<int64_t>(P.opts.k_hint, 64);
+        P.caps.res_entries = (int32_t)std::max<int64_t>(256, 4096 * 128 / k);
+    }
     if (P.opts.tile_cap) P.caps.tiles = P.opts.tile_cap;
     if (P.opts.tc_cap) P.caps.tc_sub = P.opts.tc_cap;
     if (P.opts.res_cap) P.caps.res_entries = P.opts.res_cap;

@@ -41,13 +41,14 @@ class TuneResult:
     best: dict
     ms: float
     table: list = field(default_factory=list)     # (candidate, ms)
+    k_hint: int = 0                                # the K the candidates were timed at
 
     def router_kwargs(self) -> dict:
         return {k: v for k, v in self.best.items() if k not in ("engine", "slab", "tile_cap", "item_order")}
 
     def plan_opts(self) -> dict:
         c = self.best.get("tile_cap", 0)
-        return {"tile_cap": c, "tc_cap": c, "item_order": self.best.get("item_order", 0)}
+        return {"tile_cap": c, "tc_cap": c, "item_order": self.best.get("item_order", 0), "k_hint": self.k_hint}
 
 
 def autotune(rowptr, col, val, ncols: int, m: int, B: torch.Tensor, *, dtype=torch.float16,
@@ -64,10 +65,10 @@ def autotune(rowptr, col, val, ncols: int, m: int, B: torch.Tensor, *, dtype=tor
         for kc in kernel_candidates:
             cap = kc.get("tile_cap", 0)
             plan = Plan.from_meta(meta, m, ncols, dtype, engine=kc["engine"], slab=kc["slab"], tile_cap=cap, tc_cap=cap,
-                                  item_order=kc.get("item_order", 0))
+                                  item_order=kc.get("item_order", 0), k_hint=B.shape[1])
             ms = plan.timed(B, C, warmup=warmup, iters=iters)
             table.append(({**rc, **kc}, ms))
             del plan
         del meta
     best, ms = min(table, key=lambda t: t[1])
-    return TuneResult(best, ms, table)
+    return TuneResult(best, ms, table, k_hint=int(B.shape[1]))
