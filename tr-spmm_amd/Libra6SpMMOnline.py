@@ -97,15 +97,17 @@ def _run(plan: Plan, B, k8, mOri, dimN, epoches, masks):
     return out[:, : int(dimN)].cpu().contiguous(), times
 
 
-def _plan_from_view(view_tensors, n_windows, mOri, kOri, keep):
+def _plan_from_view(view_tensors, n_windows, mOri, kOri, keep, dimN):
     v = _lib.MetaView()
     for k, val in view_tensors.items():
         setattr(v, k, val)
     v.n_windows = n_windows
     v.window = 16
+    opts = _lib.PlanOpts()
+    opts.k_hint = int(dimN)                   # the op runs at one K: size the work items for it
     h = ctypes.c_void_p()
-    check(lib().trspmm_plan_create(ctypes.byref(v), int(mOri), int(kOri), _lib.F16, _lib.current_stream(),
-                                   ctypes.byref(h)), "trspmm_plan_create")
+    check(lib().trspmm_plan_create_ex(ctypes.byref(v), int(mOri), int(kOri), _lib.F16, ctypes.byref(opts),
+                                      _lib.current_stream(), ctypes.byref(h)), "trspmm_plan_create_ex")
     return Plan(h, int(mOri), int(kOri), torch.float16, keep)
 
 
@@ -220,7 +222,7 @@ def forward_fp16_tcu_cuda_sptc_mma_parallel_online(
         "s_column": ptr(D["s_column"]), "n_groups": G, "s_value": ptr(D["s_value"]), "s_pos": ptr(D["s_pos"]),
         "s_offsets": ptr(D["s_offsets"]), "t_value_is_f32": 0, "c_value_is_f32": 0,
     }
-    plan = _plan_from_view(view, W, mOri, kOri, keep=tuple(D.values()))
+    plan = _plan_from_view(view, W, mOri, kOri, keep=tuple(D.values()), dimN=dimN)
     B, k8 = _operand(T["rhs_matrix"], kOri, dimN)
     out, (ms, ms_tc, ms_res, ms_sptc) = _run(plan, B, k8, mOri, dimN, epoches,
                                              [_lib.ROUTE_ALL, _lib.ROUTE_TC, _lib.ROUTE_RESIDUAL, _lib.ROUTE_SPTC])
@@ -237,7 +239,7 @@ def _sptc_only(sc, sv, sp, so, rhs, window, dimN, mOri, kOri, epoches):
     Dsc, Dsv, Dsp, Dso = _dev(sc), _dev(sv), _dev(sp), _dev(so)
     view = {"t_row_offset": ptr(z), "t_block_offset": ptr(z), "c_row_offset": ptr(z), "cs_row_offset": ptr(z),
             "s_column": ptr(Dsc), "n_groups": G, "s_value": ptr(Dsv), "s_pos": ptr(Dsp), "s_offsets": ptr(Dso)}
-    plan = _plan_from_view(view, W, mOri, kOri, keep=(z, Dsc, Dsv, Dsp, Dso))
+    plan = _plan_from_view(view, W, mOri, kOri, keep=(z, Dsc, Dsv, Dsp, Dso), dimN=dimN)
     B, k8 = _operand(rhs, kOri, dimN)
     out, (ms,) = _run(plan, B, k8, mOri, dimN, epoches, [_lib.ROUTE_ALL])
     return [out, torch.tensor(ms, dtype=torch.float32)]
