@@ -1,35 +1,40 @@
 """Routed SpMM benchmark (one MI355X).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rmat18_k128] [--mode tuned]
+                    [--detail FILE]
 
-A step is one routed multiply C = A·B over the whole matrix (one fused
-kernel launch), with A's routed plan and B already resident in HBM.
-Preprocessing (device router + plan) runs once before the timed region.
-The default routing is the tuned router (2:4 groups with fewer than 8 of 32
-slots filled go to the residual route, which runs as one-hot MFMA fragments
-in the same kernel); config.exact_routing times the same product on the
-reference's tiling in the same run.
-Prints ONE JSON line (rank 0):
-  value      GFLOP/s = 2·nnz·K / (ms_per_step)
-  roofline   HBM-bound: algorithmic bytes per launch
-             nnz·(2+4) + 4·(M+1) + N·K·2 + M·K·4   (fp16 values/B, int32 cols, fp32 C)
-             over the kernel's mean duration (HIP events on the launch stream),
-             against the 8 TB/s MI355X peak; traffic = measured HBM bytes per launch
-             (rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/) when recorded, else null
-  cpu_baseline  the reference's CPU path on the host: torch.sparse.mm (fp32) with
-             the host threads of one GPU's share, plus its router on a row sample
-  gpu_library_baseline  (default run; --no-extras skips it) the GPU sparse
-             library's SpMM (hipSPARSE via torch.sparse.mm, fp32) on the same
-             matrix, a reference point only
-  fwd_bwd    (default run; --no-extras skips it) the same R-MAT graph at K=256 in
-             fp16 and bf16: plain forward vs forward + backward (dB = A^T·dY and
-             dvals = SDDMM(dY, B)), with errors vs FP64, on the reference's
-             tiling; tuned_routing repeats forward and forward + backward with
-             the tuned router (the backward is the same either way)
-  batched    (default run; --no-extras skips it) 1024 Barabasi-Albert graphs
-             (m=3, n_i log-uniform in [20, 5000]), K=128: one autotuned launch
-             over the concatenated tile queue (eager and replayed from a
-             captured hipGraph) vs 1024 sequential per-graph launches
+A step is one routed multiply C = A·B over the whole matrix: R-MAT scale 18
+(262 144 rows, 8 393 760 nnz), K = 128, fp16 operands, fp32 accumulation and
+output, with A's routed plan and B already resident in HBM.  Preprocessing
+(device router + plan) runs once before the timed region.  The default
+routing is the tuned router (2:4 groups with fewer than 8 of 32 slots filled
+go to the residual route, which runs as one-hot MFMA fragments in the same
+kernel); config.exact_ms times the same product on the reference's tiling.
+
+Prints ONE JSON line (at most about 2 KB) on stdout:
+  value        GFLOP/s = 2·nnz·K / ms_per_step (HIP events on the launch stream)
+  roofline     HBM-bound: algorithmic bytes per multiply
+               nnz·(2+4) + 4·(M+1) + N·K·2 + M·K·4   (fp16 values/B, int32 cols, fp32 C)
+               over ms_per_step, against the 8 TB/s MI355X peak; traffic = measured
+               HBM bytes per multiply (rocprofv3 FETCH_SIZE + WRITE_SIZE, recorded in
+               profiles/) when recorded for this workload, else null; copy_gbs = a
+               plain device copy measured in the same run
+  config       workload, sizes, FP64 error, router time, exact-routing time and the
+               multiply timed after flushing the L2 and the 256 MB MALL (ms_cold)
+  cpu_baseline the reference's CPU path on the host: torch.sparse.mm (fp32) over the
+               whole workload with the host threads of one GPU's share, and the
+               oracle's faithful restatement of the reference router, single-threaded
+               like the reference, on the leading (hub) rows
+  fwd_bwd      (extras) the same graph at K = 256, fp16 and bf16: tuned forward vs
+               tuned forward + fused backward (dB = A^T·dY and dvals = SDDMM(dY, B)),
+               the ratio on the reference's tiling beside it, errors vs FP64
+  batched      (extras) 1024 Barabasi-Albert graphs (m=3, n_i log-uniform in
+               [20, 5000]), K=128: one autotuned launch replayed from a captured
+               hipGraph vs 1024 sequential per-graph launches, error vs FP64
+  configs      (extras) the ER n=4096 K=64 and three-route 2^20 K=256 products:
+               time, roofline fraction and error vs FP64
+Full tables (autotuner candidates, per-route figures) go to stderr and, with
+--detail FILE, into FILE as JSON.  --no-extras times the headline only.
 """
 from __future__ import annotations
 
@@ -53,7 +58,8 @@ def parse():
     p.add_argument("--steps", type=int, default=200)
     # the first ~60 multiplies after routing run about 5 % slower than the rest
     # (measured: 0.215 ms for 10 warm-up + 50 timed steps, 0.205 ms for every
-    # later set of 50), so the default warm-up is long enough to reach steady state
+    # later set of 50); the default warm-up reaches steady state, a shorter one
+    # given on the command line is honoured as given
     p.add_argument("--warmup", type=int, default=200)
     p.add_argument("--workload", default="rmat18_k128")
     p.add_argument("--mode", default="tuned", choices=["exact", "compat", "tuned"],
@@ -64,7 +70,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-check", action="store_true")
     p.add_argument("--no-extras", action="store_true",
-                   help="skip the exact-routing comparison, the library baseline, fwd+bwd and the batched measurements")
+                   help="skip the exact-routing comparison, fwd+bwd, the batched graphs and the other configs")
+    p.add_argument("--detail", default=None, help="write the run's full tables to this JSON file")
     p.add_argument("--batched", action="store_true", help="measure only the batched workload")
     p.add_argument("--per-route", action="store_true",
                    help="per-route rows, nnz and times of --workload (exact and tuned routing)")
@@ -72,6 +79,8 @@ def parse():
                    help="measure only forward vs forward+backward on --workload (fp16 and bf16)")
     p.add_argument("--matching", action="store_true",
                    help="time the four setup_24matching statistics ops on --workload's matrix")
+    p.add_argument("--library", action="store_true",
+                   help="time the GPU sparse library's fp32 SpMM (torch.sparse.mm) on --workload's matrix")
     p.add_argument("--copy-calibration", action="store_true",
                    help="also run a plain 256 MiB device copy (the reference kernel for scaling "
                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -84,15 +93,14 @@ def algorithmic_bytes(nnz, m, n, k, vb=2, cb=4):
 
 def cpu_baseline(csr, K, B16, budget_s=20.0):
     """The reference's CPU path on the host cores: its router as preprocessing
-    (the reference's own block_online.cpp built into oracle/_ref when present,
-    else the oracle's restatement of it; single-threaded like the reference)
-    on a bounded leading row sample, and torch.sparse.mm(A_csr fp32, B fp32)
-    over the whole workload with every host thread of this GPU's share
-    (median of 5 after one warm-up).  `value` is the multiply's GFLOP/s."""
+    (the oracle's faithful restatement of block_online.cpp, single-threaded like
+    the reference, printf-free) on a bounded leading row sample, and
+    torch.sparse.mm(A_csr fp32, B fp32) over the whole workload with every host
+    thread of this GPU's share (median of 5 after one warm-up).  `value` is the
+    multiply's GFLOP/s."""
     import torch
 
     import oracle
-    import oracle.ref as ref
     threads = max(1, min(16, os.cpu_count() or 1))
     torch.set_num_threads(threads)
     A = torch.sparse_csr_tensor(torch.from_numpy(csr.rowptr.astype(np.int64)), torch.from_numpy(csr.col.astype(np.int64)),
@@ -107,29 +115,21 @@ def cpu_baseline(csr, K, B16, budget_s=20.0):
     t_mm = float(np.median(times))
     # router on a leading row sample (grown x4 while it takes under budget_s / 8;
     # R-MAT's leading rows are its hubs, where the reference's matching is quadratic)
-    use_ref = ref.available()
     rows = 16 * 16
     while True:
         rows = min(rows, csr.m_pad)
         rp = csr.rowptr[: rows + 1]
         nnz = int(rp[-1])
         t0 = time.perf_counter()
-        if use_ref:
-            ref.route(rp, csr.col[:nnz], csr.val[:nnz])
-        else:
-            oracle.route(rp, csr.col[:nnz], csr.val[:nnz], faithful=True)
+        oracle.route(rp, csr.col[:nnz], csr.val[:nnz], faithful=True)
         dt = time.perf_counter() - t0
         if dt * 4 > budget_s / 2 or rows >= csr.m_pad:
             break
         rows *= 4
-    route_rate = nnz / dt
-    return {"value": round(2.0 * csr.nnz * K / t_mm / 1e9, 3), "unit": "GFLOP/s", "cores": threads,
-            "kind": "reference" if use_ref else "port",
-            "sample": f"torch.sparse.mm fp32 CSR x fp32 B over the whole workload ({csr.nnz} nnz, K={K}), "
-                      f"{threads} threads, median {t_mm * 1e3:.1f} ms; preprocessing: "
-                      f"{'the reference router (oracle/_ref)' if use_ref else 'the oracle restatement of the reference router (faithful)'}"
-                      f", 1 thread, rows [0, {rows}) ({nnz} nnz, the hub rows) in {dt:.2f} s "
-                      f"({route_rate / 1e6:.3f} M nnz/s)"}
+    return {"value": round(2.0 * csr.nnz * K / t_mm / 1e9, 3), "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "sample": f"torch.sparse.mm fp32, whole workload, {threads} threads, median {t_mm * 1e3:.1f} ms; "
+                      f"router (faithful port, 1 thread) on rows [0,{rows}) ({nnz} nnz): {dt:.2f} s, "
+                      f"{nnz / dt / 1e6:.3f} M nnz/s"}
 
 
 def traffic_record(workload, mode, dtype):
@@ -162,11 +162,44 @@ def time_steps(fn, steps, warmup):
     return e0.elapsed_time(e1) / steps
 
 
-def measure_fwd_bwd(csr, K, dtype_name, mode, steps, warmup, check=True):
-    """Forward vs forward + backward (dB = A^T dY on A^T's routed plan, dvals =
-    SDDMM(dY, B)) on one graph, routed in `mode`; preprocessing (both plans)
-    outside the timing.  The fused backward works on A's CSR, so the same
-    backward follows the tuned router's forward in `tuned_routing`."""
+def time_cold(fn, iters, flush_bytes=512 << 20):
+    """Mean ms of fn() when it starts with the L2s and the 256 MB MALL holding
+    none of its data: before each call a 512 MiB buffer is read and written
+    (outside the events), so only fn() itself is timed."""
+    import torch
+    stream = torch.cuda.current_stream()
+    flush = torch.zeros(flush_bytes // 4, dtype=torch.float32, device="cuda")
+    ev = []
+    for _ in range(iters):
+        flush.add_(1.0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        ev.append((e0, e1))
+    torch.cuda.synchronize()
+    ms = sum(a.elapsed_time(b) for a, b in ev) / iters
+    del flush
+    return ms
+
+
+def copy_bandwidth(nbytes=1 << 30, iters=10):
+    """Achievable HBM bandwidth of a plain device copy (GB/s, read + write bytes)."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    ms = time_steps(lambda: dst.copy_(src), iters, 2)
+    del src, dst
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
+
+
+def measure_fwd_bwd(csr, K, dtype_name, steps, warmup, check=True, full=False):
+    """Forward vs forward + backward on one graph at width K: the tuned
+    router's forward (the headline routing) with the fused backward (dB =
+    A^T dY and dvals = SDDMM(dY, B) in one pass over A^T's rows), and the same
+    on the reference's tiling (exact routing).  Preprocessing (plans) outside
+    the timing.  full=True also times the routed backward (dB on A^T's routed
+    plan, then the SDDMM kernel) and each backward kernel alone."""
     import torch
 
     from trspmm import graphs
@@ -175,75 +208,47 @@ def measure_fwd_bwd(csr, K, dtype_name, mode, steps, warmup, check=True):
     dev = torch.device("cuda", 0)
     B = torch.from_numpy(graphs.dense_features(csr.n, K, seed=1001).astype(np.float32)).to(dev).to(dtype)
     dY = torch.from_numpy(graphs.dense_features(csr.m, K, seed=1003).astype(np.float32)).to(dev).to(dtype)
-    t0 = time.time()
-    A = RoutedSparse(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val), csr.n,
-                     m=csr.m, mode=mode, dtype=dtype, plan_opts={"k_hint": K})
-    At = A.transpose()
-    torch.cuda.synchronize()
-    prep = time.time() - t0
     C = torch.empty((csr.m, K), device=dev, dtype=torch.float32)
     dB = torch.empty((csr.n, K), device=dev, dtype=torch.float32)
     dv = torch.empty(csr.nnz, device=dev, dtype=torch.float32)
-
-    def fwd():
-        A.matmul(B, C)
-
-    def fwd_bwd_routed():
-        A.matmul(B, C)
-        At.matmul(dY, dB)
-        A.sddmm(dY, B, dv)
-
-    def fwd_bwd():
-        A.matmul(B, C)
-        A.spmm_t_sddmm(dY, B, dB, dv)
-
-    A.spmm_t_sddmm(dY, B, dB, dv)                   # builds the fused plan (preprocessing)
-    torch.cuda.synchronize()
-    ms_f = time_steps(fwd, steps, warmup)
-    ms_fb = time_steps(fwd_bwd, steps, warmup)
-    ms_fbr = time_steps(fwd_bwd_routed, steps, warmup)
-    ms_fused = time_steps(lambda: A.spmm_t_sddmm(dY, B, dB, dv), steps, warmup)
-    ms_db = time_steps(lambda: At.matmul(dY, dB), steps, warmup)
-    ms_sd = time_steps(lambda: A.sddmm(dY, B, dv), steps, warmup)
-    # the same product routed by the tuned router: its forward, and the step
-    # with the same fused backward (which works on A's CSR, whatever the routing)
-    A_t = RoutedSparse(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val), csr.n,
-                       m=csr.m, mode="tuned", dtype=dtype, plan_opts={"k_hint": K})
-    A_t.spmm_t_sddmm(dY, B, dB, dv)
-    torch.cuda.synchronize()
-    ms_f_t = time_steps(lambda: A_t.matmul(B, C), steps, warmup)
-    ms_fb_t = time_steps(lambda: (A_t.matmul(B, C), A_t.spmm_t_sddmm(dY, B, dB, dv)), steps, warmup)
-    del A_t
-    if check:
-        A.spmm_t_sddmm(dY, B, dB, dv)                # the fused results are the ones checked below
-    flops = 3 * 2.0 * csr.nnz * K
-    out = {"workload": f"rmat18 K={K}", "dtype": dtype_name, "mode": mode, "ms_fwd": round(ms_f, 5),
-           "ms_fwd_bwd": round(ms_fb, 5), "ratio_fwd_bwd_over_fwd": round(ms_fb / ms_f, 3),
-           "backward": "fused (dB + dvals in one pass over A^T rows)", "ms_bwd_fused": round(ms_fused, 5),
-           "ms_fwd_bwd_routed": round(ms_fbr, 5), "ms_dB_routed": round(ms_db, 5), "ms_sddmm": round(ms_sd, 5),
-           "gflops_fwd_bwd": round(flops / (ms_fb * 1e-3) / 1e9, 1),
-           "roofline_frac_fwd_bwd": round(
-               (algorithmic_bytes(csr.nnz, csr.m, csr.n, K) + algorithmic_bytes(csr.nnz, csr.n, csr.m, K)
-                + csr.nnz * 8 + 4 * (csr.m + 1) + (csr.m + csr.n) * K * 2) / (ms_fb * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-           "prep_s": round(prep, 3),
-           "tuned_routing": {"ms_fwd": round(ms_f_t, 5), "ms_fwd_bwd": round(ms_fb_t, 5),
-                             "ratio_fwd_bwd_over_fwd": round(ms_fb_t / ms_f_t, 3)}}
-    if check:
-        import scipy.sparse as sp
-        Af = sp.csr_matrix((csr.val.astype(np.float64), csr.col, csr.rowptr), shape=(csr.m_pad, csr.n))
-        dY64 = np.zeros((csr.m_pad, K))
-        dY64[: csr.m] = dY.float().cpu().numpy()
-        B64 = B.float().cpu().numpy().astype(np.float64)
-        ref = np.asarray(Af.T @ dY64)
-        got = dB.cpu().numpy().astype(np.float64)
-        out["rel_err_dB_vs_fp64"] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
-        rng = np.random.default_rng(0)
-        sample = rng.choice(csr.nnz, min(csr.nnz, 50000), replace=False)
-        rows = np.searchsorted(csr.rowptr, sample, side="right") - 1
-        ref_v = np.einsum("ef,ef->e", dY64[rows], B64[csr.col[sample]])
-        got_v = dv.cpu().numpy()[sample].astype(np.float64)
-        out["rel_err_dvals_vs_fp64"] = float(np.linalg.norm(got_v - ref_v) / np.linalg.norm(ref_v))
-    del A, At
+    out = {"K": K, "dtype": dtype_name}
+    for mode in ("tuned", "exact"):
+        A = RoutedSparse(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val), csr.n,
+                         m=csr.m, mode=mode, dtype=dtype, plan_opts={"k_hint": K})
+        A.spmm_t_sddmm(dY, B, dB, dv)                   # builds the fused plan (preprocessing)
+        torch.cuda.synchronize()
+        ms_f = time_steps(lambda: A.matmul(B, C), steps, warmup)
+        ms_fb = time_steps(lambda: (A.matmul(B, C), A.spmm_t_sddmm(dY, B, dB, dv)), steps, warmup)
+        out[mode] = {"ms_fwd": round(ms_f, 5), "ms_fwd_bwd": round(ms_fb, 5), "ratio": round(ms_fb / ms_f, 3)}
+        if mode == "tuned":
+            out["ms_bwd_fused"] = round(time_steps(lambda: A.spmm_t_sddmm(dY, B, dB, dv), steps, warmup), 5)
+            if full:
+                At = A.transpose()
+                torch.cuda.synchronize()
+                out["ms_fwd_bwd_routed"] = round(time_steps(
+                    lambda: (A.matmul(B, C), At.matmul(dY, dB), A.sddmm(dY, B, dv)), steps, warmup), 5)
+                out["ms_dB_routed"] = round(time_steps(lambda: At.matmul(dY, dB), steps, warmup), 5)
+                out["ms_sddmm"] = round(time_steps(lambda: A.sddmm(dY, B, dv), steps, warmup), 5)
+                del At
+            if check:
+                A.spmm_t_sddmm(dY, B, dB, dv)        # the fused results are the ones checked below
+                torch.cuda.synchronize()
+                import scipy.sparse as sp
+                Af = sp.csr_matrix((csr.val.astype(np.float64), csr.col, csr.rowptr), shape=(csr.m_pad, csr.n))
+                dY64 = np.zeros((csr.m_pad, K))
+                dY64[: csr.m] = dY.float().cpu().numpy()
+                B64 = B.float().cpu().numpy().astype(np.float64)
+                ref = np.asarray(Af.T @ dY64)
+                got = dB.cpu().numpy().astype(np.float64)
+                out["err_dB"] = float(f"{np.linalg.norm(got - ref) / np.linalg.norm(ref):.3e}")
+                rng = np.random.default_rng(0)
+                sample = rng.choice(csr.nnz, min(csr.nnz, 50000), replace=False)
+                rows = np.searchsorted(csr.rowptr, sample, side="right") - 1
+                ref_v = np.einsum("ef,ef->e", dY64[rows], B64[csr.col[sample]])
+                got_v = dv.cpu().numpy()[sample].astype(np.float64)
+                out["err_dvals"] = float(f"{np.linalg.norm(got_v - ref_v) / np.linalg.norm(ref_v):.3e}")
+        del A
+        torch.cuda.empty_cache()
     return out
 
 
@@ -458,6 +463,53 @@ def measure_matching(csr, workload, steps, warmup):
     return out
 
 
+def measure_config(workload, steps, warmup, mode="tuned", check=True, sample_rows=8192):
+    """One product of the workload table (trspmm.graphs.WORKLOADS): routed
+    multiply time, roofline fraction and relative Frobenius error against the
+    FP64 CSR product (all rows, or a seeded sample of `sample_rows` rows for
+    matrices of more than 2^18 rows)."""
+    import torch
+
+    from trspmm import graphs
+    from trspmm.sparse import RoutedSparse
+    builder, K, bseed = graphs.WORKLOADS[workload]
+    csr = builder()
+    B16 = graphs.dense_features(csr.n, K, seed=bseed)
+    dev = torch.device("cuda", 0)
+    Bt = torch.from_numpy(B16).to(dev)
+    A = RoutedSparse(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val), csr.n,
+                     m=csr.m, mode=mode, plan_opts={"k_hint": K})
+    C = torch.empty((csr.m, K), device=dev, dtype=torch.float32)
+    ms = time_steps(lambda: A.matmul(Bt, C), steps, warmup)
+    byts = algorithmic_bytes(csr.nnz, csr.m, csr.n, K)
+    out = {"rows": csr.m, "nnz": csr.nnz, "K": K, "mode": mode, "ms": round(ms, 5),
+           "gflops": round(2.0 * csr.nnz * K / (ms * 1e-3) / 1e9, 1),
+           "frac": round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "router_s": round(A.meta.seconds, 4)}
+    if check:
+        import scipy.sparse as sp
+        rows = np.arange(csr.m)
+        if csr.m > (1 << 18):
+            rows = np.sort(np.random.default_rng(7).choice(csr.m, sample_rows, replace=False))
+        Af = sp.csr_matrix((csr.val.astype(np.float64), csr.col, csr.rowptr), shape=(csr.m_pad, csr.n))[rows]
+        ref = np.asarray(Af @ B16.astype(np.float64))
+        got = C[torch.from_numpy(rows).to(dev)].cpu().numpy().astype(np.float64)
+        out["err"] = float(f"{np.linalg.norm(got - ref) / np.linalg.norm(ref):.3e}")
+        out["err_rows"] = "all" if rows.size == csr.m else f"{rows.size} sampled"
+    del A, C, Bt
+    torch.cuda.empty_cache()
+    return out
+
+
+def emit_detail(detail, path):
+    """The run's full tables: stderr always, `path` when given."""
+    print("[bench] detail " + json.dumps(detail), file=sys.stderr, flush=True)
+    if path:
+        d = os.path.dirname(os.path.abspath(path))
+        os.makedirs(d, exist_ok=True)
+        with open(path, "w") as f:
+            json.dump(detail, f, indent=1)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -470,7 +522,10 @@ def main():
     from trspmm.sparse import RoutedSparse
 
     if a.batched:
-        print(json.dumps(measure_batched(a.steps, a.warmup, not a.no_check)), flush=True)
+        rec = measure_batched(a.steps, a.warmup, not a.no_check)
+        emit_detail({"batched": rec}, a.detail)
+        rec.pop("autotune_table_ms", None)
+        print(json.dumps(rec), flush=True)
         return
     builder, K, bseed = graphs.WORKLOADS[a.workload]
     t0 = time.time()
@@ -492,8 +547,10 @@ def main():
         return
     if a.backward:
         for dt in ("fp16", "bf16"):
-            print(json.dumps(measure_fwd_bwd(csr, K, dt, "exact" if a.mode == "tuned" else a.mode, a.steps, a.warmup,
-                                             not a.no_check)), flush=True)
+            print(json.dumps(measure_fwd_bwd(csr, K, dt, a.steps, a.warmup, not a.no_check, full=True)), flush=True)
+        return
+    if a.library:
+        print(json.dumps(gpu_library_baseline(csr, graphs.dense_features(csr.n, K, seed=bseed))), flush=True)
         return
     dtype = torch.float16 if a.dtype == "fp16" else torch.bfloat16
     B16 = graphs.dense_features(csr.n, K, seed=bseed)                    # fp16 N(0,1)
@@ -517,7 +574,6 @@ def main():
         A.matmul(Bt, C)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
     t_wall = time.perf_counter()
     e0.record(stream)
     for _ in range(a.steps):
@@ -526,6 +582,7 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_wall
     ms = e0.elapsed_time(e1) / a.steps
+    print(f"[bench] {a.steps} steps: {ms:.5f} ms per multiply", file=sys.stderr, flush=True)
     flops = 2.0 * csr.nnz * K
     gflops = flops / (ms * 1e-3) / 1e9
     byts = algorithmic_bytes(csr.nnz, csr.m, csr.n, K)
@@ -546,44 +603,58 @@ def main():
         Af = sp.csr_matrix((csr.val.astype(np.float64), csr.col, csr.rowptr), shape=(csr.m_pad, csr.n))
         ref = np.asarray(Af @ B16.astype(np.float64))[: csr.m]
         got = C.cpu().numpy().astype(np.float64)
-        err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
-    exact_routing = None
+        err = float(f"{np.linalg.norm(got - ref) / np.linalg.norm(ref):.3e}")
+    ms_cold = time_cold(lambda: A.matmul(Bt, C), max(5, min(a.steps, 20)))
+    copy_gbs = copy_bandwidth()
+    detail = {"routes": A.plan.stats, "plan_opts": A.plan_opts}
+    exact_ms = None
     if a.mode != "exact" and not a.no_extras:
         # the same product on the reference's tiling (exact routing), timed the same way
         A_x = RoutedSparse(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val),
                            csr.n, m=csr.m, mode="exact", dtype=dtype, plan_opts={"k_hint": K})
-        ms_x = time_steps(lambda: A_x.matmul(Bt, C), a.steps, a.warmup)
-        exact_routing = {"ms": round(ms_x, 5), "gflops": round(flops / (ms_x * 1e-3) / 1e9, 2),
-                         "roofline_frac": round(byts / (ms_x * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                         "routes": A_x.plan.stats}
+        exact_ms = round(time_steps(lambda: A_x.matmul(Bt, C), a.steps, a.warmup), 5)
+        detail["exact_routes"] = A_x.plan.stats
         del A_x
-    traffic = traffic_record(a.workload, a.mode, a.dtype)
     line = {
         "metric": "SpMM GFLOP/s (2·nnz·K) + %HBM roofline",
         "value": round(gflops, 2), "unit": "GFLOP/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": a.dtype, "data": "synthetic (seeded generator, trspmm.graphs)",
+        "dtype": a.dtype, "data": "synthetic (seeded generators, trspmm.graphs)",
         "config": {"workload": a.workload, "rows": csr.m, "nnz": csr.nnz, "K": K, "mode": a.mode,
-                   "plan_opts": A.plan_opts, "routes": A.plan.stats, "rel_fro_err_vs_fp64": err, "router_s": round(A.meta.seconds, 4),
-                   "exact_routing": exact_routing,
-                   "wall_s_timed": round(wall, 4)},
+                   "rel_err_fp64": err, "router_s": round(A.meta.seconds, 4), "exact_ms": exact_ms,
+                   "ms_cold": round(ms_cold, 5), "wall_s_timed": round(wall, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes": int(byts)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_record(a.workload, a.mode, a.dtype), "algorithmic_bytes": int(byts),
+                     "kernel": "k_tile_routes", "copy_gbs": round(copy_gbs, 1)},
     }
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(csr, K, B16)
+    del A, C, Bt
+    torch.cuda.empty_cache()
     if not a.no_extras and a.workload == "rmat18_k128":
-        del A, C, Bt
-        torch.cuda.empty_cache()
-        line["gpu_library_baseline"] = gpu_library_baseline(csr, B16)
         print("[bench] fwd+bwd at K=256", file=sys.stderr, flush=True)
-        line["fwd_bwd"] = [measure_fwd_bwd(csr, 256, dt, "exact" if a.mode == "tuned" else a.mode, 20, 5,
-                                           not a.no_check) for dt in ("fp16", "bf16")]
+        fb = {dt: measure_fwd_bwd(csr, 256, dt, a.steps, a.warmup, not a.no_check) for dt in ("fp16", "bf16")}
+        detail["fwd_bwd"] = fb
+        line["fwd_bwd"] = {dt: {"ms_fwd": r["tuned"]["ms_fwd"], "ms_fwd_bwd": r["tuned"]["ms_fwd_bwd"],
+                                "ratio": r["tuned"]["ratio"], "ratio_exact": r["exact"]["ratio"],
+                                "err_dB": r.get("err_dB"), "err_dvals": r.get("err_dvals")}
+                           for dt, r in fb.items()}
         del csr
         torch.cuda.empty_cache()
         print("[bench] batched graphs", file=sys.stderr, flush=True)
-        line["batched"] = measure_batched(20, 5, not a.no_check)
+        bt = measure_batched(a.steps, a.warmup, not a.no_check)
+        detail["batched"] = bt
+        line["batched"] = {"ms_graph": bt["ms_batched_graph"], "ms_seq": bt["ms_sequential_plain"],
+                           "speedup": bt["speedup_vs_sequential"], "frac": bt["roofline_frac_batched"],
+                           "err": bt.get("rel_err_batched_vs_fp64")}
+        torch.cuda.empty_cache()
+        print("[bench] configs 1 and 3", file=sys.stderr, flush=True)
+        cfgs = {w: measure_config(w, a.steps, a.warmup, check=not a.no_check)
+                for w in ("er4096_k64", "three_route_k256")}
+        detail["configs"] = cfgs
+        line["configs"] = {w: {k: r[k] for k in ("ms", "frac", "err") if k in r} for w, r in cfgs.items()}
+    emit_detail(detail, a.detail)
     print(json.dumps(line), flush=True)
 
 
