@@ -290,6 +290,20 @@ int trspmm_match_2to4_bucket(const int32_t* rowptr, const int32_t* col, int64_t 
 int trspmm_match_2to4_online(const int32_t* rowptr, const int32_t* col, int64_t rows, int64_t nnz,
                              int dense_threshold, int t_max, unsigned long long* counters, void* stream);
 
+/* Allocation-free form of trspmm_match_2to4_online: same arguments, same
+ * counters, no device allocation and no allocating library call.  It has no
+ * counterpart in the reference; it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its scratch (the window order) is a fixed 8 MiB array of
+ * device memory per GPU, held until the process ends, so it accepts at most
+ * TRSPMM_ONLINE_NOALLOC_MAX_ROWS rows (more return TRSPMM_EINVAL).  Every call
+ * on a GPU shares that array: calls that overlap on one GPU (concurrent
+ * streams) race and must not be issued. */
+#define TRSPMM_ONLINE_NOALLOC_MAX_ROWS (((1LL << 21) - 64) * 16)
+int trspmm_match_2to4_online_noalloc(const int32_t* rowptr, const int32_t* col, int64_t rows, int64_t nnz,
+                                     int dense_threshold, int t_max, unsigned long long* counters,
+                                     void* stream);
+
 /* ------------------------------------------------------------------ TCA row reordering
  * Replaces the row reordering of the reference's caller
  * (spmm/tr_csr_fp16/tr_tca_reorder.py:44-357, apply_tca_reorder): MinHash-LSH

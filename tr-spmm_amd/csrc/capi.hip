@@ -12,6 +12,8 @@ namespace trspmm {
 
 int match_online_device(const int32_t* d_rowptr, const int32_t* d_col, int64_t rows, int64_t nnz,
                         int dense_threshold, int t_max, unsigned long long* d_out, hipStream_t stream);
+int match_online_device_noalloc(const int32_t* d_rowptr, const int32_t* d_col, int64_t rows, int64_t nnz,
+                                int dense_threshold, int t_max, unsigned long long* d_out, hipStream_t stream);
 int minhash_host_coefficients(int P, uint64_t seed, uint64_t* out);
 int minhash_device(const int32_t* d_rowptr, const int32_t* d_col, int64_t rows, int P, uint64_t seed,
                    uint32_t* d_sig, hipStream_t s);
@@ -294,6 +296,15 @@ int trspmm_match_2to4_online(const int32_t* d_rowptr, const int32_t* d_col, int6
     TRSPMM_NOTNULL(d_counters, "counters");
     TRSPMM_CHECK(nnz == 0 || d_col, "col must not be null when nnz > 0");
     return match_online_device(d_rowptr, d_col, rows, nnz, dense_threshold, t_max, d_counters, (hipStream_t)stream);
+}
+
+int trspmm_match_2to4_online_noalloc(const int32_t* d_rowptr, const int32_t* d_col, int64_t rows, int64_t nnz,
+                                     int dense_threshold, int t_max, unsigned long long* d_counters, void* stream) {
+    TRSPMM_NOTNULL(d_rowptr, "rowptr");
+    TRSPMM_NOTNULL(d_counters, "counters");
+    TRSPMM_CHECK(nnz == 0 || d_col, "col must not be null when nnz > 0");
+    return match_online_device_noalloc(d_rowptr, d_col, rows, nnz, dense_threshold, t_max, d_counters,
+                                       (hipStream_t)stream);
 }
 
 // ---------------------------------------------------------------- TCA row reordering

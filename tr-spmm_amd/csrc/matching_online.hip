@@ -36,6 +36,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
+#include <string>
 
 #include "common.h"
 
@@ -538,15 +540,13 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 
 }  // namespace
 
-int match_online_device(const int32_t* d_rowptr, const int32_t* d_col, int64_t rows, int64_t nnz,
-                        int dense_threshold, int t_max, unsigned long long* d_out, hipStream_t stream) {
-    TRSPMM_CHECK(rows > 0 && rows < (int64_t)1 << 31, "match_2to4_online: rows must be in [1, 2^31)");
-    TRSPMM_CHECK(nnz >= 0 && nnz < (int64_t)1 << 31, "match_2to4_online: nnz must be in [0, 2^31)");
-    TRSPMM_CHECK(dense_threshold >= 1, "match_2to4_online: dense_threshold must be >= 1");
-    const int64_t windows = (rows + OM_ROWS - 1) / OM_ROWS;
-    // stream-ordered scratch: class counts, class fill counters, the window order
-    int32_t* scratch = nullptr;
-    TRSPMM_HIP(hipMallocAsync((void**)&scratch, 4 * (2 * OM_CLASSES + windows), stream));
+// the statistics of one call: counters zeroed, windows ordered heaviest first
+// into `scratch` (2 * OM_CLASSES + windows int32: class counts, class fill
+// counters, the window order; cleared here by a kernel, not a memset), then
+// one workgroup per window
+static int match_online_launch(const int32_t* d_rowptr, const int32_t* d_col, int64_t rows, int64_t nnz,
+                               int dense_threshold, int t_max, unsigned long long* d_out, int32_t* scratch,
+                               int64_t windows, hipStream_t stream) {
     int32_t* cnt = scratch;
     int32_t* fill = scratch + OM_CLASSES;
     int32_t* order = scratch + 2 * OM_CLASSES;
@@ -558,10 +558,84 @@ int match_online_device(const int32_t* d_rowptr, const int32_t* d_col, int64_t r
                        order);
     hipLaunchKernelGGL(k_match_online, dim3((unsigned)windows), dim3(OM_WAVE), 0, stream, d_rowptr, d_col, rows, nnz,
                        dense_threshold, t_max, order, d_out);
-    const hipError_t launched = hipGetLastError();
-    TRSPMM_HIP(hipFreeAsync(scratch, stream));
-    TRSPMM_HIP(launched);
+    return hipGetLastError() == hipSuccess ? TRSPMM_OK : TRSPMM_EHIP;
+}
+
+static int check_online_args(int64_t rows, int64_t nnz, int dense_threshold) {
+    TRSPMM_CHECK(rows > 0 && rows < (int64_t)1 << 31, "match_2to4_online: rows must be in [1, 2^31)");
+    TRSPMM_CHECK(nnz >= 0 && nnz < (int64_t)1 << 31, "match_2to4_online: nnz must be in [0, 2^31)");
+    TRSPMM_CHECK(dense_threshold >= 1, "match_2to4_online: dense_threshold must be >= 1");
     return TRSPMM_OK;
+}
+
+int match_online_device(const int32_t* d_rowptr, const int32_t* d_col, int64_t rows, int64_t nnz,
+                        int dense_threshold, int t_max, unsigned long long* d_out, hipStream_t stream) {
+    int rc;
+    if ((rc = check_online_args(rows, nnz, dense_threshold))) return rc;
+    const int64_t windows = (rows + OM_ROWS - 1) / OM_ROWS;
+    // stream-ordered scratch: class counts, class fill counters, the window order
+    int32_t* scratch = nullptr;
+    TRSPMM_HIP(hipMallocAsync((void**)&scratch, 4 * (2 * OM_CLASSES + windows), stream));
+    rc = match_online_launch(d_rowptr, d_col, rows, nnz, dense_threshold, t_max, d_out, scratch, windows, stream);
+    TRSPMM_HIP(hipFreeAsync(scratch, stream));
+    if (rc) set_error("match_2to4_online: kernel launch failed");
+    return rc;
+}
+
+// ---------------------------------------------------------------- allocation-free variant
+//
+// Scratch of the allocation-free entry point: file-scope device memory, one
+// copy per GPU, held until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is a stand-in, not a device
+// address): each device's address is taken once, with that device current,
+// by hipGetSymbolAddress, and that address is what the kernels receive.
+// Every call on a GPU shares its copy, which keeps the last call's contents:
+// the counters a call needs at zero are cleared by a kernel at its start, and
+// calls that overlap on one GPU (concurrent streams) race on it.
+constexpr int64_t kOnlineScratchInts = (int64_t)1 << 21;       // 8 MiB
+constexpr int64_t kOnlineMaxWindows = kOnlineScratchInts - 2 * OM_CLASSES;
+static __device__ __align__(256) int32_t scratch[kOnlineScratchInts];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int32_t* g_scratch_dev[kMaxDevices] = {};
+
+// the current device's address of `scratch`, looked up on the device's first call
+int online_scratch(int32_t** out) {
+    int dev = 0;
+    TRSPMM_HIP(hipGetDevice(&dev));
+    TRSPMM_CHECK(dev >= 0 && dev < kMaxDevices, "match_2to4_online_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[dev] == nullptr) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            set_error(std::string("match_2to4_online_noalloc: hipGetSymbolAddress(scratch) failed: ") +
+                      hipGetErrorString(e));
+            return TRSPMM_EHIP;
+        }
+        g_scratch_dev[dev] = (int32_t*)p;
+    }
+    *out = g_scratch_dev[dev];
+    return TRSPMM_OK;
+}
+}  // namespace
+
+int match_online_device_noalloc(const int32_t* d_rowptr, const int32_t* d_col, int64_t rows, int64_t nnz,
+                                int dense_threshold, int t_max, unsigned long long* d_out, hipStream_t stream) {
+    int rc;
+    if ((rc = check_online_args(rows, nnz, dense_threshold))) return rc;
+    const int64_t windows = (rows + OM_ROWS - 1) / OM_ROWS;
+    TRSPMM_CHECK(windows <= kOnlineMaxWindows,
+                 "match_2to4_online_noalloc: at most " + std::to_string(kOnlineMaxWindows * OM_ROWS) +
+                     " rows (its fixed scratch holds the order of " + std::to_string(kOnlineMaxWindows) +
+                     " windows); use trspmm_match_2to4_online");
+    int32_t* s = nullptr;
+    if ((rc = online_scratch(&s))) return rc;
+    rc = match_online_launch(d_rowptr, d_col, rows, nnz, dense_threshold, t_max, d_out, s, windows, stream);
+    if (rc) set_error("match_2to4_online_noalloc: kernel launch failed");
+    return rc;
 }
 
 }  // namespace trspmm

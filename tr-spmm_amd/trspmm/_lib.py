@@ -25,7 +25,7 @@ EXPORTS = [
     "trspmm_plan_stats", "trspmm_spmm", "trspmm_spmm_timed", "trspmm_memcpy", "trspmm_host_sort_order",
     "trspmm_csr_transpose", "trspmm_sddmm", "trspmm_bwd_plan_create", "trspmm_bwd_plan_free",
     "trspmm_spmm_t_sddmm", "trspmm_plan_create_ex", "trspmm_plan_from_meta_ex", "trspmm_match_2to4",
-    "trspmm_match_2to4_hash", "trspmm_match_2to4_bucket", "trspmm_match_2to4_online",
+    "trspmm_match_2to4_hash", "trspmm_match_2to4_bucket", "trspmm_match_2to4_online", "trspmm_match_2to4_online_noalloc",
     "trspmm_minhash", "trspmm_minhash_coefficients", "trspmm_tca_order",
 ]
 
@@ -133,6 +133,7 @@ def lib():
     for fn in (L.trspmm_match_2to4_hash, L.trspmm_match_2to4_bucket):
         fn.argtypes = [_P, _P, _I64, _I64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(MatchStats)]
     L.trspmm_match_2to4_online.argtypes = [_P, _P, _I64, _I64, ctypes.c_int, ctypes.c_int, _P, _P]
+    L.trspmm_match_2to4_online_noalloc.argtypes = [_P, _P, _I64, _I64, ctypes.c_int, ctypes.c_int, _P, _P]
     L.trspmm_minhash.argtypes = [_P, _P, _I64, ctypes.c_int, ctypes.c_uint64, _P, _P]
     L.trspmm_minhash_coefficients.argtypes = [ctypes.c_int, ctypes.c_uint64, _P]
     L.trspmm_tca_order.argtypes = [_P, _P, _I64, _I64, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
