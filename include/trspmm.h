@@ -156,6 +156,14 @@ typedef struct trspmm_plan_opts {
                                  so that a residual work item holds about the same work at
                                  any K: 4096 * 128 / K' entries (at least 256),
                                  K' = max(K, 64).  Any K still runs on the plan. */
+    int32_t res_order;        /* order of the matrix-core residual engine's entries inside a
+                                 window: 0 = by column (the default; concurrent windows then
+                                 sweep their columns in the same direction and share B rows
+                                 in L2), 1 = row order (the residual CSR's) */
+    int32_t tile_kernel;      /* 0 = the deep-pipelined tile kernel (the default: full 128-feature
+                                 B rows per step, 16 KB of gathers in flight per wave);
+                                 1 = the earlier kernel (64-feature slabs, one sub-step ahead),
+                                 kept for comparison.  Persistent plans (item_order 2) run 1. */
     int32_t reserved;
 } trspmm_plan_opts;
 

@@ -156,11 +156,17 @@ def test_persistent_queue_many_items_per_workgroup(cuda):
     c = g.csr_from_pairs(np.repeat(np.arange(n), 6), rng.integers(0, n, 6 * n), rng.standard_normal(6 * n), n, n)
     B = torch.randn(c.n, 64, generator=torch.Generator().manual_seed(5)).half().to(cuda)
     args = (torch.from_numpy(c.rowptr), torch.from_numpy(c.col), torch.from_numpy(c.val), c.n)
-    A0 = RoutedSparse(*args, m=c.m, plan_opts={"item_order": 0})
+    # the persistent queue runs the one-sub-step-ahead tile kernel: compare it with
+    # that kernel's one-workgroup-per-item launch (the deep kernel sums in another order)
+    A0 = RoutedSparse(*args, m=c.m, plan_opts={"item_order": 0, "tile_kernel": 1})
     A2 = RoutedSparse(*args, m=c.m, plan_opts={"item_order": 2})
-    A3 = RoutedSparse(*args, m=c.m, plan_opts={"item_order": 3})
+    A3 = RoutedSparse(*args, m=c.m, plan_opts={"item_order": 3, "tile_kernel": 1})
     want = A3.matmul(B)                                # window order: heaviest-first sorting changes nothing
     assert torch.equal(A0.matmul(B), want)
     for rep in range(5):
         print(f"[persistent] launch {rep}", flush=True)
         assert torch.equal(A2.matmul(B), want), rep
+    # the deep kernel (the default) is likewise independent of the item order
+    D0 = RoutedSparse(*args, m=c.m, plan_opts={"item_order": 0})
+    D3 = RoutedSparse(*args, m=c.m, plan_opts={"item_order": 3})
+    assert torch.equal(D0.matmul(B), D3.matmul(B))

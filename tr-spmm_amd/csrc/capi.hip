@@ -34,7 +34,7 @@ extern "C" {
 
 const char* trspmm_last_error(void) { return trspmm::last_error(); }
 
-int trspmm_version(void) { return 1; }
+int trspmm_version(void) { return 2; }
 
 int trspmm_device_arch(char* buf, int len) {
     int dev = 0;

@@ -191,6 +191,8 @@ struct PlanDev {
     // pulling the heaviest-first items from the queue counters; 3 = as 0 with
     // the items in window order
     int32_t item_order = 0;
+    // tile kernel: 0 = deep-pipelined (k_routes), 1 = one sub-step ahead (k_tile_routes)
+    int32_t tile_kernel = 0;
     // persistent tile queue: queue[x * kQueueStride] = tickets taken from
     // XCD queue x, queue[8 * kQueueStride] = workgroups that have left; all
     // zero between launches

@@ -517,6 +517,396 @@ void k_tile_routes(PlanDev p, const uint16_t* __restrict__ B, int64_t ldb, int K
     }
 }
 
+// ============================================================ deep-pipelined tile routes
+//
+// k_routes: the same work items, routes and matrix instructions as
+// k_tile_routes, with the gathers reorganised around what bounds the kernel,
+// the B-row data a wave has in flight.  Measured with a gather-only kernel
+// over R-MAT scale 18's residual entries (K = 128, one wave per window, 8
+// waves per CU): 4 KB in flight per wave (k_tile_routes' figure) 0.212 ms,
+// 8 KB 0.129 ms, 16 KB 0.121 ms; direct-to-LDS loads of the same rows
+// (global_load_lds_dwordx4) were slower than loads into registers at equal
+// bytes in flight (0.138 against 0.118 ms), so the rows go through registers:
+//   * a pass covers PW = 128 features (64 when K <= 64): a step stages full
+//     B rows, 32 (TC sub-blocks, residual entries) or 64 (2:4 tiles) of them;
+//   * the next step's gathers (8 KB, or 16 KB for 64-row steps) are in flight
+//     while a step is applied (two 4 KB steps at PW = 64);
+//   * B is read as a raw buffer: one offset register per load, and masked
+//     pieces use an out-of-range offset, which returns zeros without a memory
+//     access;
+//   * a step's column indices (residual: records) are loaded 2 DEP steps
+//     ahead and staged in LDS DEP steps ahead; its A fragment is loaded DEP
+//     steps ahead, right after the previous use of its ring slot.  Loads
+//     return in issue order, so waiting for them never waits for gathers that
+//     are still wanted in flight;
+//   * staged rows sit RS = 2 PW + 32 bytes apart and logical row 32 h + 8 kg + j
+//     (j < 8) is stored as physical row 32 h + 4 j + kg.  The transposed reads
+//     of the MFMA B operand (rows 8 kg + j, j < 4, of all four kg at once) then
+//     fall on distinct 32-byte bank windows, the 16-byte parking writes cover
+//     each bank once per row, and every address is a per-lane base plus an
+//     immediate.
+template <int PW>
+struct DeepCfg {
+    static constexpr int PIECES = PW / 8;                  // 16-byte pieces per staged row
+    static constexpr int RS = 2 * PW + 32;                 // LDS bytes per staged row
+    // steps in flight for steps of R staged rows: one step of 8 KB (32 rows x 128
+    // features; 64-row steps at PW = 128 keep one step of 16 KB), or two steps of
+    // 4 KB at PW = 64.  (Gather-only kernel, R-MAT scale 18, 8 waves per CU: 16 KB
+    // per wave 0.121 ms, 8 KB 0.129 ms; 32 KB per wave at 4 waves per CU
+    // 0.154 ms; the registers of 16 KB in flight do not fit two waves per SIMD
+    // beside the step's other state.)
+    static constexpr int dep(int R) { return PW == 128 ? 1 : (R == 32 ? 2 : 1); }
+};
+
+__device__ inline int deep_phys(int L) { return (L & ~31) | ((L & 7) << 2) | ((L >> 3) & 3); }
+
+// A step's metadata travels in two hops.  Its column indices (residual: its
+// 8-byte records) are loaded into registers 2 DEP steps ahead, one per lane
+// (lane l holds staged row l), and written to a small LDS ring DEP steps
+// ahead; from there each lane reads the columns of the rows it gathers, and
+// (residual) the high words of the 8 entries its one-hot fragment covers,
+// all as one base address plus immediates.  Invalid entries reach the ring
+// as column -1 and high word 0 (the plan also stores a zero value for a
+// record whose column is invalid).
+struct DeepRec {
+    int32_t cl;
+    uint32_t hi;
+};
+// a step's A fragment (TC, 2:4), loaded right after the fragment of the step
+// DEP earlier was applied (ring of DEP); by the time it is needed, the gathers
+// issued before it have been waited for, and those issued after it stay in flight
+struct DeepFrag {
+    u32x4 a;
+    int32_t idx;
+    bool ok;
+};
+
+// staged rows of step s (s clamped to the last step)
+template <int KIND>
+__device__ inline int deep_rows(int32_t first, int32_t end, int n, int s) {
+    const int sc = s < n ? s : n - 1;
+    if constexpr (KIND == 2) {
+        const int32_t u = first + 32 * sc;
+        return end - u < 32 ? end - u : 32;
+    } else {
+        constexpr int PER = KIND == 0 ? 8 : 16;
+        const int32_t u = first + 4 * sc;
+        return (end - u) * PER < 4 * PER ? (end - u) * PER : 4 * PER;
+    }
+}
+
+// 16-byte (or smaller) load at a uniform base plus a 32-bit byte offset (the
+// scalar-base addressing form: one offset register per load, no 64-bit math)
+template <typename T>
+__device__ inline T load_at(const void* base, uint32_t byte_off) {
+    return *(const T*)((const uint8_t*)base + byte_off);
+}
+
+template <int KIND>
+__device__ inline void deep_rec(const PlanDev& p, int32_t first, int32_t end, int n, int s, int lane, DeepRec& m) {
+    const int sc = s < n ? s : n - 1;                      // clamped: the loads stay in range
+    const int nrows = deep_rows<KIND>(first, end, n, s);
+    const uint32_t l = (uint32_t)(lane < nrows ? lane : nrows - 1);
+    if constexpr (KIND == 0) {
+        m.cl = load_at<int32_t>(p.tc_cols, 4u * ((uint32_t)(first + 4 * sc) * 8u + l));
+    } else if constexpr (KIND == 1) {
+        m.cl = load_at<int32_t>(p.sp_cols, 4u * ((uint32_t)(first + 4 * sc) * 16u + l));
+    } else {
+        const uint64_t x = load_at<uint64_t>(p.res_rec, 8u * ((uint32_t)(first + 32 * sc) + l));
+        m.cl = (int32_t)(uint32_t)x;
+        m.hi = (uint32_t)(x >> 32);
+    }
+}
+
+// the records of step s into LDS ring slot ms: 8 bytes per lane, lane l's at
+// byte 8 l (lanes past the step's rows store column -1, value 0; no branch)
+template <int KIND>
+__device__ inline void deep_stage(const DeepRec& m, int32_t first, int32_t end, int n, int s, uint8_t* __restrict__ ms,
+                                  int lane) {
+    const bool v = (s < n) & (lane < deep_rows<KIND>(first, end, n, s));
+    if constexpr (KIND == 2)
+        *(u32x2*)(ms + 8 * lane) = u32x2{(uint32_t)(v ? m.cl : -1), v ? m.hi : 0u};
+    else
+        *(int32_t*)(ms + 8 * lane) = v ? m.cl : -1;
+}
+
+template <int KIND>
+__device__ inline void deep_frag(const PlanDev& p, int32_t first, int32_t end, int n, int s, int lane, DeepFrag& f) {
+    const int sc = s < n ? s : n - 1;
+    const int r = lane & 15, kg = lane >> 4;
+    const int32_t my = first + 4 * sc + kg, mc = my < end ? my : end - 1;
+    f.ok = (s < n) & (my < end);
+    const uint32_t row = (uint32_t)mc * 16u + (uint32_t)r;
+    if constexpr (KIND == 0) {
+        f.a = load_at<u32x4>(p.tc_dense, 16u * row);
+    } else if constexpr (KIND == 1) {
+        f.a = load_at<u32x4>(p.sp_a, 16u * row);
+        f.idx = load_at<uint16_t>(p.sp_idx, 2u * row);
+    }
+}
+
+// B as a raw buffer: 16-byte loads at a 32-bit byte offset from one scalar
+// descriptor.  An offset at or beyond the buffer's size returns zeros without
+// a memory access, which is how masked pieces are staged as zeros (the
+// launcher keeps B under 2^31 bytes for this kernel, so 2^31 is always out).
+constexpr uint32_t kBufferWord3 = 0x00020000u;      // gfx9-family raw buffer (32-bit data format)
+constexpr uint32_t kOutOfRange = 0x80000000u;
+
+// the step's gathers: load k of lane l is piece l % PIECES of staged row
+// RPL k + l / PIECES (RPL = 64 / PIECES rows per load), its column read from
+// the step's LDS ring slot
+template <int PW, int R>
+__device__ inline void deep_gather(u32x4 (&g)[R * DeepCfg<PW>::PIECES / 64], const uint8_t* __restrict__ ms,
+                                   __amdgpu_buffer_rsrc_t rsB, uint32_t ldb2, int f0, int fw, int lane) {
+    constexpr int PIECES = DeepCfg<PW>::PIECES;
+    constexpr int RPL = 64 / PIECES;
+    const int piece = lane % PIECES;
+    const int32_t* mcol = (const int32_t*)(ms + 8 * (lane / PIECES));
+    const bool fok = piece * 8 < fw;
+    const uint32_t fo = 2u * (uint32_t)(f0 + piece * 8);
+#pragma unroll
+    for (int k = 0; k < R * PIECES / 64; ++k) {
+        const int32_t col = mcol[2 * RPL * k];
+        const uint32_t off = (fok & (col >= 0)) ? (uint32_t)col * ldb2 + fo : kOutOfRange;
+        g[k] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rsB, off, 0, 0));
+    }
+    // keeps the loads here: the step loop's early exits would otherwise let the
+    // compiler sink them to their first use, a step or two later, which leaves
+    // nothing in flight (a compiler barrier only; no wait is emitted for it)
+    asm volatile("" ::: "memory");
+}
+
+// staged row L (load k, lane l) goes to physical row deep_phys(L): for load k
+// that is the lane's row of load 0 plus a constant
+template <int PW, int R>
+__device__ inline void deep_park(const u32x4 (&g)[R * DeepCfg<PW>::PIECES / 64], uint8_t* __restrict__ Bs, int lane) {
+    constexpr int PIECES = DeepCfg<PW>::PIECES;
+    constexpr int RPL = 64 / PIECES;
+    constexpr int RS = DeepCfg<PW>::RS;
+    const int l0 = lane / PIECES;                          // staged row of load 0
+    uint8_t* base = Bs + deep_phys(l0) * RS + 16 * (lane % PIECES);
+#pragma unroll
+    for (int k = 0; k < R * PIECES / 64; ++k) {
+        // deep_phys(RPL k + l0) - deep_phys(l0) = deep_phys(RPL k) for RPL = 4 or 8, l0 < RPL
+        const int L = RPL * k;
+        const int dphys = (L & ~31) + ((L & 7) << 2) + ((L >> 3) & 3);
+        *(u32x4*)(base + dphys * RS) = g[k];
+    }
+}
+
+// Steps [0, n) of one route kind for one pass of PW features from feature
+// fp.  Step s: park s (its gathers were issued DEP steps ago), stage the
+// records of s + DEP in LDS, load the records of s + 2 DEP, issue the gathers
+// of s + DEP into the registers just parked, apply s, then load the A
+// fragment of s + DEP into the slot s's fragment used.  The loop is unrolled
+// so that every ring index is a compile-time constant.
+template <int KIND, int PW, typename DT>
+__device__ inline void run_deep(const PlanDev& p, __amdgpu_buffer_rsrc_t rsB, uint32_t ldb2, int K, int fp,
+                                int32_t first, int32_t end, int n, uint8_t* __restrict__ Bs, uint8_t* __restrict__ Ms,
+                                int lane, f32x4 (&acc)[PW / 16]) {
+    using X8 = typename DT::x8;
+    using X16 = typename DT::x16;
+    constexpr int R = KIND == 1 ? 64 : 32;
+    constexpr int NL = R * DeepCfg<PW>::PIECES / 64;       // loads per lane per step
+    constexpr int DEP = DeepCfg<PW>::dep(R);               // steps in flight
+    static_assert(DEP == 1 || DEP == 2, "1 or 2 steps in flight");
+    constexpr int MR = 2 * DEP;                            // record / fragment rings (>= DEP + 1)
+    constexpr int U = MR;                                  // unroll: a multiple of DEP and MR
+    constexpr int NB = PW / 16;
+    constexpr int RS = DeepCfg<PW>::RS;
+    constexpr int MS = 512;                                // bytes per LDS record slot (64 lanes x 8)
+    const int r = lane & 15, kg = lane >> 4;
+    const int fw = K - fp < PW ? K - fp : PW;
+    // transposed reads: lane (kg, r) of n-block nb reads logical rows 8 kg + r / 4
+    // (+ 4, + 32, + 36), features 16 nb + 4 (r % 4) .. + 3
+    const uint8_t* rd = Bs + ((r & 12) + kg) * RS + 8 * (r & 3);
+
+    DeepRec m[MR];
+    DeepFrag fr[DEP];
+    u32x4 g[DEP][NL];
+    // prologue: records of steps 0 .. 2 DEP - 1, the first DEP staged, then the
+    // first DEP steps' fragments and gathers
+#pragma unroll
+    for (int j = 0; j < MR; ++j) deep_rec<KIND>(p, first, end, n, j, lane, m[j]);
+#pragma unroll
+    for (int j = 0; j < DEP; ++j) deep_stage<KIND>(m[j], first, end, n, j, Ms + MS * j, lane);
+    if constexpr (KIND != 2) {
+#pragma unroll
+        for (int j = 0; j < DEP; ++j) deep_frag<KIND>(p, first, end, n, j, lane, fr[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < DEP; ++j) deep_gather<PW, R>(g[j], Ms + MS * j, rsB, ldb2, fp, fw, lane);
+
+    auto body = [&](int s, auto par_tag) {
+        constexpr int PAR = decltype(par_tag)::value;
+        constexpr int GB = PAR % DEP;
+        constexpr int NXT = (PAR + DEP) % MR;              // ring slot of step s + DEP
+        __builtin_amdgcn_sched_barrier(0);
+        deep_park<PW, R>(g[GB], Bs, lane);
+        deep_stage<KIND>(m[NXT], first, end, n, s + DEP, Ms + MS * NXT, lane);
+        deep_rec<KIND>(p, first, end, n, s + 2 * DEP, lane, m[(PAR + 2 * DEP) % MR]);
+        deep_gather<PW, R>(g[GB], Ms + MS * NXT, rsB, ldb2, fp, fw, lane);
+        X8 a;
+        int idx = 0x4444;
+        if constexpr (KIND == 2) {
+            // one-hot fragment: lane (kg, r) takes entries 8 kg .. 8 kg + 7 of the step
+            const uint32_t* hw = (const uint32_t*)(Ms + MS * (PAR % MR) + 64 * kg + 4);
+            u16x8 abits = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const uint32_t h = hw[2 * q];
+                abits[q] = ((h >> 16) & 0xffu) == (uint32_t)r ? (uint16_t)(h & 0xffffu) : (uint16_t)0;
+            }
+            a = __builtin_bit_cast(X8, abits);
+        } else {
+            const DeepFrag& cf = fr[GB];
+            a = __builtin_bit_cast(X8, cf.ok ? cf.a : u32x4{0u, 0u, 0u, 0u});
+            if constexpr (KIND == 1) idx = cf.ok ? cf.idx : 0x4444;     // (0,1) in every group: valid, times zeros
+        }
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const uint16_t* ap = (const uint16_t*)(rd + 32 * nb);
+            const s16x4 b0 = lds_tr16(ap), b1 = lds_tr16(ap + 8 * RS);          // logical rows +4: physical +16
+            if constexpr (KIND == 1) {
+                const s16x4 b2 = lds_tr16(ap + 16 * RS), b3 = lds_tr16(ap + 24 * RS);   // logical +32, +36
+                const s16x16 bb = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3],
+                                   b2[0], b2[1], b2[2], b2[3], b3[0], b3[1], b3[2], b3[3]};
+                acc[nb] = mma_sparse<DT>(a, __builtin_bit_cast(X16, bb), acc[nb], idx);
+            } else {
+                const s16x8 bb = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+                acc[nb] = mma_dense<DT>(a, __builtin_bit_cast(X8, bb), acc[nb]);
+            }
+        }
+        // keep the transposed reads next to the matrix instruction that takes them
+        // (hoisted all at once they hold 32 more registers than the step can spare)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            __builtin_amdgcn_sched_group_barrier(0x100, KIND == 1 ? 4 : 2, 0);   // DS reads
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                    // MFMA
+        }
+        if constexpr (KIND != 2) deep_frag<KIND>(p, first, end, n, s + DEP, lane, fr[GB]);
+    };
+    for (int s = 0; s < n; s += U) {
+        body(s, std::integral_constant<int, 0>{});
+        if (s + 1 >= n) break;
+        body(s + 1, std::integral_constant<int, 1>{});
+        if constexpr (U >= 4) {
+            if (s + 2 >= n) break;
+            body(s + 2, std::integral_constant<int, 2>{});
+            if (s + 3 >= n) break;
+            body(s + 3, std::integral_constant<int, 3>{});
+        }
+        if constexpr (U == 8) {
+            if (s + 4 >= n) break;
+            body(s + 4, std::integral_constant<int, 4>{});
+            if (s + 5 >= n) break;
+            body(s + 5, std::integral_constant<int, 5>{});
+            if (s + 6 >= n) break;
+            body(s + 6, std::integral_constant<int, 6>{});
+            if (s + 7 >= n) break;
+            body(s + 7, std::integral_constant<int, 7>{});
+        }
+    }
+}
+
+// one work item: passes of PW features; per pass the routes' steps, then the
+// 16-row tile written (full lines through LDS, or atomically for split windows)
+template <int PW, typename DT>
+__device__ __forceinline__ void deep_item(const PlanDev& p, int64_t it, const uint16_t* B, int64_t ldb,
+                                          int K, float* __restrict__ C, int64_t ldc, uint32_t mask, uint8_t* Bs,
+                                          uint8_t* Ms, int lane) {
+    constexpr int NB = PW / 16;
+    const int4 item = p.titems[it];
+    const int64_t w = item.x;
+    const bool split = item.y != 0;
+    int32_t s0 = 0, s1 = 0, t0 = 0, t1 = 0, e0 = 0, e1 = 0;
+    if (item.y == 0) {
+        s0 = p.tc_sub_off[w], s1 = p.tc_sub_off[w + 1];
+        t0 = p.sp_tile_off[w], t1 = p.sp_tile_off[w + 1];
+        if (p.res_mfma) e0 = p.res_rowptr[w * 16], e1 = p.res_rowptr[w * 16 + 16];
+    } else if (item.y == 1) {
+        s0 = item.z, s1 = item.w;
+    } else if (item.y == 2) {
+        t0 = item.z, t1 = item.w;
+    } else {
+        e0 = item.z, e1 = item.w;
+    }
+    if (!(mask & TRSPMM_ROUTE_TC)) s1 = s0;
+    if (!(mask & TRSPMM_ROUTE_SPTC)) t1 = t0;
+    if (!(mask & TRSPMM_ROUTE_RESIDUAL)) e1 = e0;
+    const int r = lane & 15, kg = lane >> 4;
+    const int nts = (s1 - s0 + 3) / 4, ntt = (t1 - t0 + 3) / 4, nte = (e1 - e0 + 31) / 32;
+    const int64_t row0 = w * 16;
+    const int rows_out = p.m_out - row0 < 16 ? (int)(p.m_out - row0) : 16;
+    float* Crow = C + row0 * ldc;                              // the tile's first row (uniform)
+    const __amdgpu_buffer_rsrc_t rsB =
+        __builtin_amdgcn_make_buffer_rsrc((void*)B, (short)0, (int)(uint32_t)(2 * p.ncols * ldb), (int)kBufferWord3);
+    const uint32_t ldb2 = 2u * (uint32_t)ldb;
+    for (int fp = 0; fp < K; fp += PW) {
+        f32x4 acc[NB];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        // the item's bounds, re-read in each pass: keeps the compiler from hoisting the
+        // steps' address arithmetic out of the pass loop into long-lived registers
+        int32_t bs0 = s0, bs1 = s1, bt0 = t0, bt1 = t1, be0 = e0, be1 = e1;
+        asm volatile("" : "+s"(bs0), "+s"(bs1), "+s"(bt0), "+s"(bt1), "+s"(be0), "+s"(be1));
+        if (nts > 0) run_deep<0, PW, DT>(p, rsB, ldb2, K, fp, bs0, bs1, nts, Bs, Ms, lane, acc);
+        if (ntt > 0) run_deep<1, PW, DT>(p, rsB, ldb2, K, fp, bt0, bt1, ntt, Bs, Ms, lane, acc);
+        if (nte > 0) run_deep<2, PW, DT>(p, rsB, ldb2, K, fp, be0, be1, nte, Bs, Ms, lane, acc);
+        const int width = K - fp < PW ? K - fp : PW;          // features of this pass (multiple of 8)
+        // the output stride, re-read here for the same reason as the bounds above
+        uint32_t ldc32 = (uint32_t)ldc;
+        asm volatile("" : "+s"(ldc32));
+        // accumulator element i of lane (kg, n) = C[16w + 4kg + i, fp + 16nb + n]
+        if (!split) {
+            float* T = reinterpret_cast<float*>(Bs);           // [16][PW] fp32, the staging space is free
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) T[(kg * 4 + i) * PW + nb * 16 + r] = acc[nb][i];
+            // store k of lane l writes 16 bytes of row prow + RPS k, piece l % (PW / 4):
+            // one per-lane base and uniform strides (addresses kept out of the loops above)
+            constexpr int PPR = PW / 4, RPS = 64 / PPR;
+            const int prow = lane / PPR, piece = lane % PPR;
+            const bool pok = piece < width / 4;
+            const float* tsrc = T + prow * PW + 4 * piece;
+            float* cdst = Crow + (uint32_t)prow * ldc32 + (uint32_t)(fp + 4 * piece);
+            const uint32_t cstep = (uint32_t)RPS * ldc32;
+#pragma unroll
+            for (int k = 0; k < 16 / RPS; ++k) {
+                if (pok & (prow + RPS * k < rows_out)) {
+                    const f32x4 x = *(const f32x4*)(tsrc + RPS * k * PW);
+                    __builtin_nontemporal_store(x, (f32x4*)(cdst + (uint32_t)k * cstep));
+                }
+            }
+        } else {
+            float* adst = Crow + (uint32_t)(4 * kg) * ldc32 + (uint32_t)(fp + r);
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                const int f = nb * 16 + r;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if ((kg * 4 + i < rows_out) & (f < width) & (acc[nb][i] != 0.f))
+                        unsafeAtomicAdd(adst + (uint32_t)i * ldc32 + 16 * nb, acc[nb][i]);
+                }
+            }
+        }
+    }
+}
+
+// one workgroup (one wave) per work item, in the plan's item order (see
+// k_tile_routes); 19 KB of LDS per wave at PW = 128, two waves per SIMD
+template <int PW, typename DT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_routes(PlanDev p, const uint16_t* B, int64_t ldb, int K, float* __restrict__ C, int64_t ldc,
+              uint32_t mask, int64_t grid) {
+    __shared__ __align__(16) uint8_t Bs[64 * DeepCfg<PW>::RS];
+    __shared__ __align__(16) uint8_t Ms[4 * 512];          // record ring: up to 4 slots of 64 x 8 bytes
+    static_assert(16 * PW * 4 <= 64 * DeepCfg<PW>::RS, "the output tile must fit the staging space");
+    const int64_t it = p.item_order == 1 ? xcd_window(blockIdx.x, grid) : (int64_t)blockIdx.x;
+    if (it < p.n_titems) deep_item<PW, DT>(p, it, B, ldb, K, C, ldc, mask, Bs, Ms, threadIdx.x);
+}
+
 // ============================================================ residual route
 //
 // One wave64 per work item.  The item's entries (the 16 rows of a window, or
@@ -734,7 +1124,15 @@ int launch_typed(const PlanDev& p, const void* B, int64_t ldb, int64_t K, float*
             grid = std::min<int64_t>(xcd_grid(p.n_titems), std::max<int64_t>(kXcds, resident / kXcds * kXcds));
         }
         TRSPMM_CHECK(grid <= 0x7fffffff, "too many work items");
-        if (p.slab == 128)
+        // the deep kernel addresses B as a raw buffer of fewer than 2^31 bytes
+        if (p.tile_kernel == 0 && p.item_order != 2 && p.ncols * ldb < ((int64_t)1 << 30)) {
+            if (K <= 64)
+                hipLaunchKernelGGL((k_routes<64, DT>), dim3((unsigned)grid), dim3(64), 0, s, p, (const uint16_t*)B, ldb,
+                                   (int)K, C, ldc, mask, grid);
+            else
+                hipLaunchKernelGGL((k_routes<128, DT>), dim3((unsigned)grid), dim3(64), 0, s, p, (const uint16_t*)B,
+                                   ldb, (int)K, C, ldc, mask, grid);
+        } else if (p.slab == 128)
             hipLaunchKernelGGL((k_tile_routes<128, DT>), dim3((unsigned)grid), dim3(64), 0, s, p, (const uint16_t*)B,
                                ldb, (int)K, C, ldc, mask, grid);
         else

@@ -73,7 +73,8 @@ class MetaView(ctypes.Structure):
 class PlanOpts(ctypes.Structure):
     _fields_ = [("residual_engine", ctypes.c_int32), ("slab", ctypes.c_int32), ("tile_cap", ctypes.c_int32),
                 ("tc_cap", ctypes.c_int32), ("res_cap", ctypes.c_int32), ("item_order", ctypes.c_int32),
-                ("k_hint", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("k_hint", ctypes.c_int32), ("res_order", ctypes.c_int32),
+                ("tile_kernel", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class MatchStats(ctypes.Structure):

@@ -165,7 +165,7 @@ class Plan:
     @staticmethod
     def from_meta(meta: RoutedMeta, m_out: int, ncols: int, dtype=torch.float16, engine: str | None = None,
                   slab: int | None = None, tile_cap: int = 0, tc_cap: int = 0, res_cap: int = 0,
-                  item_order: int = 0, k_hint: int = 0) -> "Plan":
+                  item_order: int = 0, k_hint: int = 0, res_order: int = 0, tile_kernel: int = 0) -> "Plan":
         """engine: None (default), "valu" or "mfma" residual engine; slab: None, 64 or 128
         features per pass of the tile kernel; *_cap: split a window into work items of at
         most this many 2:4 tiles / TC sub-blocks / residual entries (0 = default);
@@ -174,12 +174,17 @@ class Plan:
         the heaviest-first items from the plan's queue counters (launches of such a plan must
         not overlap on concurrent streams); 3 = as 0 in window order; k_hint: the feature width
         K the plan will mostly run at (0 = not known): a residual cap left at 0 is then sized
-        for it, 4096 * 128 / max(K, 64) entries (any K still runs)."""
+        for it, 4096 * 128 / max(K, 64) entries (any K still runs); res_order: 0 = the
+        matrix-core residual engine's entries of a window by column (the default), 1 = in
+        row order; tile_kernel: 0 = the deep-pipelined tile kernel (default), 1 = the earlier
+        one-sub-step-ahead kernel (persistent plans always run it)."""
         opts = _lib.PlanOpts()
         opts.residual_engine = {None: _lib.ENGINE_DEFAULT, "valu": _lib.ENGINE_VALU, "mfma": _lib.ENGINE_MFMA}[engine]
         opts.slab = int(slab or 0)
         opts.tile_cap, opts.tc_cap, opts.res_cap, opts.item_order = int(tile_cap), int(tc_cap), int(res_cap), int(item_order)
         opts.k_hint = int(k_hint)
+        opts.res_order = int(res_order)
+        opts.tile_kernel = int(tile_kernel)
         h = ctypes.c_void_p()
         check(lib().trspmm_plan_from_meta_ex(meta.handle, int(m_out), int(ncols), dtype_code(dtype), ctypes.byref(opts),
                                              _lib.current_stream(), ctypes.byref(h)), "trspmm_plan_from_meta_ex")
