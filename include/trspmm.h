@@ -30,7 +30,8 @@
  * Entry points with no reference counterpart (the reference has no backward
  * and no device router):
  *   trspmm_csr_transpose, trspmm_sddmm, trspmm_bwd_plan_*, trspmm_spmm_t_sddmm
- *   (backward pass), trspmm_plan_*_ex (plan options), trspmm_memcpy,
+ *   (backward pass), trspmm_plan_*_ex (plan options), trspmm_plan_set_values
+ *   (value refresh), trspmm_memcpy,
  *   trspmm_host_sort_order (diagnostics).
  */
 #ifndef TRSPMM_H
@@ -80,8 +81,14 @@ typedef struct trspmm_route_cfg {
     int32_t unit_min_nnz;   /* LIBRA_SPTC_UNIT_MIN_NNZ (2) */
     int32_t mode;           /* TRSPMM_MODE_* */
     int32_t sptc_min_nnz;   /* TUNED: a 2:4 group with fewer nonzeros goes to the residual route */
-    int32_t reserved;
+    int32_t flags;          /* TRSPMM_ROUTE_* bits (0 = none) */
 } trspmm_route_cfg;
+
+/* trspmm_route_cfg.flags: also record the CSR edge behind every value slot
+ * (no reference counterpart).  Plans built from such a router result carry
+ * the maps and accept new values with trspmm_plan_set_values; the 29 arrays
+ * are unchanged. */
+enum { TRSPMM_ROUTE_EDGES = 1 };
 
 typedef struct trspmm_meta trspmm_meta;
 
@@ -185,6 +192,19 @@ int trspmm_plan_create_packed(const int32_t* s_tile_column, const int32_t* s_pac
                               int64_t n_tiles, int64_t n_windows, int32_t window, int64_t m_out,
                               int64_t ncols, void* stream, trspmm_plan** out);
 void trspmm_plan_free(trspmm_plan* plan);
+/* New values for a plan built from a router result with TRSPMM_ROUTE_EDGES
+ * (no reference counterpart; the reference's op takes its values on every
+ * call, mGCN_online.cpp:760-800, which this makes possible without routing
+ * again).  The value of the routed CSR's edge e becomes
+ * vals[index ? index[e] : e] (fp32, device memory), converted to the plan's
+ * operand type exactly as at plan creation; index (device, nnz entries, may
+ * be null) serves plans of A^T fed with A's values (index = the perm of
+ * trspmm_csr_transpose).  nnz = the routed CSR's nonzeros; n_vals = entries of
+ * vals (reads outside [0, n_vals) give 0).  Stream-ordered, no allocation;
+ * must not overlap a multiply on the same plan.  TRSPMM_EUNSUPPORTED for a
+ * plan without edge maps. */
+int trspmm_plan_set_values(trspmm_plan* plan, const float* vals, int64_t n_vals, const int32_t* index, int64_t nnz,
+                           void* stream);
 /* counts: [windows, tc sub-blocks, tc nnz, 2:4 tiles, residual work items, residual nnz] */
 int trspmm_plan_stats(const trspmm_plan* plan, int64_t* counts6);
 
@@ -219,13 +239,15 @@ int trspmm_csr_transpose(const int32_t* d_rowptr, const int32_t* d_col, const fl
                          int32_t* perm, void* stream);
 
 /* dvals[e] = sum_f X[row_e, f] * Y[col_e, f] for every edge e of the CSR
- * (rowptr[rows + 1], col[nnz]; device memory): the gradient of
+ * (rowptr[rows + 1], col[nnz]; device memory; Y has ncols rows, and an edge
+ * whose column lies outside [0, ncols) gets 0 without Y being read): the gradient of
  * sum(dY * (A·B)) with respect to A's values, X = dY [rows, K] (ldx),
  * Y = B [ncols, K] (ldy), both `dtype` (TRSPMM_F16 / TRSPMM_BF16), fp32
  * accumulation, fp32 dvals in A's edge order.  K, ldx, ldy multiples of 8;
  * X and Y 16-byte aligned.  Stream-ordered. */
-int trspmm_sddmm(const int32_t* d_rowptr, int64_t rows, const int32_t* d_col, int64_t nnz, const void* X,
-                 int64_t ldx, const void* Y, int64_t ldy, int64_t K, int dtype, float* dvals, void* stream);
+int trspmm_sddmm(const int32_t* d_rowptr, int64_t rows, const int32_t* d_col, int64_t nnz, int64_t ncols,
+                 const void* X, int64_t ldx, const void* Y, int64_t ldy, int64_t K, int dtype, float* dvals,
+                 void* stream);
 
 /* Fused backward over the rows of A^T: one pass computes both
  *   dB[N, K]  = A^T · dY          (fp32, ldd: every row written)
@@ -237,10 +259,16 @@ int trspmm_sddmm(const int32_t* d_rowptr, int64_t rows, const int32_t* d_col, in
  * one plan must not overlap.  K multiple of 8, at most 512; dY [ncols, K]
  * (lddy) and B [rows, K] (ldb) in `dtype`, 16-byte aligned. */
 typedef struct trspmm_bwd_plan trspmm_bwd_plan;
+/* Rejects (TRSPMM_EINVAL) column indices outside [0, ncols) and a perm that
+ * is not a permutation of [0, nnz). */
 int trspmm_bwd_plan_create(const int32_t* t_rowptr, const int32_t* t_col, const float* t_val,
                            const int32_t* perm, int64_t rows, int64_t nnz, int64_t ncols, int dtype,
                            void* stream, trspmm_bwd_plan** out);
 void trspmm_bwd_plan_free(trspmm_bwd_plan* plan);
+/* New values of A (vals [nnz], fp32, A's edge order, device memory) for a
+ * fused-backward plan; stream-ordered, no allocation, must not overlap a call
+ * on the same plan.  No reference counterpart. */
+int trspmm_bwd_plan_set_values(trspmm_bwd_plan* plan, const float* vals, int64_t nnz, void* stream);
 int trspmm_spmm_t_sddmm(const trspmm_bwd_plan* plan, const void* dY, int64_t lddy, const void* B,
                         int64_t ldb, int64_t K, float* dB, int64_t lddb, float* dvals, void* stream);
 

@@ -17,11 +17,12 @@ struct BwdPlan {
 
 int bwd_plan_build(const int32_t* t_rowptr, const int32_t* t_col, const float* t_val, const int32_t* perm,
                    int64_t rows, int64_t nnz, int64_t ncols, int dtype, hipStream_t s, BwdPlan& P);
+int bwd_plan_set_values(BwdPlan& P, const float* vals, int64_t nnz, hipStream_t s);
 int launch_bwd(const BwdPlan& P, const void* dY, int64_t lddy, const void* B, int64_t ldb, int64_t K, float* dB,
                int64_t lddb, float* dvals, hipStream_t s);
 
-int launch_sddmm(const int32_t* rowptr, int64_t rows, const int32_t* col, int64_t nnz, const void* X, int64_t ldx,
-                 const void* Y, int64_t ldy, int64_t K, int dtype, float* out, hipStream_t s);
+int launch_sddmm(const int32_t* rowptr, int64_t rows, const int32_t* col, int64_t nnz, int64_t ncols, const void* X,
+                 int64_t ldx, const void* Y, int64_t ldy, int64_t K, int dtype, float* out, hipStream_t s);
 
 }  // namespace trspmm
 

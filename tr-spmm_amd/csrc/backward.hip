@@ -150,7 +150,7 @@ __global__ __launch_bounds__(SDDMM_BLOCK) void k_sddmm(const int32_t* __restrict
                                                        const int32_t* __restrict__ col, int64_t nnz,
                                                        const uint16_t* __restrict__ X, int64_t ldx,
                                                        const uint16_t* __restrict__ Y, int64_t ldy, int K,
-                                                       float* __restrict__ out) {
+                                                       int64_t ncols, float* __restrict__ out) {
     constexpr int U = SDDMM_U;
     const int lane = threadIdx.x & 63;
     const int j = lane % LPR;
@@ -164,10 +164,13 @@ __global__ __launch_bounds__(SDDMM_BLOCK) void k_sddmm(const int32_t* __restrict
 
     for (int64_t eb = g0; eb < g1; eb += U) {
         int32_t c[U], r[U];
+        bool ok[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int64_t e = eb + u < elast ? eb + u : elast;   // clamped: unconditional loads
-            c[u] = col[e];
+            const int32_t cu = col[e];
+            ok[u] = cu >= 0 && (int64_t)cu < ncols;      // out-of-range column: reads row 0, gives 0
+            c[u] = ok[u] ? cu : 0;
             if (eb + u < g1) {
                 while (e >= row_end) {
                     ++row;
@@ -189,6 +192,8 @@ __global__ __launch_bounds__(SDDMM_BLOCK) void k_sddmm(const int32_t* __restrict
 #pragma unroll
             for (int u = 0; u < U; ++u) part[u] += dot8<DT>(xv[u], yv[u]);
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u) part[u] = ok[u] ? part[u] : 0.f;
         using R = EdgeReduce<LPR>;
         const int base = R::run(part, j);
         if ((j & R::LOWMASK) == 0) {
@@ -202,8 +207,8 @@ __global__ __launch_bounds__(SDDMM_BLOCK) void k_sddmm(const int32_t* __restrict
 }
 
 template <typename DT>
-int launch_sddmm_typed(const int32_t* rowptr, int64_t rows, const int32_t* col, int64_t nnz, const void* X,
-                       int64_t ldx, const void* Y, int64_t ldy, int64_t K, float* out, hipStream_t s) {
+int launch_sddmm_typed(const int32_t* rowptr, int64_t rows, const int32_t* col, int64_t nnz, int64_t ncols,
+                       const void* X, int64_t ldx, const void* Y, int64_t ldy, int64_t K, float* out, hipStream_t s) {
     const int lpr = K >= 128 ? 16 : (K >= 64 ? 8 : 4);
     const int64_t groups = (nnz + SDDMM_EPG - 1) / SDDMM_EPG;
     const int64_t threads = groups * lpr;
@@ -212,13 +217,13 @@ int launch_sddmm_typed(const int32_t* rowptr, int64_t rows, const int32_t* col, 
     const auto* y = (const uint16_t*)Y;
     if (lpr == 16)
         hipLaunchKernelGGL((k_sddmm<16, DT>), grid, dim3(SDDMM_BLOCK), 0, s, rowptr, rows, col, nnz, x, ldx, y, ldy,
-                           (int)K, out);
+                           (int)K, ncols, out);
     else if (lpr == 8)
         hipLaunchKernelGGL((k_sddmm<8, DT>), grid, dim3(SDDMM_BLOCK), 0, s, rowptr, rows, col, nnz, x, ldx, y, ldy,
-                           (int)K, out);
+                           (int)K, ncols, out);
     else
         hipLaunchKernelGGL((k_sddmm<4, DT>), grid, dim3(SDDMM_BLOCK), 0, s, rowptr, rows, col, nnz, x, ldx, y, ldy,
-                           (int)K, out);
+                           (int)K, ncols, out);
     TRSPMM_HIP(hipGetLastError());
     return TRSPMM_OK;
 }
@@ -431,6 +436,20 @@ __global__ void k_to16(const float* __restrict__ src, int64_t n, int dtype, uint
     if (i < n) dst[i] = dtype == TRSPMM_BF16 ? f32_to_bf16_bits(src[i]) : f32_to_f16_rne(src[i]);
 }
 
+// dst[inv[e]] = src[e] converted as k_to16
+__global__ void k_scatter16(const float* __restrict__ src, const int32_t* __restrict__ inv, int64_t n, int dtype,
+                            uint16_t* __restrict__ dst) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) dst[inv[e]] = dtype == TRSPMM_BF16 ? f32_to_bf16_bits(src[e]) : f32_to_f16_rne(src[e]);
+}
+
+// err |= 1 unless inv[perm[i]] == i for every i (perm entries already in range)
+__global__ void k_check_inverse(const int32_t* __restrict__ perm, const int32_t* __restrict__ inv, int64_t n,
+                                int32_t* __restrict__ err) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && inv[perm[i]] != (int32_t)i) atomicOr(err, 1);
+}
+
 template <typename DT>
 int launch_bwd_typed(const BwdPlan& P, const void* dY, int64_t lddy, const void* B, int64_t ldb, int64_t K,
                      float* dB, int64_t lddb, float* dvals, hipStream_t s) {
@@ -512,18 +531,23 @@ int csr_transpose(const int32_t* rowptr, const int32_t* col, const float* val, i
     return TRSPMM_OK;
 }
 
-int launch_sddmm(const int32_t* rowptr, int64_t rows, const int32_t* col, int64_t nnz, const void* X, int64_t ldx,
-                 const void* Y, int64_t ldy, int64_t K, int dtype, float* out, hipStream_t s) {
-    TRSPMM_CHECK(rows >= 0 && nnz >= 0, "sddmm: negative size");
+int launch_sddmm(const int32_t* rowptr, int64_t rows, const int32_t* col, int64_t nnz, int64_t ncols, const void* X,
+                 int64_t ldx, const void* Y, int64_t ldy, int64_t K, int dtype, float* out, hipStream_t s) {
+    TRSPMM_CHECK(rows >= 0 && nnz >= 0 && ncols >= 0, "sddmm: negative size");
     TRSPMM_CHECK(K > 0 && K % 8 == 0 && ldx % 8 == 0 && ldy % 8 == 0 && ldx >= K && ldy >= K,
                  "sddmm: K and the row strides must be multiples of 8 with stride >= K");
     TRSPMM_CHECK(((uintptr_t)X % 16) == 0 && ((uintptr_t)Y % 16) == 0, "sddmm: X and Y must be 16-byte aligned");
     TRSPMM_CHECK(dtype == TRSPMM_F16 || dtype == TRSPMM_BF16, "sddmm: dtype must be TRSPMM_F16 or TRSPMM_BF16");
     TRSPMM_CHECK(nnz < (int64_t)1 << 31, "sddmm: nnz must fit int32");
     if (nnz == 0) return TRSPMM_OK;
-    TRSPMM_CHECK(rowptr && col && X && Y && out, "sddmm: null pointer");
-    if (dtype == TRSPMM_F16) return launch_sddmm_typed<F16>(rowptr, rows, col, nnz, X, ldx, Y, ldy, K, out, s);
-    return launch_sddmm_typed<BF16>(rowptr, rows, col, nnz, X, ldx, Y, ldy, K, out, s);
+    TRSPMM_CHECK(rowptr && col && out, "sddmm: null pointer");
+    if (ncols == 0) {                               // no column is in range: every edge gets 0
+        TRSPMM_HIP(hipMemsetAsync(out, 0, 4 * nnz, s));
+        return TRSPMM_OK;
+    }
+    TRSPMM_CHECK(X && Y, "sddmm: null pointer");
+    if (dtype == TRSPMM_F16) return launch_sddmm_typed<F16>(rowptr, rows, col, nnz, ncols, X, ldx, Y, ldy, K, out, s);
+    return launch_sddmm_typed<BF16>(rowptr, rows, col, nnz, ncols, X, ldx, Y, ldy, K, out, s);
 }
 
 int bwd_plan_build(const int32_t* t_rowptr, const int32_t* t_col, const float* t_val, const int32_t* perm,
@@ -599,13 +623,45 @@ int bwd_plan_build(const int32_t* t_rowptr, const int32_t* t_col, const float* t
     if (!zero.empty())
         TRSPMM_HIP(hipMemcpyAsync(P.zero_rows.p, zero.data(), 4 * zero.size(), hipMemcpyHostToDevice, s));
     if (nnz > 0) {
+        // columns of A^T index dY's rows: all in [0, ncols); perm must be a
+        // permutation of [0, nnz) (entries in range, and the inverse built from
+        // it maps every entry back to its own position)
+        DevBuf err;
+        if ((rc = err.alloc(8))) return rc;
+        TRSPMM_HIP(hipMemsetAsync(err.p, 0, 8, s));
+        hipLaunchKernelGGL(k_check_cols, dim3(nblk(nnz)), dim3(256), 0, s, t_col, nnz, ncols, err.as<int32_t>());
+        TRSPMM_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_check_cols, dim3(nblk(nnz)), dim3(256), 0, s, perm, nnz, nnz, err.as<int32_t>() + 1);
+        TRSPMM_HIP(hipGetLastError());
+        int32_t bad[2] = {0, 0};
+        TRSPMM_HIP(hipMemcpyAsync(bad, err.p, 8, hipMemcpyDeviceToHost, s));
+        TRSPMM_HIP(hipStreamSynchronize(s));
+        TRSPMM_CHECK(bad[0] == 0, "bwd plan: a column index lies outside [0, ncols)");
+        TRSPMM_CHECK(bad[1] == 0, "bwd plan: perm is not a permutation of [0, nnz)");
         TRSPMM_HIP(hipMemcpyAsync(P.col.p, t_col, 4 * nnz, hipMemcpyDeviceToDevice, s));
         hipLaunchKernelGGL(k_to16, dim3(nblk(nnz)), dim3(256), 0, s, t_val, nnz, dtype, P.val.as<uint16_t>());
         TRSPMM_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_invert, dim3(nblk(nnz)), dim3(256), 0, s, perm, nnz, P.invperm.as<int32_t>());
         TRSPMM_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_check_inverse, dim3(nblk(nnz)), dim3(256), 0, s, perm, P.invperm.as<int32_t>(), nnz,
+                           err.as<int32_t>());
+        TRSPMM_HIP(hipGetLastError());
+        TRSPMM_HIP(hipMemcpyAsync(bad, err.p, 4, hipMemcpyDeviceToHost, s));
+        TRSPMM_HIP(hipStreamSynchronize(s));
+        TRSPMM_CHECK(bad[0] == 0, "bwd plan: perm is not a permutation of [0, nnz)");
     }
     TRSPMM_HIP(hipStreamSynchronize(s));               // host vectors go out of scope
+    return TRSPMM_OK;
+}
+
+// new values of A (A's edge order): val[invperm[e]] = vals[e] in the plan's type
+int bwd_plan_set_values(BwdPlan& P, const float* vals, int64_t nnz, hipStream_t s) {
+    TRSPMM_CHECK(nnz == P.nnz, "bwd plan set_values: nnz must be the plan's nonzero count");
+    if (nnz == 0) return TRSPMM_OK;
+    TRSPMM_CHECK(vals != nullptr, "bwd plan set_values: vals must not be null");
+    hipLaunchKernelGGL(k_scatter16, dim3(nblk(nnz)), dim3(256), 0, s, vals, P.invperm.as<int32_t>(), nnz, P.dtype,
+                       P.val.as<uint16_t>());
+    TRSPMM_HIP(hipGetLastError());
     return TRSPMM_OK;
 }
 

@@ -150,7 +150,30 @@ int trspmm_plan_from_meta_ex(const trspmm_meta* meta, int64_t m_out, int64_t nco
     v.t_value_is_f32 = 1;
     v.c_value_is_f32 = 1;
     v.window = M.window;
-    return trspmm_plan_create_ex(&v, m_out, ncols, dtype, opts, stream, out);
+    if (!M.has_edges) return trspmm_plan_create_ex(&v, m_out, ncols, dtype, opts, stream, out);
+    TRSPMM_NOTNULL(out, "out");
+    *out = nullptr;
+    trspmm_plan* p = new (std::nothrow) trspmm_plan();
+    if (!p) {
+        set_error("out of host memory");
+        return TRSPMM_ENOMEM;
+    }
+    if (opts) p->plan.opts = *opts;
+    const EdgeMaps em{(const int32_t*)M.edge_data[0], (const int32_t*)M.edge_data[1], (const int32_t*)M.edge_data[2],
+                      (const int32_t*)M.edge_data[3], M.nnz};
+    const int rc = build_plan(v, m_out, ncols, dtype, (hipStream_t)stream, p->plan, &em);
+    if (rc != TRSPMM_OK) {
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return TRSPMM_OK;
+}
+
+int trspmm_plan_set_values(trspmm_plan* plan, const float* vals, int64_t n_vals, const int32_t* index, int64_t nnz,
+                           void* stream) {
+    TRSPMM_NOTNULL(plan, "plan");
+    return plan_set_values(plan->plan, vals, n_vals, index, nnz, (hipStream_t)stream);
 }
 
 int trspmm_plan_create_packed(const int32_t* s_tile_column, const int32_t* s_packed_a, const int32_t* s_packed_meta,
@@ -244,9 +267,10 @@ int trspmm_csr_transpose(const int32_t* d_rowptr, const int32_t* d_col, const fl
     return csr_transpose(d_rowptr, d_col, d_val, rows, nnz, ncols, t_rowptr, t_col, t_val, perm, (hipStream_t)stream);
 }
 
-int trspmm_sddmm(const int32_t* d_rowptr, int64_t rows, const int32_t* d_col, int64_t nnz, const void* X,
-                 int64_t ldx, const void* Y, int64_t ldy, int64_t K, int dtype, float* dvals, void* stream) {
-    return launch_sddmm(d_rowptr, rows, d_col, nnz, X, ldx, Y, ldy, K, dtype, dvals, (hipStream_t)stream);
+int trspmm_sddmm(const int32_t* d_rowptr, int64_t rows, const int32_t* d_col, int64_t nnz, int64_t ncols,
+                 const void* X, int64_t ldx, const void* Y, int64_t ldy, int64_t K, int dtype, float* dvals,
+                 void* stream) {
+    return launch_sddmm(d_rowptr, rows, d_col, nnz, ncols, X, ldx, Y, ldy, K, dtype, dvals, (hipStream_t)stream);
 }
 
 int trspmm_bwd_plan_create(const int32_t* t_rowptr, const int32_t* t_col, const float* t_val, const int32_t* perm,
@@ -268,6 +292,11 @@ int trspmm_bwd_plan_create(const int32_t* t_rowptr, const int32_t* t_col, const 
 }
 
 void trspmm_bwd_plan_free(trspmm_bwd_plan* plan) { delete plan; }
+
+int trspmm_bwd_plan_set_values(trspmm_bwd_plan* plan, const float* vals, int64_t nnz, void* stream) {
+    TRSPMM_NOTNULL(plan, "plan");
+    return bwd_plan_set_values(plan->plan, vals, nnz, (hipStream_t)stream);
+}
 
 int trspmm_spmm_t_sddmm(const trspmm_bwd_plan* plan, const void* dY, int64_t lddy, const void* B, int64_t ldb,
                         int64_t K, float* dB, int64_t lddb, float* dvals, void* stream) {

@@ -5,19 +5,36 @@
 
 namespace trspmm {
 
+// The routed CSR edge behind each value of the router's t_value, c_value,
+// c_value_short and s_value arrays (-1 = an empty slot); nnz = the CSR's nonzeros.
+struct EdgeMaps {
+    const int32_t *t_edge, *c_edge, *cs_edge, *s_edge;
+    int64_t nnz;
+};
+
 struct Plan {
     PlanDev dev;
     DevBuf tc_sub_off, tc_cols, tc_bits, tc_valoff, tc_vals, tc_dense;
     DevBuf sp_tile_off, sp_cols, sp_a, sp_idx;
     DevBuf res_rowptr, res_cols, res_vals, res_rec;
     DevBuf titems, ritems, split_windows, queue;
+    // edge maps (plans built with EdgeMaps): the CSR edge behind tc_vals[i],
+    // the (low, high) halves of sp_a[i], res_vals[i] and res_rec[i]
+    // (rec_edge empty when the records are in res_vals order)
+    DevBuf tc_edge, sp_edge, res_edge, rec_edge;
+    bool has_edges = false;
+    int64_t src_nnz = 0;
     SplitCaps caps;
     trspmm_plan_opts opts{};   // requested engine / slab (0 = default)
     int finalize();
     int build_items(hipStream_t s);   // work-item list from the per-window ranges
 };
 
-int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtype, hipStream_t s, Plan& P);
+int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtype, hipStream_t s, Plan& P,
+               const EdgeMaps* edges = nullptr);
+// new values for a plan with edge maps: value of CSR edge e = vals[index ? index[e] : e]
+// (0 outside [0, n_vals)), converted to the plan's type as at build time
+int plan_set_values(Plan& P, const float* vals, int64_t n_vals, const int32_t* index, int64_t nnz, hipStream_t s);
 int build_plan_packed(const int32_t* s_tile_column, const int32_t* s_packed_a, const int32_t* s_packed_meta,
                       const int32_t* s_window_tile_offset, int64_t T, int64_t W, int window, int64_t m_out,
                       int64_t ncols, hipStream_t s, Plan& P);

@@ -81,7 +81,8 @@ __global__ void k_res_row_fill(ResSpans R, int64_t rows, const int32_t* __restri
                                const int32_t* __restrict__ c_column, const void* __restrict__ c_value,
                                const int32_t* __restrict__ cs_column, const void* __restrict__ cs_value,
                                int value_is_f32, int dtype, int64_t ncols, int32_t* __restrict__ cols,
-                               uint16_t* __restrict__ vals) {
+                               uint16_t* __restrict__ vals, const int32_t* __restrict__ c_edge,
+                               const int32_t* __restrict__ cs_edge, int32_t* __restrict__ edge) {
     const int64_t r = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
     if (r >= rows) return;
     const int lane = threadIdx.x & 63;
@@ -91,10 +92,12 @@ __global__ void k_res_row_fill(ResSpans R, int64_t rows, const int32_t* __restri
     for (int32_t i = lane; i < nl; i += 64) {
         cols[base + i] = clean_col(c_column[l0 + i], ncols);
         vals[base + i] = to_plan_bits(c_value, l0 + i, value_is_f32, dtype);
+        if (edge) edge[base + i] = c_edge[l0 + i];
     }
     for (int32_t i = lane; i < s1 - s0; i += 64) {
         cols[base + nl + i] = clean_col(cs_column[s0 + i], ncols);
         vals[base + nl + i] = to_plan_bits(cs_value, s0 + i, value_is_f32, dtype);
+        if (edge) edge[base + nl + i] = cs_edge[s0 + i];
     }
 }
 
@@ -117,24 +120,29 @@ __global__ void k_tile_counts(const int32_t* __restrict__ s_offsets, int64_t W, 
 // One row's (v0, v1, p0, p1) of a 2:4 group, normalised as block_online.cpp:190-210:
 // both hits -> p0 < p1; one hit at q -> zero-valued companion index (1 if q == 0
 // else 0); no hit or a missing group -> indices (0, 1) with zero values.
-__device__ inline void slot_pair(const float* s_value, const int8_t* s_pos, int32_t g, int32_t g1, int r, int window,
-                                 float& v0, float& v1, int& p0, int& p1) {
+// With s_edge, (e0, e1) are the CSR edges behind (v0, v1) (-1 for a zero filler).
+__device__ inline void slot_pair(const float* s_value, const int8_t* s_pos, const int32_t* s_edge, int32_t g,
+                                 int32_t g1, int r, int window, float& v0, float& v1, int& p0, int& p1, int& e0,
+                                 int& e1) {
     v0 = v1 = 0.f;
     p0 = 0;
     p1 = 1;
+    e0 = e1 = -1;
     if (g >= g1 || r >= window) return;
     const int64_t b = ((int64_t)g * window + r) * 2;
     const int q0 = s_pos[b], q1 = s_pos[b + 1];
     const float x0 = s_value[b], x1 = s_value[b + 1];
+    const int d0 = s_edge ? s_edge[b] : -1, d1 = s_edge ? s_edge[b + 1] : -1;
     if (q0 >= 0 && q1 >= 0) {
-        if (q0 <= q1) { p0 = q0; p1 = q1; v0 = x0; v1 = x1; }
-        else          { p0 = q1; p1 = q0; v0 = x1; v1 = x0; }
+        if (q0 <= q1) { p0 = q0; p1 = q1; v0 = x0; v1 = x1; e0 = d0; e1 = d1; }
+        else          { p0 = q1; p1 = q0; v0 = x1; v1 = x0; e0 = d1; e1 = d0; }
     } else if (q0 >= 0 || q1 >= 0) {
         const int q = q0 >= 0 ? q0 : q1;
         const float x = q0 >= 0 ? x0 : x1;
+        const int d = q0 >= 0 ? d0 : d1;
         const int dummy = q == 0 ? 1 : 0;
-        if (dummy < q) { p0 = dummy; p1 = q; v1 = x; }
-        else           { p0 = q; p1 = dummy; v0 = x; }
+        if (dummy < q) { p0 = dummy; p1 = q; v1 = x; e1 = d; }
+        else           { p0 = q; p1 = dummy; v0 = x; e0 = d; }
     }
 }
 
@@ -142,9 +150,10 @@ __device__ inline void slot_pair(const float* s_value, const int8_t* s_pos, int3
 // its index word and (row 0) the tile's columns.
 __global__ void k_tiles_from_groups(const int32_t* __restrict__ s_offsets, const int32_t* __restrict__ s_column,
                                     const float* __restrict__ s_value, const int8_t* __restrict__ s_pos,
-                                    const int32_t* __restrict__ tile_off, int64_t W, int window,
-                                    int64_t ncols, int dtype, int32_t* __restrict__ sp_cols,
-                                    uint32_t* __restrict__ sp_a, uint16_t* __restrict__ sp_idx) {
+                                    const int32_t* __restrict__ s_edge, const int32_t* __restrict__ tile_off,
+                                    int64_t W, int window, int64_t ncols, int dtype, int32_t* __restrict__ sp_cols,
+                                    uint32_t* __restrict__ sp_a, uint16_t* __restrict__ sp_idx,
+                                    int2* __restrict__ sp_edge) {
     const int64_t w = blockIdx.x;
     if (w >= W) return;
     const int32_t g0 = s_offsets[w], g1 = s_offsets[w + 1];
@@ -156,11 +165,12 @@ __global__ void k_tiles_from_groups(const int32_t* __restrict__ s_offsets, const
         for (int c = 0; c < 4; ++c) {
             const int32_t g = g0 + 4 * j + c;
             float v0, v1;
-            int p0, p1;
-            slot_pair(s_value, s_pos, g, g1, r, window, v0, v1, p0, p1);
+            int p0, p1, e0, e1;
+            slot_pair(s_value, s_pos, s_edge, g, g1, r, window, v0, v1, p0, p1, e0, e1);
             const uint32_t lo = dtype == TRSPMM_BF16 ? f32_to_bf16_bits(v0) : f32_to_f16_bits(v0);
             const uint32_t hi = dtype == TRSPMM_BF16 ? f32_to_bf16_bits(v1) : f32_to_f16_bits(v1);
             sp_a[(t * 16 + r) * 4 + c] = lo | (hi << 16);
+            if (sp_edge) sp_edge[(t * 16 + r) * 4 + c] = make_int2(e0, e1);
             idx |= (uint32_t)((p0 & 3) | ((p1 & 3) << 2)) << (4 * c);
             if (r == 0)
                 for (int q = 0; q < 4; ++q)
@@ -279,6 +289,43 @@ __global__ void k_res_sort_keys(const int32_t* __restrict__ rowptr, int64_t rows
     const uint64_t w = (uint64_t)(r / 16) << 32;
     for (int32_t e = rowptr[r] + (threadIdx.x & 63); e < rowptr[r + 1]; e += 64)
         keys[e] = w | (uint32_t)(cols[e] + 1);
+}
+
+// value of routed-CSR edge e for a refresh: vals[index ? index[e] : e], 0 for
+// an empty slot (e < 0) or an index outside [0, n_vals)
+struct NewValues {
+    const float* vals;
+    int64_t n_vals;
+    const int32_t* index;
+    int dtype;
+    __device__ uint16_t bits(int32_t e) const {
+        if (e < 0) return 0;
+        const int64_t j = index ? (int64_t)index[e] : (int64_t)e;
+        const float f = (j >= 0 && j < n_vals) ? vals[j] : 0.f;
+        return dtype == TRSPMM_BF16 ? f32_to_bf16_bits(f) : f32_to_f16_bits(f);
+    }
+};
+
+__global__ void k_refresh_16(const int32_t* __restrict__ edge, int64_t n, NewValues nv, uint16_t* __restrict__ dst) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = nv.bits(edge[i]);
+}
+
+__global__ void k_refresh_pairs(const int2* __restrict__ edge, int64_t n, NewValues nv, uint32_t* __restrict__ dst) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int2 e = edge[i];
+        dst[i] = (uint32_t)nv.bits(e.x) | ((uint32_t)nv.bits(e.y) << 16);
+    }
+}
+
+// record value bits (32..47) from the edge; an out-of-range column keeps 0
+__global__ void k_refresh_records(const int32_t* __restrict__ edge, int64_t n, NewValues nv,
+                                  uint64_t* __restrict__ rec) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t r = rec[i];
+        const uint64_t v = (int32_t)(uint32_t)r >= 0 ? nv.bits(edge[i]) : 0;
+        rec[i] = (r & ~(0xffffull << 32)) | (v << 32);
+    }
 }
 
 inline unsigned blocks_for(int64_t n, int threads = 256) {
@@ -453,12 +500,28 @@ static int sort_records_by_column(Plan& P, int64_t rows, int64_t total, hipStrea
     TRSPMM_HIP(rocprim::radix_sort_pairs(tmp.p, tmp_bytes, keys.as<uint64_t>(), keys2.as<uint64_t>(),
                                          P.res_rec.as<uint64_t>(), rec2.as<uint64_t>(), (size_t)total, 0, 32 + wbits, s));
     TRSPMM_HIP(hipMemcpyAsync(P.res_rec.p, rec2.p, 8 * total, hipMemcpyDeviceToDevice, s));
+    if (P.has_edges) {
+        // the same keys through the same stable sort: the edges land beside their records
+        if ((rc = P.rec_edge.alloc(4 * total))) return rc;
+        size_t eb = 0;
+        TRSPMM_HIP(rocprim::radix_sort_pairs(nullptr, eb, keys.as<uint64_t>(), keys2.as<uint64_t>(),
+                                             P.res_edge.as<int32_t>(), P.rec_edge.as<int32_t>(), (size_t)total, 0,
+                                             32 + wbits, s));
+        if (eb > tmp_bytes) {
+            tmp_bytes = eb;
+            if ((rc = tmp.alloc(tmp_bytes))) return rc;
+        }
+        TRSPMM_HIP(rocprim::radix_sort_pairs(tmp.p, tmp_bytes, keys.as<uint64_t>(), keys2.as<uint64_t>(),
+                                             P.res_edge.as<int32_t>(), P.rec_edge.as<int32_t>(), (size_t)total, 0,
+                                             32 + wbits, s));
+    }
     TRSPMM_HIP(hipStreamSynchronize(s));     // the temporaries are freed on return
     return TRSPMM_OK;
 }
 
 // residual row CSR from the long/short part lists
-static int build_residual(const trspmm_meta_view& v, int64_t ncols, int dtype, hipStream_t s, Plan& P) {
+static int build_residual(const trspmm_meta_view& v, int64_t ncols, int dtype, hipStream_t s, Plan& P,
+                          const EdgeMaps* em) {
     const int64_t rows = 16 * P.dev.windows;
     int rc;
     DevBuf cnt;
@@ -477,9 +540,12 @@ static int build_residual(const trspmm_meta_view& v, int64_t ncols, int dtype, h
     P.dev.n_rval = total;
     if (total > 0) {
         if ((rc = P.res_cols.alloc(4 * (int64_t)total)) || (rc = P.res_vals.alloc(2 * (int64_t)total))) return rc;
+        if (em && (rc = P.res_edge.alloc(4 * (int64_t)total))) return rc;
         hipLaunchKernelGGL(k_res_row_fill, dim3(blocks_for(rows, 4)), dim3(256), 0, s, RS, rows,
                            P.res_rowptr.as<int32_t>(), v.c_column, v.c_value, v.cs_column, v.cs_value,
-                           v.c_value_is_f32, dtype, ncols, P.res_cols.as<int32_t>(), P.res_vals.as<uint16_t>());
+                           v.c_value_is_f32, dtype, ncols, P.res_cols.as<int32_t>(), P.res_vals.as<uint16_t>(),
+                           em ? em->c_edge : nullptr, em ? em->cs_edge : nullptr,
+                           em ? P.res_edge.as<int32_t>() : nullptr);
         TRSPMM_HIP(hipGetLastError());
         if ((rc = P.res_rec.alloc(8 * (int64_t)total))) return rc;
         hipLaunchKernelGGL(k_res_records, dim3(blocks_for(rows, 4)), dim3(256), 0, s, P.res_rowptr.as<int32_t>(),
@@ -509,7 +575,8 @@ static int tile_slab(const trspmm_plan_opts& o) {
     return (e && std::atoi(e) == 128) ? 128 : 64;
 }
 
-int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtype, hipStream_t s, Plan& P) {
+int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtype, hipStream_t s, Plan& P,
+               const EdgeMaps* em) {
     TRSPMM_CHECK(v.window == 16, "the routed multiply requires window == 16 (16-row MFMA tiles)");
     TRSPMM_CHECK(P.opts.residual_engine >= 0 && P.opts.residual_engine <= TRSPMM_ENGINE_MFMA,
                  "plan options: residual_engine must be TRSPMM_ENGINE_DEFAULT, _VALU or _MFMA");
@@ -547,6 +614,8 @@ int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtyp
     P.dev.dtype = dtype;
     P.dev.n_tsub = v.n_tsub;
     P.dev.n_tval = v.n_tval;
+    P.has_edges = em != nullptr;
+    P.src_nnz = em ? em->nnz : 0;
 
     int rc;
     DevBuf tile_cnt;
@@ -570,6 +639,10 @@ int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtyp
             hipLaunchKernelGGL(k_copy_vals, dim3(blocks_for(v.n_tval)), dim3(256), 0, s, v.t_value, v.n_tval,
                                v.t_value_is_f32, dtype, P.tc_vals.as<uint16_t>());
         TRSPMM_HIP(hipGetLastError());
+        if (em && v.n_tval > 0) {
+            if ((rc = P.tc_edge.alloc(4 * v.n_tval))) return rc;
+            TRSPMM_HIP(hipMemcpyAsync(P.tc_edge.p, em->t_edge, 4 * v.n_tval, hipMemcpyDeviceToDevice, s));
+        }
         if ((rc = P.tc_dense.alloc(2 * 16 * 8 * v.n_tsub))) return rc;
         hipLaunchKernelGGL(k_tc_dense, dim3(blocks_for(16 * v.n_tsub)), dim3(256), 0, s, P.tc_bits.as<uint64_t>(),
                            P.tc_valoff.as<int32_t>(), P.tc_vals.as<uint16_t>(), v.n_tsub, P.tc_dense.as<uint16_t>());
@@ -577,7 +650,7 @@ int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtyp
     }
 
     // residual route: one CSR over the padded rows
-    if ((rc = build_residual(v, ncols, dtype, s, P))) return rc;
+    if ((rc = build_residual(v, ncols, dtype, s, P, em))) return rc;
 
     // 2:4 tiles
     if (W > 0) {
@@ -594,9 +667,11 @@ int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtyp
         if ((rc = P.sp_cols.alloc(4 * 16 * (int64_t)n_tiles)) || (rc = P.sp_a.alloc(4 * 64 * (int64_t)n_tiles)) ||
             (rc = P.sp_idx.alloc(2 * 16 * (int64_t)n_tiles)))
             return rc;
+        if (em && (rc = P.sp_edge.alloc(8 * 64 * (int64_t)n_tiles))) return rc;
         hipLaunchKernelGGL(k_tiles_from_groups, dim3((unsigned)W), dim3(256), 0, s, v.s_offsets, v.s_column,
-                           v.s_value, v.s_pos, P.sp_tile_off.as<int32_t>(), W, v.window, ncols, dtype,
-                           P.sp_cols.as<int32_t>(), P.sp_a.as<uint32_t>(), P.sp_idx.as<uint16_t>());
+                           v.s_value, v.s_pos, em ? em->s_edge : nullptr, P.sp_tile_off.as<int32_t>(), W, v.window,
+                           ncols, dtype, P.sp_cols.as<int32_t>(), P.sp_a.as<uint32_t>(), P.sp_idx.as<uint16_t>(),
+                           em ? P.sp_edge.as<int2>() : nullptr);
         TRSPMM_HIP(hipGetLastError());
     }
     TRSPMM_HIP(hipStreamSynchronize(s));   // temporaries are freed on return
@@ -631,6 +706,48 @@ int build_plan_packed(const int32_t* s_tile_column, const int32_t* s_packed_a, c
     }
     TRSPMM_HIP(hipStreamSynchronize(s));
     return P.build_items(s);
+}
+
+// Value refresh: the routes, tiles, records and work items stay; only the
+// 16-bit values are rewritten, each slot from the CSR edge it was built from,
+// in the plan's own layouts (the dense TC rows are rebuilt from the new TC
+// values).  Stream-ordered and allocation-free.
+int plan_set_values(Plan& P, const float* vals, int64_t n_vals, const int32_t* index, int64_t nnz, hipStream_t s) {
+    if (!P.has_edges) {
+        set_error("plan_set_values: the plan carries no edge maps (route with TRSPMM_ROUTE_EDGES and build it "
+                  "with trspmm_plan_from_meta)");
+        return TRSPMM_EUNSUPPORTED;
+    }
+    TRSPMM_CHECK(nnz == P.src_nnz, "plan_set_values: nnz must be the routed CSR's nonzero count");
+    TRSPMM_CHECK(n_vals >= 0, "plan_set_values: n_vals must be >= 0");
+    TRSPMM_CHECK(n_vals == 0 || vals, "plan_set_values: vals must not be null");
+    const NewValues nv{vals, n_vals, index, P.dev.dtype};
+    const int64_t S = P.dev.n_tsub, TV = P.dev.n_tval, T = P.dev.n_tiles, R = P.dev.n_rval;
+    if (TV > 0) {
+        hipLaunchKernelGGL(k_refresh_16, dim3(blocks_for(TV)), dim3(256), 0, s, P.tc_edge.as<int32_t>(), TV, nv,
+                           P.tc_vals.as<uint16_t>());
+        TRSPMM_HIP(hipGetLastError());
+    }
+    if (S > 0) {
+        hipLaunchKernelGGL(k_tc_dense, dim3(blocks_for(16 * S)), dim3(256), 0, s, P.tc_bits.as<uint64_t>(),
+                           P.tc_valoff.as<int32_t>(), P.tc_vals.as<uint16_t>(), S, P.tc_dense.as<uint16_t>());
+        TRSPMM_HIP(hipGetLastError());
+    }
+    if (T > 0) {
+        hipLaunchKernelGGL(k_refresh_pairs, dim3(blocks_for(64 * T)), dim3(256), 0, s, P.sp_edge.as<int2>(), 64 * T,
+                           nv, P.sp_a.as<uint32_t>());
+        TRSPMM_HIP(hipGetLastError());
+    }
+    if (R > 0) {
+        hipLaunchKernelGGL(k_refresh_16, dim3(blocks_for(R)), dim3(256), 0, s, P.res_edge.as<int32_t>(), R, nv,
+                           P.res_vals.as<uint16_t>());
+        TRSPMM_HIP(hipGetLastError());
+        const int32_t* re = P.rec_edge.p ? P.rec_edge.as<int32_t>() : P.res_edge.as<int32_t>();
+        hipLaunchKernelGGL(k_refresh_records, dim3(blocks_for(R)), dim3(256), 0, s, re, R, nv,
+                           P.res_rec.as<uint64_t>());
+        TRSPMM_HIP(hipGetLastError());
+    }
+    return TRSPMM_OK;
 }
 
 }  // namespace trspmm

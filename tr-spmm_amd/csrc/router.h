@@ -20,11 +20,25 @@ struct Meta {
     int64_t count[29] = {};
     int window = 16;
     int64_t rows = 0;
+    int64_t nnz = 0;
     double seconds = 0.0;
     int alloc(int i, int64_t n) {
         count[i] = n;
         int rc = buf[i].alloc(meta_elem_size(i) * (size_t)(n > 0 ? n : 0));
         data[i] = buf[i].p;
+        return rc;
+    }
+    // optional edge maps (TRSPMM_ROUTE_EDGES): the CSR edge behind each value of
+    // t_value [0], c_value [1], c_value_short [2] and s_value [3] (-1 = empty slot)
+    DevBuf edge_buf[4];
+    void* edge_data[4] = {};
+    int64_t edge_count[4] = {};
+    bool has_edges = false;
+    int edge_alloc(int i, int64_t n) {
+        has_edges = true;
+        edge_count[i] = n;
+        int rc = edge_buf[i].alloc(4 * (size_t)(n > 0 ? n : 0));
+        edge_data[i] = edge_buf[i].p;
         return rc;
     }
 };

@@ -572,6 +572,10 @@ struct Emit {
     int32_t *s_window, *s_offsets;
     int32_t *s_tile_window, *s_tile_group, *s_tile_column, *s_packed_a, *s_packed_meta, *s_window_tile_offset;
     uint8_t* res_flag;       // per edge: goes to the residual route
+    // optional (null = not requested): the CSR edge behind each emitted value,
+    // t_edge [TV] beside t_value, s_edge [G, window, 2] beside s_value (-1 = empty)
+    int32_t* t_edge = nullptr;
+    int32_t* s_edge = nullptr;
 };
 
 // TC: one thread per (window, kept chunk) = sub-block
@@ -632,6 +636,7 @@ __global__ void k_emit_tc(RunArrays R, WinOut O, Emit E, int64_t W, int window, 
                 if (E.erow[e] - row0 == lr) {
                     if (lr >= 8) hi |= 1ull << ((lr - 8) * 8 + q);
                     else lo |= 1ull << (lr * 8 + q);
+                    if (E.t_edge) E.t_edge[voff] = e;
                     E.t_value[voff++] = E.val[e];
                     ++ptr[q];
                 }
@@ -748,6 +753,7 @@ __global__ void k_emit_groups(RunArrays R, WinOut O, Emit E, int64_t W, int wind
         int used = 0;
         float v[2] = {0.f, 0.f};
         int8_t ps[2] = {-1, -1};
+        int32_t ed[2] = {-1, -1};
         for (int q = 0; q < n; ++q) {
             if (ptr[q] >= R.start[runs[q] + 1]) continue;
             const int32_t e = E.sval[ptr[q]];
@@ -756,6 +762,7 @@ __global__ void k_emit_groups(RunArrays R, WinOut O, Emit E, int64_t W, int wind
             mask |= (uint8_t)(1u << q);
             if (used < 2) {
                 v[used] = E.val[e];
+                ed[used] = e;
                 ps[used] = (int8_t)q;
             } else if (exact) {
                 E.res_flag[e] = 1;
@@ -768,6 +775,10 @@ __global__ void k_emit_groups(RunArrays R, WinOut O, Emit E, int64_t W, int wind
         E.s_value[2 * b + 1] = v[1];
         E.s_pos[2 * b] = ps[0];
         E.s_pos[2 * b + 1] = ps[1];
+        if (E.s_edge) {
+            E.s_edge[2 * b] = ed[0];
+            E.s_edge[2 * b + 1] = ed[1];
+        }
     }
 }
 
@@ -826,6 +837,8 @@ struct ResOut {
     float* c_value;
     int32_t *cs_row_offset, *cs_row, *cs_atomic, *cs_column;
     float* cs_value;
+    int32_t* c_edge = nullptr;     // optional: the CSR edge of each long / short entry
+    int32_t* cs_edge = nullptr;
 };
 
 __global__ __launch_bounds__(256) void k_res_emit(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
@@ -866,9 +879,11 @@ __global__ __launch_bounds__(256) void k_res_emit(const int32_t* __restrict__ ro
             if (k < el) {
                 O.c_column[O.ql[r] + k] = col[e];
                 O.c_value[O.ql[r] + k] = val[e];
+                if (O.c_edge) O.c_edge[O.ql[r] + k] = e;
             } else {
                 O.cs_column[O.qs[r] + k - el] = col[e];
                 O.cs_value[O.qs[r] + k - el] = val[e];
+                if (O.cs_edge) O.cs_edge[O.qs[r] + k - el] = e;
             }
         }
         k0 += __popcll(m);
@@ -964,6 +979,7 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
     TRSPMM_CHECK(cfg.mode == TRSPMM_MODE_COMPAT || cfg.mode == TRSPMM_MODE_EXACT || cfg.mode == TRSPMM_MODE_TUNED,
                  "router mode must be COMPAT, EXACT or TUNED");
     TRSPMM_CHECK(cfg.sptc_min_nnz >= 0 && cfg.sptc_min_nnz <= 32, "sptc_min_nnz must be in [0, 32]");
+    TRSPMM_CHECK((cfg.flags & ~TRSPMM_ROUTE_EDGES) == 0, "unknown route flags");
     TRSPMM_CHECK(rows >= 0 && nnz >= 0 && nnz < ((int64_t)1 << 31), "bad rows / nnz");
     TRSPMM_CHECK(ncols > 0 && ncols < ((int64_t)1 << 31), "ncols must be in [1, 2^31)");
     const int window = cfg.window;
@@ -973,6 +989,7 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
     TRSPMM_CHECK(colbits + winbits <= 64, "index range too large");
     M.window = window;
     M.rows = rows;
+    M.nnz = nnz;
     const bool exact = cfg.mode == TRSPMM_MODE_EXACT;
     int rc;
 
@@ -1191,6 +1208,12 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
     E.s_packed_meta = (int32_t*)M.data[27];
     E.s_window_tile_offset = (int32_t*)M.data[28];
     E.res_flag = res_flag.as<uint8_t>();
+    if (cfg.flags & TRSPMM_ROUTE_EDGES) {
+        if ((rc = M.edge_alloc(0, TV)) || (rc = M.edge_alloc(3, G * window * 2))) return rc;
+        E.t_edge = (int32_t*)M.edge_data[0];
+        E.s_edge = (int32_t*)M.edge_data[3];
+        if (TV > 0) TRSPMM_HIP(hipMemsetAsync(E.t_edge, 0xff, 4 * TV, s));   // -1: no edge
+    }
     if (S > 0) {
         hipLaunchKernelGGL(k_emit_tc, dim3(nblk(S)), dim3(256), 0, s, R, O, E, W, window, cfg.part_t, S);
         TRSPMM_HIP(hipGetLastError());
@@ -1243,6 +1266,11 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
               (int32_t*)M.data[7], (int32_t*)M.data[8], (int32_t*)M.data[9], (int32_t*)M.data[10],
               (float*)M.data[11], (int32_t*)M.data[12], (int32_t*)M.data[13], (int32_t*)M.data[14],
               (int32_t*)M.data[15], (float*)M.data[16]};
+    if (cfg.flags & TRSPMM_ROUTE_EDGES) {
+        if ((rc = M.edge_alloc(1, EL)) || (rc = M.edge_alloc(2, ES))) return rc;
+        RO.c_edge = (int32_t*)M.edge_data[1];
+        RO.cs_edge = (int32_t*)M.edge_data[2];
+    }
     if (rows > 0) {
         hipLaunchKernelGGL(k_res_emit, dim3(nblk(rows, 4)), dim3(256), 0, s, rowptr, col, val, res_flag.as<uint8_t>(),
                            n_tcsub.as<int32_t>(), rows, window, cfg.part_c, cfg.short_c, RO);

@@ -16,6 +16,7 @@ OK, EINVAL, EHIP, ENOMEM, EUNSUPPORTED = 0, 1, 2, 3, 4
 F16, BF16 = 0, 1
 ROUTE_TC, ROUTE_SPTC, ROUTE_RESIDUAL, ROUTE_ALL = 1, 2, 4, 7
 MODE_COMPAT, MODE_EXACT, MODE_TUNED = 0, 1, 2
+ROUTE_EDGES = 1   # trspmm_route_cfg.flags: record the CSR edge behind every value slot
 
 # every symbol include/trspmm.h declares (checked by tests/test_abi.py)
 EXPORTS = [
@@ -26,7 +27,8 @@ EXPORTS = [
     "trspmm_csr_transpose", "trspmm_sddmm", "trspmm_bwd_plan_create", "trspmm_bwd_plan_free",
     "trspmm_spmm_t_sddmm", "trspmm_plan_create_ex", "trspmm_plan_from_meta_ex", "trspmm_match_2to4",
     "trspmm_match_2to4_hash", "trspmm_match_2to4_bucket", "trspmm_match_2to4_online", "trspmm_match_2to4_online_noalloc",
-    "trspmm_minhash", "trspmm_minhash_coefficients", "trspmm_tca_order",
+    "trspmm_minhash", "trspmm_minhash_coefficients", "trspmm_tca_order", "trspmm_plan_set_values",
+    "trspmm_bwd_plan_set_values",
 ]
 
 ENGINE_DEFAULT, ENGINE_VALU, ENGINE_MFMA = 0, 1, 2
@@ -37,7 +39,7 @@ class RouteCfg(ctypes.Structure):
         ("part_t", ctypes.c_int32), ("part_c", ctypes.c_int32), ("short_c", ctypes.c_int32),
         ("density", ctypes.c_int32), ("window", ctypes.c_int32), ("wide", ctypes.c_int32),
         ("tc_min_util", ctypes.c_float), ("skip_tc", ctypes.c_int32), ("unit_min_nnz", ctypes.c_int32),
-        ("mode", ctypes.c_int32), ("sptc_min_nnz", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("mode", ctypes.c_int32), ("sptc_min_nnz", ctypes.c_int32), ("flags", ctypes.c_int32),
     ]
 
 
@@ -125,9 +127,11 @@ def lib():
     L.trspmm_memcpy.argtypes = [_P, _P, _I64, _P]
     L.trspmm_host_sort_order.argtypes = [_P, ctypes.c_int32, _P]
     L.trspmm_csr_transpose.argtypes = [_P, _P, _P, _I64, _I64, _I64, _P, _P, _P, _P, _P]
-    L.trspmm_sddmm.argtypes = [_P, _I64, _P, _I64, _P, _I64, _P, _I64, _I64, ctypes.c_int, _P, _P]
+    L.trspmm_sddmm.argtypes = [_P, _I64, _P, _I64, _I64, _P, _I64, _P, _I64, _I64, ctypes.c_int, _P, _P]
     L.trspmm_bwd_plan_create.argtypes = [_P, _P, _P, _P, _I64, _I64, _I64, ctypes.c_int, _P, ctypes.POINTER(_P)]
     L.trspmm_bwd_plan_free.argtypes = [_P]
+    L.trspmm_bwd_plan_set_values.argtypes = [_P, _P, _I64, _P]
+    L.trspmm_plan_set_values.argtypes = [_P, _P, _I64, _P, _I64, _P]
     L.trspmm_spmm_t_sddmm.argtypes = [_P, _P, _I64, _P, _I64, _I64, _P, _I64, _P, _P]
     L.trspmm_match_2to4.argtypes = [_P, _P, _I64, _I64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.POINTER(MatchStats)]

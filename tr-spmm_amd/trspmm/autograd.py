@@ -13,8 +13,10 @@ them together, gathering each dY row once ("fused", the default); otherwise,
 or with backward="routed", dB runs on the routed kernels over A^T's cached
 plan and dvals on the SDDMM kernel.
 
-The forward multiplies with the plan built from A's values: `vals` carries the
-gradient and must hold those values (the plan is not rebuilt per call).
+The forward multiplies with the values in `vals`: when they are not the ones
+A's plans hold (another tensor, or the same one modified in place since), the
+plans of A, A^T and the fused backward take them first (RoutedSparse.set_values,
+a few value-refresh kernels; the routes are not rebuilt).
 """
 from __future__ import annotations
 
@@ -28,16 +30,20 @@ class RoutedSpMM(torch.autograd.Function):
     def forward(ctx, A: RoutedSparse, values: torch.Tensor, B: torch.Tensor, backward: str = "fused"):
         if backward not in ("fused", "routed"):
             raise ValueError("backward must be 'fused' or 'routed'")
+        A.sync_values(values)
         ctx.A = A
+        ctx.epoch = A.values_epoch
         ctx.mode = backward
-        ctx.save_for_backward(B)
+        ctx.save_for_backward(B, values)
         return A.matmul(B)
 
     @staticmethod
     def backward(ctx, dC: torch.Tensor):
         A: RoutedSparse = ctx.A
-        (B,) = ctx.saved_tensors
+        B, values = ctx.saved_tensors
         dY = dC.to(A.dtype).contiguous()
+        if A.values_epoch != ctx.epoch:      # a later forward set other values: back to this one's
+            A.set_values(values)
         d_values = d_B = None
         want_v, want_b = ctx.needs_input_grad[1], ctx.needs_input_grad[2]
         if want_v and want_b and ctx.mode == "fused" and dY.shape[1] <= 512:

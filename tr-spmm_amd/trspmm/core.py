@@ -70,9 +70,11 @@ MODES = {"compat": _lib.MODE_COMPAT, "exact": _lib.MODE_EXACT, "tuned": _lib.MOD
 class RoutedMeta:
     """The router's 29 arrays, resident on the GPU (owned by the library)."""
 
-    def __init__(self, handle, window: int):
+    def __init__(self, handle, window: int, nnz: int = 0, edges: bool = False):
         self._h = handle
         self.window = window
+        self.nnz = nnz              # nonzeros of the routed CSR
+        self.edges = edges          # routed with edge maps (plans accept new values)
         self.seconds = lib().trspmm_meta_seconds(handle)
         self._views = None
 
@@ -111,13 +113,15 @@ TUNED_SPTC_MIN_NNZ = 8   # default 2:4 occupancy floor of the tuned router (kept
 
 def route(rowptr: torch.Tensor, col: torch.Tensor, val: torch.Tensor, ncols: int, *, part_t=32, part_c=32,
           short_c=3, density=8, window=16, wide=16, tc_min_util=0.0, skip_tc=None, unit_min_nnz=None,
-          mode="exact", sptc_min_nnz=None) -> RoutedMeta:
+          mode="exact", sptc_min_nnz=None, edges=False) -> RoutedMeta:
     """Routes a canonical CSR on the GPU (see trspmm_route).
 
     mode "compat": the reference's metadata bit for bit; "exact": the same t_*/s_*
     with the reference's dropped 2:4 overflow moved to the residual route;
     "tuned": exact plus 2:4 groups with fewer than `sptc_min_nnz` kept slots
-    moved to the residual route."""
+    moved to the residual route.  edges: also record the CSR edge behind every
+    value slot, so that plans built from the result accept new values
+    (Plan.set_values) without routing again."""
     _require_cuda()
     es, eu = env_router_knobs()
     mode_code = MODES[mode] if isinstance(mode, str) else int(mode)
@@ -125,7 +129,7 @@ def route(rowptr: torch.Tensor, col: torch.Tensor, val: torch.Tensor, ncols: int
         sptc_min_nnz = TUNED_SPTC_MIN_NNZ if mode_code == _lib.MODE_TUNED else 0
     cfg = _lib.RouteCfg(part_t, part_c, short_c, density, window, wide, float(tc_min_util),
                         int(es if skip_tc is None else bool(skip_tc)), int(eu if unit_min_nnz is None else unit_min_nnz),
-                        mode_code, int(sptc_min_nnz), 0)
+                        mode_code, int(sptc_min_nnz), _lib.ROUTE_EDGES if edges else 0)
     dev = torch.device("cuda", torch.cuda.current_device())
     rp = rowptr.to(dev, torch.int32).contiguous()
     ci = col.to(dev, torch.int32).contiguous()
@@ -133,7 +137,7 @@ def route(rowptr: torch.Tensor, col: torch.Tensor, val: torch.Tensor, ncols: int
     h = ctypes.c_void_p()
     check(lib().trspmm_route(ptr(rp), ptr(ci), ptr(vv), rp.numel() - 1, ci.numel(), int(ncols), ctypes.byref(cfg),
                              _lib.current_stream(), ctypes.byref(h)), "trspmm_route")
-    return RoutedMeta(h, window)
+    return RoutedMeta(h, window, ci.numel(), bool(edges))
 
 
 class Plan:
@@ -188,7 +192,26 @@ class Plan:
         h = ctypes.c_void_p()
         check(lib().trspmm_plan_from_meta_ex(meta.handle, int(m_out), int(ncols), dtype_code(dtype), ctypes.byref(opts),
                                              _lib.current_stream(), ctypes.byref(h)), "trspmm_plan_from_meta_ex")
-        return Plan(h, m_out, ncols, dtype)
+        plan = Plan(h, m_out, ncols, dtype)
+        plan.src_nnz = meta.nnz if meta.edges else None
+        return plan
+
+    def set_values(self, vals: torch.Tensor, index: torch.Tensor | None = None):
+        """New values for a plan built from a meta routed with edges=True: the
+        routed CSR's edge e takes vals[index[e]] (index=None: vals[e]); fp32 device
+        tensors, index int32 of the routed CSR's nnz entries.  Stream-ordered."""
+        nnz = getattr(self, "src_nnz", None)
+        if nnz is None:
+            raise ValueError("this plan has no edge maps: route with edges=True and build it with Plan.from_meta")
+        if not vals.is_cuda or vals.dtype != torch.float32 or vals.dim() != 1 or not vals.is_contiguous():
+            raise ValueError("vals must be a contiguous 1-D float32 device tensor")
+        if index is not None and (not index.is_cuda or index.dtype != torch.int32 or index.shape != (nnz,)
+                                  or not index.is_contiguous()):
+            raise ValueError(f"index must be a contiguous int32 device tensor of shape ({nnz},)")
+        if index is None and vals.numel() != nnz:
+            raise ValueError(f"vals must have {nnz} entries (the routed CSR's nonzeros)")
+        check(lib().trspmm_plan_set_values(self._h, ptr(vals), vals.numel(), ptr(index) if index is not None else None,
+                                           nnz, _lib.current_stream()), "trspmm_plan_set_values")
 
     def matmul(self, B: torch.Tensor, out: torch.Tensor | None = None, route_mask: int = _lib.ROUTE_ALL):
         check_operand(B, self.ncols, self.dtype)

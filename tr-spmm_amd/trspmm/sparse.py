@@ -8,10 +8,15 @@ Preprocess once (device router -> device plan), multiply many times:
 For the backward pass the handle also builds, once and on first use, the
 routed plan of A^T (`transpose()`), and computes the value gradient over A's
 pattern (`sddmm()`); `trspmm.autograd.spmm` wires both into torch.autograd.
+
+The router records the CSR edge behind every value slot, so new values
+(`set_values()`) refresh the plans of A and A^T and the fused-backward plan
+in place, without routing again.
 """
 from __future__ import annotations
 
 import ctypes
+import weakref
 
 import torch
 
@@ -52,7 +57,7 @@ class RoutedSparse:
         self.mode = mode
         self.router_kwargs = dict(router_kwargs)
         self.window = int(router_kwargs.get("window", 16))
-        self.meta = route(rowptr, col, val, self.ncols, mode=mode, **router_kwargs)
+        self.meta = route(rowptr, col, val, self.ncols, mode=mode, edges=True, **router_kwargs)
         self.engine, self.slab = engine, slab
         self.plan_opts = dict(plan_opts or {})      # tile_cap / tc_cap / res_cap / item_order
         self.plan = Plan.from_meta(self.meta, self.m, self.ncols, dtype, engine=engine, slab=slab, **self.plan_opts)
@@ -65,6 +70,35 @@ class RoutedSparse:
         self._t = None
         self._tcsr = None
         self._bwd = None
+        self._synced = (weakref.ref(self.values), self.values._version)   # values the plans hold
+        self.values_epoch = 0                                               # counts set_values calls
+
+    def set_values(self, values: torch.Tensor):
+        """New values of A's nonzeros ([nnz], A's edge order, any float dtype):
+        the plans of A (and of A^T and the fused backward, once built) take them
+        in place; the pattern and routes stay."""
+        v = values.detach()
+        if v.shape != (self.nnz,):
+            raise ValueError(f"values must have shape ({self.nnz},), got {tuple(v.shape)}")
+        self.values = v.to(self.values.device, torch.float32).contiguous()
+        self.plan.set_values(self.values)
+        if self._tcsr is not None:
+            t_val, perm = self._tcsr[2], self._tcsr[3]
+            torch.index_select(self.values, 0, perm, out=t_val)
+            if self._t is not None:
+                self._t.set_values(t_val)
+            if self._bwd is not None:
+                check(lib().trspmm_bwd_plan_set_values(self._bwd.handle, ptr(self.values), self.nnz,
+                                                       _lib.current_stream()), "trspmm_bwd_plan_set_values")
+        self._synced = (weakref.ref(values), values._version)
+        self.values_epoch += 1
+
+    def sync_values(self, values: torch.Tensor):
+        """set_values(values) unless the plans already hold this tensor at its
+        current version (the same object, not modified in place since)."""
+        ref, version = self._synced
+        if ref() is not values or values._version != version:
+            self.set_values(values)
 
     @property
     def stats(self):
@@ -160,7 +194,7 @@ class RoutedSparse:
             out = torch.empty(self.nnz, dtype=torch.float32, device=X.device)
         elif out.dtype != torch.float32 or out.shape != (self.nnz,) or not out.is_contiguous() or not out.is_cuda:
             raise ValueError(f"out must be a contiguous float32 device tensor of shape ({self.nnz},)")
-        check(lib().trspmm_sddmm(ptr(self.rowptr), self.rows, ptr(self.col), self.nnz, ptr(X), X.stride(0), ptr(Y),
-                                 Y.stride(0), X.shape[1], dtype_code(self.dtype), ptr(out), _lib.current_stream()),
-              "trspmm_sddmm")
+        check(lib().trspmm_sddmm(ptr(self.rowptr), self.rows, ptr(self.col), self.nnz, self.ncols, ptr(X), X.stride(0),
+                                 ptr(Y), Y.stride(0), X.shape[1], dtype_code(self.dtype), ptr(out),
+                                 _lib.current_stream()), "trspmm_sddmm")
         return out
