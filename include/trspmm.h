@@ -80,7 +80,8 @@ typedef struct trspmm_route_cfg {
     int32_t skip_tc;        /* LIBRA_SKIP_TC */
     int32_t unit_min_nnz;   /* LIBRA_SPTC_UNIT_MIN_NNZ (2) */
     int32_t mode;           /* TRSPMM_MODE_* */
-    int32_t sptc_min_nnz;   /* TUNED: a 2:4 group with fewer nonzeros goes to the residual route */
+    int32_t sptc_min_nnz;   /* TUNED: a 2:4 group with fewer nonzeros goes to the residual route
+                               (0 keeps every group, 33 none) */
     int32_t flags;          /* TRSPMM_ROUTE_* bits (0 = none) */
 } trspmm_route_cfg;
 

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.h"
+#include "plan.h"
 
 namespace trspmm {
 
@@ -13,6 +14,12 @@ struct BwdPlan {
     DevBuf rowptr, col, val, invperm, dvals_t, items, zero_rows;
     int64_t rows = 0, nnz = 0, ncols = 0, n_items = 0, n_zero = 0;
     int dtype = TRSPMM_F16;
+    // routed path (the default): A^T routed with every nonzero a residual record,
+    // its multiply computing the value gradient of each record on the side
+    // (D), pos[e] = the record of A's edge e, perm = A's edge of each A^T entry
+    Plan at;
+    DevBuf pos, perm, D;
+    bool routed = false;
 };
 
 int bwd_plan_build(const int32_t* t_rowptr, const int32_t* t_col, const float* t_val, const int32_t* perm,

@@ -10,6 +10,7 @@
 //                  (X = dY rows x K, Y = B ncols x K, fp16/bf16, fp32 result):
 //                  the gradient of sum(C * dY), C = A·B, with respect to A's
 //                  values, in A's edge order.
+#include <cstdlib>
 #include <cstring>   // before rocPRIM: its host code calls memset
 
 #include <rocprim/rocprim.hpp>
@@ -18,6 +19,7 @@
 #include <vector>
 
 #include "backward.h"
+#include "router.h"
 #include "scan.h"
 
 namespace trspmm {
@@ -443,6 +445,20 @@ __global__ void k_scatter16(const float* __restrict__ src, const int32_t* __rest
     if (e < n) dst[inv[e]] = dtype == TRSPMM_BF16 ? f32_to_bf16_bits(src[e]) : f32_to_f16_rne(src[e]);
 }
 
+// pos[perm[e]] = i for the A^T edge e of record i: the record of each A edge
+__global__ void k_pos_from_records(const int32_t* __restrict__ edge, int64_t n, const int32_t* __restrict__ perm,
+                                   int64_t nnz, int32_t* __restrict__ pos) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t e = edge[i];
+        if (e >= 0 && e < nnz) pos[perm[e]] = (int32_t)i;
+    }
+}
+
+__global__ void k_count_negative(const int32_t* __restrict__ v, int64_t n, int32_t* __restrict__ cnt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && v[i] < 0) atomicAdd(cnt, 1);
+}
+
 // err |= 1 unless inv[perm[i]] == i for every i (perm entries already in range)
 __global__ void k_check_inverse(const int32_t* __restrict__ perm, const int32_t* __restrict__ inv, int64_t n,
                                 int32_t* __restrict__ err) {
@@ -550,6 +566,9 @@ int launch_sddmm(const int32_t* rowptr, int64_t rows, const int32_t* col, int64_
     return launch_sddmm_typed<BF16>(rowptr, rows, col, nnz, ncols, X, ldx, Y, ldy, K, out, s);
 }
 
+static int build_routed(const int32_t* t_rowptr, const int32_t* t_col, const float* t_val, const int32_t* perm,
+                        int64_t rows, int64_t nnz, int64_t ncols, int dtype, hipStream_t s, BwdPlan& P);
+
 int bwd_plan_build(const int32_t* t_rowptr, const int32_t* t_col, const float* t_val, const int32_t* perm,
                    int64_t rows, int64_t nnz, int64_t ncols, int dtype, hipStream_t s, BwdPlan& P) {
     TRSPMM_CHECK(rows >= 0 && nnz >= 0 && ncols >= 0, "bwd plan: negative size");
@@ -651,6 +670,56 @@ int bwd_plan_build(const int32_t* t_rowptr, const int32_t* t_col, const float* t
         TRSPMM_CHECK(bad[0] == 0, "bwd plan: perm is not a permutation of [0, nnz)");
     }
     TRSPMM_HIP(hipStreamSynchronize(s));               // host vectors go out of scope
+    return build_routed(t_rowptr, t_col, t_val, perm, rows, nnz, ncols, dtype, s, P);
+}
+
+// The routed backward: A^T routed with every nonzero a residual record (no
+// dense or 2:4 tiles: the fused value gradient reads the residual records),
+// its plan with edge maps, and pos[e] = the record that holds A's edge e.
+// TRSPMM_BACKWARD=lanes in the environment keeps the lane-group kernel instead.
+static int build_routed(const int32_t* t_rowptr, const int32_t* t_col, const float* t_val, const int32_t* perm,
+                        int64_t rows, int64_t nnz, int64_t ncols, int dtype, hipStream_t s, BwdPlan& P) {
+    const char* env = std::getenv("TRSPMM_BACKWARD");
+    if ((env && std::strcmp(env, "lanes") == 0) || nnz == 0 || rows == 0 || ncols == 0) return TRSPMM_OK;
+    trspmm_route_cfg cfg{};
+    cfg.part_t = 32;
+    cfg.part_c = 32;
+    cfg.short_c = 3;
+    cfg.density = 8;
+    cfg.window = 16;
+    cfg.wide = 16;
+    cfg.tc_min_util = 0.f;
+    cfg.skip_tc = 1;
+    cfg.unit_min_nnz = 2;
+    cfg.mode = TRSPMM_MODE_TUNED;
+    cfg.sptc_min_nnz = 33;
+    cfg.flags = TRSPMM_ROUTE_EDGES;
+    int rc;
+    {
+        Meta M;
+        if ((rc = route_device(t_rowptr, t_col, t_val, rows, nnz, ncols, cfg, s, M))) return rc;
+        P.at.opts.residual_engine = TRSPMM_ENGINE_MFMA;
+        const EdgeMaps em = meta_edges(M);
+        if ((rc = build_plan(meta_view(M), rows, ncols, dtype, s, P.at, &em))) return rc;
+    }
+    if (!fused_sddmm_supported(P.at.dev) || P.at.dev.n_rval != nnz) return TRSPMM_OK;
+    DevBuf cnt;
+    if ((rc = P.pos.alloc(4 * nnz)) || (rc = P.perm.alloc(4 * nnz)) || (rc = P.D.alloc(4 * nnz)) ||
+        (rc = cnt.alloc(4)))
+        return rc;
+    TRSPMM_HIP(hipMemsetAsync(P.pos.p, 0xff, 4 * nnz, s));
+    TRSPMM_HIP(hipMemsetAsync(cnt.p, 0, 4, s));
+    TRSPMM_HIP(hipMemcpyAsync(P.perm.p, perm, 4 * nnz, hipMemcpyDeviceToDevice, s));
+    const int32_t* rec_edge = P.at.rec_edge.p ? P.at.rec_edge.as<int32_t>() : P.at.res_edge.as<int32_t>();
+    hipLaunchKernelGGL(k_pos_from_records, dim3(nblk(nnz)), dim3(256), 0, s, rec_edge, nnz, perm, nnz,
+                       P.pos.as<int32_t>());
+    TRSPMM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_count_negative, dim3(nblk(nnz)), dim3(256), 0, s, P.pos.as<int32_t>(), nnz, cnt.as<int32_t>());
+    TRSPMM_HIP(hipGetLastError());
+    int32_t missing = 0;
+    TRSPMM_HIP(hipMemcpyAsync(&missing, cnt.p, 4, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipStreamSynchronize(s));
+    P.routed = missing == 0;
     return TRSPMM_OK;
 }
 
@@ -662,11 +731,24 @@ int bwd_plan_set_values(BwdPlan& P, const float* vals, int64_t nnz, hipStream_t 
     hipLaunchKernelGGL(k_scatter16, dim3(nblk(nnz)), dim3(256), 0, s, vals, P.invperm.as<int32_t>(), nnz, P.dtype,
                        P.val.as<uint16_t>());
     TRSPMM_HIP(hipGetLastError());
+    if (P.at.has_edges) return plan_set_values(P.at, vals, nnz, P.perm.as<int32_t>(), nnz, s);
     return TRSPMM_OK;
 }
 
 int launch_bwd(const BwdPlan& P, const void* dY, int64_t lddy, const void* B, int64_t ldb, int64_t K, float* dB,
                int64_t lddb, float* dvals, hipStream_t s) {
+    if (P.routed) {
+        // dB = A^T dY on A^T's routed plan, the value gradient of every record on
+        // the side, then into A's edge order
+        TRSPMM_CHECK(dvals, "fused backward: null pointer");
+        int rc = launch_spmm_sddmm(P.at.dev, dY, lddy, K, dB, lddb, B, ldb, P.rows, P.D.as<float>(), s);
+        if (rc) return rc;
+        const int64_t per = ((P.nnz + kGatherSpan - 1) / kGatherSpan + 7) / 8;
+        hipLaunchKernelGGL(k_gather_f32, dim3((unsigned)(8 * per)), dim3(256), 0, s, P.D.as<float>(),
+                           P.pos.as<int32_t>(), P.nnz, per, dvals);
+        TRSPMM_HIP(hipGetLastError());
+        return TRSPMM_OK;
+    }
     TRSPMM_CHECK(K > 0 && K % 8 == 0 && K <= 512, "fused backward: K must be a multiple of 8 in [8, 512]");
     TRSPMM_CHECK(lddy % 8 == 0 && ldb % 8 == 0 && lddy >= K && ldb >= K && lddb % 4 == 0 && lddb >= K,
                  "fused backward: row strides must be >= K, dY/B strides multiples of 8, dB stride of 4");

@@ -119,37 +119,7 @@ int trspmm_plan_from_meta_ex(const trspmm_meta* meta, int64_t m_out, int64_t nco
                              const trspmm_plan_opts* opts, void* stream, trspmm_plan** out) {
     TRSPMM_NOTNULL(meta, "meta");
     const Meta& M = meta->meta;
-    trspmm_meta_view v{};
-    v.t_row_offset = (const int32_t*)M.data[0];
-    v.n_tparts = M.count[4];
-    v.t_block_offset = (const int32_t*)M.data[1];
-    v.t_column = (const int32_t*)M.data[2];
-    v.n_tsub = M.count[2] / 8;
-    v.t_value = M.data[3];
-    v.n_tval = M.count[3];
-    v.t_window_row = (const int32_t*)M.data[4];
-    v.t_binary = (const int64_t*)M.data[6];
-    v.c_row_offset = (const int32_t*)M.data[7];
-    v.n_cparts = M.count[8];
-    v.c_row = (const int32_t*)M.data[8];
-    v.c_column = (const int32_t*)M.data[10];
-    v.n_cval = M.count[10];
-    v.c_value = M.data[11];
-    v.cs_row_offset = (const int32_t*)M.data[12];
-    v.n_csparts = M.count[13];
-    v.cs_row = (const int32_t*)M.data[13];
-    v.cs_column = (const int32_t*)M.data[15];
-    v.n_csval = M.count[15];
-    v.cs_value = M.data[16];
-    v.s_column = (const int32_t*)M.data[17];
-    v.n_groups = M.count[21];
-    v.s_value = (const float*)M.data[19];
-    v.s_pos = (const int8_t*)M.data[20];
-    v.s_offsets = (const int32_t*)M.data[22];
-    v.n_windows = M.count[22] - 1;
-    v.t_value_is_f32 = 1;
-    v.c_value_is_f32 = 1;
-    v.window = M.window;
+    const trspmm_meta_view v = meta_view(M);
     if (!M.has_edges) return trspmm_plan_create_ex(&v, m_out, ncols, dtype, opts, stream, out);
     TRSPMM_NOTNULL(out, "out");
     *out = nullptr;
@@ -159,8 +129,7 @@ int trspmm_plan_from_meta_ex(const trspmm_meta* meta, int64_t m_out, int64_t nco
         return TRSPMM_ENOMEM;
     }
     if (opts) p->plan.opts = *opts;
-    const EdgeMaps em{(const int32_t*)M.edge_data[0], (const int32_t*)M.edge_data[1], (const int32_t*)M.edge_data[2],
-                      (const int32_t*)M.edge_data[3], M.nnz};
+    const EdgeMaps em = meta_edges(M);
     const int rc = build_plan(v, m_out, ncols, dtype, (hipStream_t)stream, p->plan, &em);
     if (rc != TRSPMM_OK) {
         delete p;

@@ -42,6 +42,13 @@ int build_plan_packed(const int32_t* s_tile_column, const int32_t* s_packed_a, c
 // routed multiply (spmm.hip)
 int launch_spmm(const PlanDev& p, const void* B, int64_t ldb, int64_t K, float* C, int64_t ldc, uint32_t mask,
                 hipStream_t s);
+// dB = (the plan's matrix) dY and, for each residual record of the plan's
+// window row r and column c, D[record] = <X[r], dY[c]> (the value gradient of
+// a plan of A^T, X = the forward's B).  The plan must hold residual records
+// only (fused_sddmm_supported).
+int fused_sddmm_supported(const PlanDev& p);
+int launch_spmm_sddmm(const PlanDev& p, const void* dY, int64_t lddy, int64_t K, float* dB, int64_t lddb,
+                      const void* X, int64_t ldx, int64_t x_rows, float* D, hipStream_t s);
 
 }  // namespace trspmm
 

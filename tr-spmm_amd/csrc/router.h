@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.h"
+#include "plan.h"
 
 namespace trspmm {
 
@@ -45,6 +46,11 @@ struct Meta {
 
 int route_device(const int32_t* rowptr, const int32_t* col, const float* val, int64_t rows, int64_t nnz,
                  int64_t ncols, const trspmm_route_cfg& cfg, hipStream_t s, Meta& M);
+
+// device views of a router result's arrays (the op's inputs, fp32 values) and
+// of its edge maps (when it has them)
+trspmm_meta_view meta_view(const Meta& M);
+EdgeMaps meta_edges(const Meta& M);
 
 }  // namespace trspmm
 

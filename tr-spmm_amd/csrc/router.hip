@@ -978,7 +978,7 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
     TRSPMM_CHECK(cfg.part_t > 0 && cfg.part_c > 0, "partSize_t and partSize_c must be positive");
     TRSPMM_CHECK(cfg.mode == TRSPMM_MODE_COMPAT || cfg.mode == TRSPMM_MODE_EXACT || cfg.mode == TRSPMM_MODE_TUNED,
                  "router mode must be COMPAT, EXACT or TUNED");
-    TRSPMM_CHECK(cfg.sptc_min_nnz >= 0 && cfg.sptc_min_nnz <= 32, "sptc_min_nnz must be in [0, 32]");
+    TRSPMM_CHECK(cfg.sptc_min_nnz >= 0 && cfg.sptc_min_nnz <= 33, "sptc_min_nnz must be in [0, 33]");
     TRSPMM_CHECK((cfg.flags & ~TRSPMM_ROUTE_EDGES) == 0, "unknown route flags");
     TRSPMM_CHECK(rows >= 0 && nnz >= 0 && nnz < ((int64_t)1 << 31), "bad rows / nnz");
     TRSPMM_CHECK(ncols > 0 && ncols < ((int64_t)1 << 31), "ncols must be in [1, 2^31)");
@@ -1282,6 +1282,46 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
     const auto t_end = std::chrono::steady_clock::now();
     M.seconds = std::chrono::duration<double>(t_end - t_begin).count();
     return TRSPMM_OK;
+}
+
+trspmm_meta_view meta_view(const Meta& M) {
+    trspmm_meta_view v{};
+    v.t_row_offset = (const int32_t*)M.data[0];
+    v.n_tparts = M.count[4];
+    v.t_block_offset = (const int32_t*)M.data[1];
+    v.t_column = (const int32_t*)M.data[2];
+    v.n_tsub = M.count[2] / 8;
+    v.t_value = M.data[3];
+    v.n_tval = M.count[3];
+    v.t_window_row = (const int32_t*)M.data[4];
+    v.t_binary = (const int64_t*)M.data[6];
+    v.c_row_offset = (const int32_t*)M.data[7];
+    v.n_cparts = M.count[8];
+    v.c_row = (const int32_t*)M.data[8];
+    v.c_column = (const int32_t*)M.data[10];
+    v.n_cval = M.count[10];
+    v.c_value = M.data[11];
+    v.cs_row_offset = (const int32_t*)M.data[12];
+    v.n_csparts = M.count[13];
+    v.cs_row = (const int32_t*)M.data[13];
+    v.cs_column = (const int32_t*)M.data[15];
+    v.n_csval = M.count[15];
+    v.cs_value = M.data[16];
+    v.s_column = (const int32_t*)M.data[17];
+    v.n_groups = M.count[21];
+    v.s_value = (const float*)M.data[19];
+    v.s_pos = (const int8_t*)M.data[20];
+    v.s_offsets = (const int32_t*)M.data[22];
+    v.n_windows = M.count[22] - 1;
+    v.t_value_is_f32 = 1;
+    v.c_value_is_f32 = 1;
+    v.window = M.window;
+    return v;
+}
+
+EdgeMaps meta_edges(const Meta& M) {
+    return EdgeMaps{(const int32_t*)M.edge_data[0], (const int32_t*)M.edge_data[1], (const int32_t*)M.edge_data[2],
+                    (const int32_t*)M.edge_data[3], M.nnz};
 }
 
 }  // namespace trspmm

@@ -555,7 +555,7 @@ struct DeepCfg {
     // per wave 0.121 ms, 8 KB 0.129 ms; 32 KB per wave at 4 waves per CU
     // 0.154 ms; the registers of 16 KB in flight do not fit two waves per SIMD
     // beside the step's other state.)
-    static constexpr int dep(int R) { return PW == 128 ? 1 : (R == 32 ? 2 : 1); }
+    static constexpr int dep(int R) { return PW >= 128 ? 1 : (R == 32 ? 2 : 1); }
 };
 
 __device__ inline int deep_phys(int L) { return (L & ~31) | ((L & 7) << 2) | ((L >> 3) & 3); }
@@ -571,6 +571,7 @@ __device__ inline int deep_phys(int L) { return (L & ~31) | ((L & 7) << 2) | ((L
 struct DeepRec {
     int32_t cl;
     uint32_t hi;
+    float part;        // fused SDDMM, passes after the first: the record's partial sum so far
 };
 // a step's A fragment (TC, 2:4), loaded right after the fragment of the step
 // DEP earlier was applied (ring of DEP); by the time it is needed, the gathers
@@ -580,6 +581,29 @@ struct DeepFrag {
     int32_t idx;
     bool ok;
 };
+
+// Fused SDDMM (residual route of a plan of A^T, every nonzero a record): for
+// record k of a step, (lr, slot k) of the window's 16 x 32 block
+//     S = X_w G^T,   X_w = the 16 rows of X (rows of A^T, i.e. B of the forward)
+// in this pass's features, G = the step's 32 staged rows (rows of dY), is the
+// value gradient <X[row], dY[col]> of that nonzero; D[record] receives it (the
+// first pass stores, later passes add what the earlier ones stored).  A pass
+// covers H PW-feature halves, each step gathered once per half, so K <= 2 PW
+// is one pass (H = 2 holds both halves' accumulators).  xa holds the pass's A
+// operand (lane (kg, r): X row r, features 32 ks + 8 kg .. + 7).
+template <int PW, int H>
+struct Sdd {
+    u32x4 xa[H * PW / 32];
+    float* D;
+    bool rmw;
+};
+struct NoSdd {};
+
+// slot of block h (16 of the step's 32 staged rows) read by lane (kg, r): the
+// physical rows of one block's 16 slots are two per residue mod 8, so each
+// 16-lane group's 16-byte reads cover the LDS banks twice (2-way, the least
+// the 2 PW + 32-byte row stride allows)
+__device__ inline int sdd_slot(int h, int r) { return ((r >> 2) << 3) | ((2 * h + ((r >> 1) & 1)) << 1) | (r & 1); }
 
 // staged rows of step s (s clamped to the last step)
 template <int KIND>
@@ -603,10 +627,15 @@ __device__ inline T load_at(const void* base, uint32_t byte_off) {
 }
 
 template <int KIND>
-__device__ inline void deep_rec(const PlanDev& p, int32_t first, int32_t end, int n, int s, int lane, DeepRec& m) {
+__device__ inline void deep_rec(const PlanDev& p, int32_t first, int32_t end, int n, int s, int lane, DeepRec& m,
+                                const float* part = nullptr) {
     const int sc = s < n ? s : n - 1;                      // clamped: the loads stay in range
     const int nrows = deep_rows<KIND>(first, end, n, s);
     const uint32_t l = (uint32_t)(lane < nrows ? lane : nrows - 1);
+    if (part) {                                            // lane 32 + j: the partial of row j
+        const uint32_t j = (uint32_t)((lane & 31) < nrows ? (lane & 31) : nrows - 1);
+        m.part = load_at<float>(part, 4u * ((uint32_t)(first + 32 * sc) + j));
+    }
     if constexpr (KIND == 0) {
         m.cl = load_at<int32_t>(p.tc_cols, 4u * ((uint32_t)(first + 4 * sc) * 8u + l));
     } else if constexpr (KIND == 1) {
@@ -622,10 +651,13 @@ __device__ inline void deep_rec(const PlanDev& p, int32_t first, int32_t end, in
 // byte 8 l (lanes past the step's rows store column -1, value 0; no branch)
 template <int KIND>
 __device__ inline void deep_stage(const DeepRec& m, int32_t first, int32_t end, int n, int s, uint8_t* __restrict__ ms,
-                                  int lane) {
+                                  int lane, bool part = false) {
     const bool v = (s < n) & (lane < deep_rows<KIND>(first, end, n, s));
+    // (residual steps have at most 32 rows: bytes 256.. of the slot are free for
+    // the partials of the fused SDDMM, row j's at 256 + 8 j)
     if constexpr (KIND == 2)
-        *(u32x2*)(ms + 8 * lane) = u32x2{(uint32_t)(v ? m.cl : -1), v ? m.hi : 0u};
+        *(u32x2*)(ms + 8 * lane) = (part & (lane >= 32)) ? u32x2{__builtin_bit_cast(uint32_t, m.part), 0u}
+                                                         : u32x2{(uint32_t)(v ? m.cl : -1), v ? m.hi : 0u};
     else
         *(int32_t*)(ms + 8 * lane) = v ? m.cl : -1;
 }
@@ -687,7 +719,7 @@ __device__ inline void deep_park(const u32x4 (&g)[R * DeepCfg<PW>::PIECES / 64],
     uint8_t* base = Bs + deep_phys(l0) * RS + 16 * (lane % PIECES);
 #pragma unroll
     for (int k = 0; k < R * PIECES / 64; ++k) {
-        // deep_phys(RPL k + l0) - deep_phys(l0) = deep_phys(RPL k) for RPL = 4 or 8, l0 < RPL
+        // deep_phys(RPL k + l0) - deep_phys(l0) = deep_phys(RPL k) for RPL = 2, 4 or 8, l0 < RPL
         const int L = RPL * k;
         const int dphys = (L & ~31) + ((L & 7) << 2) + ((L >> 3) & 3);
         *(u32x4*)(base + dphys * RS) = g[k];
@@ -700,10 +732,13 @@ __device__ inline void deep_park(const u32x4 (&g)[R * DeepCfg<PW>::PIECES / 64],
 // of s + DEP into the registers just parked, apply s, then load the A
 // fragment of s + DEP into the slot s's fragment used.  The loop is unrolled
 // so that every ring index is a compile-time constant.
-template <int KIND, int PW, typename DT>
+template <int KIND, int PW, typename DT, int H = 1, typename SD = NoSdd>
 __device__ inline void run_deep(const PlanDev& p, __amdgpu_buffer_rsrc_t rsB, uint32_t ldb2, int K, int fp,
                                 int32_t first, int32_t end, int n, uint8_t* __restrict__ Bs, uint8_t* __restrict__ Ms,
-                                int lane, f32x4 (&acc)[PW / 16]) {
+                                int lane, f32x4 (&acc)[H][PW / 16], const SD& sd = SD{}) {
+    constexpr bool FUSED = !std::is_same<SD, NoSdd>::value;
+    static_assert(!FUSED || KIND == 2, "the fused SDDMM covers residual records");
+    static_assert(H == 1 || (FUSED && H == 2), "two halves per step: the fused SDDMM only");
     using X8 = typename DT::x8;
     using X16 = typename DT::x16;
     constexpr int R = KIND == 1 ? 64 : 32;
@@ -711,15 +746,23 @@ __device__ inline void run_deep(const PlanDev& p, __amdgpu_buffer_rsrc_t rsB, ui
     constexpr int DEP = DeepCfg<PW>::dep(R);               // steps in flight
     static_assert(DEP == 1 || DEP == 2, "1 or 2 steps in flight");
     constexpr int MR = 2 * DEP;                            // record / fragment rings (>= DEP + 1)
-    constexpr int U = MR;                                  // unroll: a multiple of DEP and MR
+    constexpr int U = MR;                                  // unroll: a multiple of DEP, MR and H
     constexpr int NB = PW / 16;
     constexpr int RS = DeepCfg<PW>::RS;
     constexpr int MS = 512;                                // bytes per LDS record slot (64 lanes x 8)
+    static_assert(U % H == 0, "the unrolled bodies cover whole steps");
     const int r = lane & 15, kg = lane >> 4;
-    const int fw = K - fp < PW ? K - fp : PW;
+    // virtual step v = H s + half: step s's records, features fp + PW half ..
+    auto fw_of = [&](int half) { return K - (fp + PW * half) < PW ? K - (fp + PW * half) : PW; };
+    const int nv = H * n;
     // transposed reads: lane (kg, r) of n-block nb reads logical rows 8 kg + r / 4
     // (+ 4, + 32, + 36), features 16 nb + 4 (r % 4) .. + 3
     const uint8_t* rd = Bs + ((r & 12) + kg) * RS + 8 * (r & 3);
+    // fused SDDMM: the B operand of block h, slot sdd_slot(h, r), features 32 ks + 8 kg ..
+    const uint8_t* sdr0 = Bs + deep_phys(sdd_slot(0, r)) * RS + 16 * kg;
+    const uint8_t* sdr1 = Bs + deep_phys(sdd_slot(1, r)) * RS + 16 * kg;
+    const float* part = nullptr;
+    if constexpr (FUSED) part = sd.rmw ? sd.D : nullptr;
 
     DeepRec m[MR];
     DeepFrag fr[DEP];
@@ -727,25 +770,30 @@ __device__ inline void run_deep(const PlanDev& p, __amdgpu_buffer_rsrc_t rsB, ui
     // prologue: records of steps 0 .. 2 DEP - 1, the first DEP staged, then the
     // first DEP steps' fragments and gathers
 #pragma unroll
-    for (int j = 0; j < MR; ++j) deep_rec<KIND>(p, first, end, n, j, lane, m[j]);
+    for (int j = 0; j < MR; ++j) deep_rec<KIND>(p, first, end, n, j / H, lane, m[j], part);
 #pragma unroll
-    for (int j = 0; j < DEP; ++j) deep_stage<KIND>(m[j], first, end, n, j, Ms + MS * j, lane);
+    for (int j = 0; j < DEP; ++j) deep_stage<KIND>(m[j], first, end, n, j / H, Ms + MS * j, lane, part != nullptr);
     if constexpr (KIND != 2) {
 #pragma unroll
         for (int j = 0; j < DEP; ++j) deep_frag<KIND>(p, first, end, n, j, lane, fr[j]);
     }
 #pragma unroll
-    for (int j = 0; j < DEP; ++j) deep_gather<PW, R>(g[j], Ms + MS * j, rsB, ldb2, fp, fw, lane);
+    for (int j = 0; j < DEP; ++j)
+        deep_gather<PW, R>(g[j], Ms + MS * j, rsB, ldb2, fp + PW * (j % H), fw_of(j % H), lane);
+    f32x4 sacc[2];                                         // fused SDDMM: S of the step's two blocks
 
-    auto body = [&](int s, auto par_tag) {
+    auto body = [&](int v, auto par_tag) {
         constexpr int PAR = decltype(par_tag)::value;
         constexpr int GB = PAR % DEP;
-        constexpr int NXT = (PAR + DEP) % MR;              // ring slot of step s + DEP
+        constexpr int NXT = (PAR + DEP) % MR;              // ring slot of virtual step v + DEP
+        constexpr int HALF = PAR % H;                      // (U is a multiple of H: v % H == PAR % H)
+        constexpr int NHALF = (PAR + DEP) % H;             // half of virtual step v + DEP
+        const int s = v / H;                               // the records' step
         __builtin_amdgcn_sched_barrier(0);
         deep_park<PW, R>(g[GB], Bs, lane);
-        deep_stage<KIND>(m[NXT], first, end, n, s + DEP, Ms + MS * NXT, lane);
-        deep_rec<KIND>(p, first, end, n, s + 2 * DEP, lane, m[(PAR + 2 * DEP) % MR]);
-        deep_gather<PW, R>(g[GB], Ms + MS * NXT, rsB, ldb2, fp, fw, lane);
+        deep_stage<KIND>(m[NXT], first, end, n, (v + DEP) / H, Ms + MS * NXT, lane, part != nullptr);
+        deep_rec<KIND>(p, first, end, n, (v + 2 * DEP) / H, lane, m[(PAR + 2 * DEP) % MR], part);
+        deep_gather<PW, R>(g[GB], Ms + MS * NXT, rsB, ldb2, fp + PW * NHALF, fw_of(NHALF), lane);
         X8 a;
         int idx = 0x4444;
         if constexpr (KIND == 2) {
@@ -771,10 +819,10 @@ __device__ inline void run_deep(const PlanDev& p, __amdgpu_buffer_rsrc_t rsB, ui
                 const s16x4 b2 = lds_tr16(ap + 16 * RS), b3 = lds_tr16(ap + 24 * RS);   // logical +32, +36
                 const s16x16 bb = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3],
                                    b2[0], b2[1], b2[2], b2[3], b3[0], b3[1], b3[2], b3[3]};
-                acc[nb] = mma_sparse<DT>(a, __builtin_bit_cast(X16, bb), acc[nb], idx);
+                acc[HALF][nb] = mma_sparse<DT>(a, __builtin_bit_cast(X16, bb), acc[HALF][nb], idx);
             } else {
                 const s16x8 bb = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
-                acc[nb] = mma_dense<DT>(a, __builtin_bit_cast(X8, bb), acc[nb]);
+                acc[HALF][nb] = mma_dense<DT>(a, __builtin_bit_cast(X8, bb), acc[HALF][nb]);
             }
         }
         // keep the transposed reads next to the matrix instruction that takes them
@@ -784,38 +832,79 @@ __device__ inline void run_deep(const PlanDev& p, __amdgpu_buffer_rsrc_t rsB, ui
             __builtin_amdgcn_sched_group_barrier(0x100, KIND == 1 ? 4 : 2, 0);   // DS reads
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                    // MFMA
         }
+        if constexpr (FUSED) {
+            // S = X_w G^T for the step's two 16-slot blocks; lane (kg, r) ends with
+            // S[4 kg + i][slot sdd_slot(h, r)], i < 4, and keeps the one of its record's row
+            const uint8_t* slot = Ms + MS * (PAR % MR);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint8_t* sdr = h == 0 ? sdr0 : sdr1;
+                if constexpr (HALF == 0) sacc[h] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int ks = 0; ks < PW / 32; ++ks)
+                    sacc[h] = mma_dense<DT>(__builtin_bit_cast(X8, sd.xa[HALF * (PW / 32) + ks]),
+                                            *(const X8*)(sdr + 64 * ks), sacc[h]);
+            }
+            if constexpr (HALF == H - 1) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const f32x4 sa = sacc[h];
+                const int L = sdd_slot(h, r);
+                const uint32_t hw = *(const uint32_t*)(slot + 8 * L + 4);
+                const int i = (int)((hw >> 16) & 0xffu) - 4 * kg;
+                float v = i == 0 ? sa[0] : (i == 1 ? sa[1] : (i == 2 ? sa[2] : sa[3]));
+                if (part) v += *(const float*)(slot + 256 + 8 * L);
+                // into the record's column word (no longer read): the step's 32
+                // values then leave in slot order, one 128-byte store
+                if ((unsigned)i < 4u) *(float*)(slot + 8 * L) = v;
+            }
+            const int32_t rec = first + 32 * s + lane;
+            const float v = *(const float*)(slot + 8 * (lane & 31));
+            if ((lane < 32) & (rec < end)) sd.D[rec] = v;
+            }
+        }
         if constexpr (KIND != 2) deep_frag<KIND>(p, first, end, n, s + DEP, lane, fr[GB]);
     };
-    for (int s = 0; s < n; s += U) {
-        body(s, std::integral_constant<int, 0>{});
-        if (s + 1 >= n) break;
-        body(s + 1, std::integral_constant<int, 1>{});
+    for (int v = 0; v < nv; v += U) {
+        body(v, std::integral_constant<int, 0>{});
+        if (v + 1 >= nv) break;
+        body(v + 1, std::integral_constant<int, 1>{});
         if constexpr (U >= 4) {
-            if (s + 2 >= n) break;
-            body(s + 2, std::integral_constant<int, 2>{});
-            if (s + 3 >= n) break;
-            body(s + 3, std::integral_constant<int, 3>{});
+            if (v + 2 >= nv) break;
+            body(v + 2, std::integral_constant<int, 2>{});
+            if (v + 3 >= nv) break;
+            body(v + 3, std::integral_constant<int, 3>{});
         }
         if constexpr (U == 8) {
-            if (s + 4 >= n) break;
-            body(s + 4, std::integral_constant<int, 4>{});
-            if (s + 5 >= n) break;
-            body(s + 5, std::integral_constant<int, 5>{});
-            if (s + 6 >= n) break;
-            body(s + 6, std::integral_constant<int, 6>{});
-            if (s + 7 >= n) break;
-            body(s + 7, std::integral_constant<int, 7>{});
+            if (v + 4 >= nv) break;
+            body(v + 4, std::integral_constant<int, 4>{});
+            if (v + 5 >= nv) break;
+            body(v + 5, std::integral_constant<int, 5>{});
+            if (v + 6 >= nv) break;
+            body(v + 6, std::integral_constant<int, 6>{});
+            if (v + 7 >= nv) break;
+            body(v + 7, std::integral_constant<int, 7>{});
         }
     }
 }
 
 // one work item: passes of PW features; per pass the routes' steps, then the
 // 16-row tile written (full lines through LDS, or atomically for split windows)
-template <int PW, typename DT>
+// fused SDDMM operand of a launch (X = null: none)
+struct SddArgs {
+    const uint16_t* X;     // [x_rows, ldx] 16-bit, rows indexed like the output rows
+    int64_t ldx, x_rows;
+    float* D;              // one float per residual record
+};
+
+// FH: 0 = the plain multiply; 1 or 2 = with the fused SDDMM, passes of FH halves
+template <int PW, typename DT, int FH = 0>
 __device__ __forceinline__ void deep_item(const PlanDev& p, int64_t it, const uint16_t* B, int64_t ldb,
                                           int K, float* __restrict__ C, int64_t ldc, uint32_t mask, uint8_t* Bs,
-                                          uint8_t* Ms, int lane) {
+                                          uint8_t* Ms, int lane, const SddArgs& sa = SddArgs{}) {
     constexpr int NB = PW / 16;
+    constexpr bool FUSED = FH > 0;
+    constexpr int H = FUSED ? FH : 1;
     const int4 item = p.titems[it];
     const int64_t w = item.x;
     const bool split = item.y != 0;
@@ -842,35 +931,59 @@ __device__ __forceinline__ void deep_item(const PlanDev& p, int64_t it, const ui
     const __amdgpu_buffer_rsrc_t rsB =
         __builtin_amdgcn_make_buffer_rsrc((void*)B, (short)0, (int)(uint32_t)(2 * p.ncols * ldb), (int)kBufferWord3);
     const uint32_t ldb2 = 2u * (uint32_t)ldb;
-    for (int fp = 0; fp < K; fp += PW) {
-        f32x4 acc[NB];
+    for (int fp = 0; fp < K; fp += H * PW) {
+        f32x4 acc[H][NB];
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int hf = 0; hf < H; ++hf)
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) acc[hf][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
         // the item's bounds, re-read in each pass: keeps the compiler from hoisting the
         // steps' address arithmetic out of the pass loop into long-lived registers
         int32_t bs0 = s0, bs1 = s1, bt0 = t0, bt1 = t1, be0 = e0, be1 = e1;
         asm volatile("" : "+s"(bs0), "+s"(bs1), "+s"(bt0), "+s"(bt1), "+s"(be0), "+s"(be1));
-        if (nts > 0) run_deep<0, PW, DT>(p, rsB, ldb2, K, fp, bs0, bs1, nts, Bs, Ms, lane, acc);
-        if (ntt > 0) run_deep<1, PW, DT>(p, rsB, ldb2, K, fp, bt0, bt1, ntt, Bs, Ms, lane, acc);
-        if (nte > 0) run_deep<2, PW, DT>(p, rsB, ldb2, K, fp, be0, be1, nte, Bs, Ms, lane, acc);
-        const int width = K - fp < PW ? K - fp : PW;          // features of this pass (multiple of 8)
+        if constexpr (FUSED) {
+            // the plan holds residual records only (checked by the launcher)
+            Sdd<PW, H> sd;
+            sd.D = sa.D;
+            sd.rmw = fp > 0;
+            const __amdgpu_buffer_rsrc_t rsX = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)sa.X, (short)0, (int)(uint32_t)(2 * sa.x_rows * sa.ldx), (int)kBufferWord3);
+            const bool rok = row0 + r < sa.x_rows;
+#pragma unroll
+            for (int ks = 0; ks < H * PW / 32; ++ks) {
+                const int f = fp + 32 * ks + 8 * kg;
+                const uint32_t off = (rok & (f < K)) ? (uint32_t)(row0 + r) * (uint32_t)(2 * sa.ldx) + 2u * (uint32_t)f
+                                                     : kOutOfRange;
+                sd.xa[ks] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rsX, off, 0, 0));
+            }
+            if (nte > 0) run_deep<2, PW, DT, H>(p, rsB, ldb2, K, fp, be0, be1, nte, Bs, Ms, lane, acc, sd);
+        } else {
+            if (nts > 0) run_deep<0, PW, DT>(p, rsB, ldb2, K, fp, bs0, bs1, nts, Bs, Ms, lane, acc);
+            if (ntt > 0) run_deep<1, PW, DT>(p, rsB, ldb2, K, fp, bt0, bt1, ntt, Bs, Ms, lane, acc);
+            if (nte > 0) run_deep<2, PW, DT>(p, rsB, ldb2, K, fp, be0, be1, nte, Bs, Ms, lane, acc);
+        }
+#pragma unroll
+        for (int hf = 0; hf < H; ++hf) {
+        const int fh = fp + PW * hf;                           // this half's first feature
+        if (fh >= K) break;
+        const int width = K - fh < PW ? K - fh : PW;          // features of this half (multiple of 8)
         // the output stride, re-read here for the same reason as the bounds above
         uint32_t ldc32 = (uint32_t)ldc;
         asm volatile("" : "+s"(ldc32));
-        // accumulator element i of lane (kg, n) = C[16w + 4kg + i, fp + 16nb + n]
+        // accumulator element i of lane (kg, n) = C[16w + 4kg + i, fh + 16nb + n]
         if (!split) {
             float* T = reinterpret_cast<float*>(Bs);           // [16][PW] fp32, the staging space is free
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) T[(kg * 4 + i) * PW + nb * 16 + r] = acc[nb][i];
+                for (int i = 0; i < 4; ++i) T[(kg * 4 + i) * PW + nb * 16 + r] = acc[hf][nb][i];
             // store k of lane l writes 16 bytes of row prow + RPS k, piece l % (PW / 4):
             // one per-lane base and uniform strides (addresses kept out of the loops above)
             constexpr int PPR = PW / 4, RPS = 64 / PPR;
             const int prow = lane / PPR, piece = lane % PPR;
             const bool pok = piece < width / 4;
             const float* tsrc = T + prow * PW + 4 * piece;
-            float* cdst = Crow + (uint32_t)prow * ldc32 + (uint32_t)(fp + 4 * piece);
+            float* cdst = Crow + (uint32_t)prow * ldc32 + (uint32_t)(fh + 4 * piece);
             const uint32_t cstep = (uint32_t)RPS * ldc32;
 #pragma unroll
             for (int k = 0; k < 16 / RPS; ++k) {
@@ -880,31 +993,34 @@ __device__ __forceinline__ void deep_item(const PlanDev& p, int64_t it, const ui
                 }
             }
         } else {
-            float* adst = Crow + (uint32_t)(4 * kg) * ldc32 + (uint32_t)(fp + r);
+            float* adst = Crow + (uint32_t)(4 * kg) * ldc32 + (uint32_t)(fh + r);
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) {
                 const int f = nb * 16 + r;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    if ((kg * 4 + i < rows_out) & (f < width) & (acc[nb][i] != 0.f))
-                        unsafeAtomicAdd(adst + (uint32_t)i * ldc32 + 16 * nb, acc[nb][i]);
+                    if ((kg * 4 + i < rows_out) & (f < width) & (acc[hf][nb][i] != 0.f))
+                        unsafeAtomicAdd(adst + (uint32_t)i * ldc32 + 16 * nb, acc[hf][nb][i]);
                 }
             }
+        }
         }
     }
 }
 
 // one workgroup (one wave) per work item, in the plan's item order (see
 // k_tile_routes); 19 KB of LDS per wave at PW = 128, two waves per SIMD
-template <int PW, typename DT>
+template <int PW, typename DT, int FH = 0>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void k_routes(PlanDev p, const uint16_t* B, int64_t ldb, int K, float* __restrict__ C, int64_t ldc,
-              uint32_t mask, int64_t grid) {
-    __shared__ __align__(16) uint8_t Bs[64 * DeepCfg<PW>::RS];
+              uint32_t mask, int64_t grid, SddArgs sa) {
+    // staged rows: 64 (2:4 steps), 32 when only residual steps run (the fused SDDMM)
+    constexpr int ROWS = FH > 0 ? 32 : 64;
+    __shared__ __align__(16) uint8_t Bs[ROWS * DeepCfg<PW>::RS];
     __shared__ __align__(16) uint8_t Ms[4 * 512];          // record ring: up to 4 slots of 64 x 8 bytes
-    static_assert(16 * PW * 4 <= 64 * DeepCfg<PW>::RS, "the output tile must fit the staging space");
+    static_assert(16 * PW * 4 <= ROWS * DeepCfg<PW>::RS, "the output tile must fit the staging space");
     const int64_t it = p.item_order == 1 ? xcd_window(blockIdx.x, grid) : (int64_t)blockIdx.x;
-    if (it < p.n_titems) deep_item<PW, DT>(p, it, B, ldb, K, C, ldc, mask, Bs, Ms, threadIdx.x);
+    if (it < p.n_titems) deep_item<PW, DT, FH>(p, it, B, ldb, K, C, ldc, mask, Bs, Ms, threadIdx.x, sa);
 }
 
 // ============================================================ residual route
@@ -1128,10 +1244,10 @@ int launch_typed(const PlanDev& p, const void* B, int64_t ldb, int64_t K, float*
         if (p.tile_kernel == 0 && p.item_order != 2 && p.ncols * ldb < ((int64_t)1 << 30)) {
             if (K <= 64)
                 hipLaunchKernelGGL((k_routes<64, DT>), dim3((unsigned)grid), dim3(64), 0, s, p, (const uint16_t*)B, ldb,
-                                   (int)K, C, ldc, mask, grid);
+                                   (int)K, C, ldc, mask, grid, SddArgs{});
             else
                 hipLaunchKernelGGL((k_routes<128, DT>), dim3((unsigned)grid), dim3(64), 0, s, p, (const uint16_t*)B,
-                                   ldb, (int)K, C, ldc, mask, grid);
+                                   ldb, (int)K, C, ldc, mask, grid, SddArgs{});
         } else if (p.slab == 128)
             hipLaunchKernelGGL((k_tile_routes<128, DT>), dim3((unsigned)grid), dim3(64), 0, s, p, (const uint16_t*)B,
                                ldb, (int)K, C, ldc, mask, grid);
@@ -1159,7 +1275,54 @@ int launch_typed(const PlanDev& p, const void* B, int64_t ldb, int64_t K, float*
     return TRSPMM_OK;
 }
 
+template <typename DT>
+int launch_fused_typed(const PlanDev& p, const void* dY, int64_t lddy, int64_t K, float* dB, int64_t lddb,
+                       const SddArgs& sa, hipStream_t s) {
+    if (p.n_split > 0) {
+        hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)p.n_split), dim3(256), 0, s, p.split_windows, p.m_out, (int)K,
+                           dB, lddb);
+        TRSPMM_HIP(hipGetLastError());
+    }
+    if (p.n_titems == 0) return TRSPMM_OK;
+    const int64_t grid = xcd_grid(p.n_titems);
+    TRSPMM_CHECK(grid <= 0x7fffffff, "too many work items");
+    // K <= 256 is one pass: 128-feature halves, each step's rows gathered per half
+    if (K <= 64)
+        hipLaunchKernelGGL((k_routes<64, DT, 1>), dim3((unsigned)grid), dim3(64), 0, s, p, (const uint16_t*)dY, lddy,
+                           (int)K, dB, lddb, (uint32_t)TRSPMM_ROUTE_ALL, grid, sa);
+    else if (K <= 128)
+        hipLaunchKernelGGL((k_routes<128, DT, 1>), dim3((unsigned)grid), dim3(64), 0, s, p, (const uint16_t*)dY,
+                           lddy, (int)K, dB, lddb, (uint32_t)TRSPMM_ROUTE_ALL, grid, sa);
+    else
+        hipLaunchKernelGGL((k_routes<128, DT, 2>), dim3((unsigned)grid), dim3(64), 0, s, p, (const uint16_t*)dY,
+                           lddy, (int)K, dB, lddb, (uint32_t)TRSPMM_ROUTE_ALL, grid, sa);
+    TRSPMM_HIP(hipGetLastError());
+    return TRSPMM_OK;
+}
+
 }  // namespace
+
+int fused_sddmm_supported(const PlanDev& p) {
+    return p.n_tsub == 0 && p.n_tiles == 0 && p.res_mfma && p.tile_kernel == 0 && p.item_order != 2;
+}
+
+int launch_spmm_sddmm(const PlanDev& p, const void* dY, int64_t lddy, int64_t K, float* dB, int64_t lddb,
+                      const void* X, int64_t ldx, int64_t x_rows, float* D, hipStream_t s) {
+    TRSPMM_CHECK(fused_sddmm_supported(p), "fused SDDMM: the plan must hold residual records only, on the matrix-core "
+                                           "engine, with the default tile kernel and item order");
+    TRSPMM_CHECK(K > 0 && K % 8 == 0 && K <= (int64_t)1 << 24, "K must be a positive multiple of 8");
+    TRSPMM_CHECK(lddy >= K && lddy % 8 == 0 && ldx >= K && ldx % 8 == 0, "dY and X strides must be >= K, multiples of 8");
+    TRSPMM_CHECK(lddb >= K && lddb % 4 == 0, "dB stride must be >= K and a multiple of 4");
+    TRSPMM_CHECK(((uintptr_t)dY & 15) == 0 && ((uintptr_t)dB & 15) == 0 && ((uintptr_t)X & 15) == 0,
+                 "dY, dB and X must be 16-byte aligned");
+    TRSPMM_CHECK(p.ncols * lddy < ((int64_t)1 << 30) && x_rows * ldx < ((int64_t)1 << 30),
+                 "fused SDDMM: dY and X must hold fewer than 2^30 elements");
+    TRSPMM_CHECK(p.n_rval == 0 || D != nullptr, "fused SDDMM: D must not be null");
+    if (p.windows == 0) return TRSPMM_OK;
+    const SddArgs sa{(const uint16_t*)X, ldx, x_rows, D};
+    if (p.dtype == TRSPMM_BF16) return launch_fused_typed<BF16>(p, dY, lddy, K, dB, lddb, sa, s);
+    return launch_fused_typed<F16>(p, dY, lddy, K, dB, lddb, sa, s);
+}
 
 int launch_spmm(const PlanDev& p, const void* B, int64_t ldb, int64_t K, float* C, int64_t ldc, uint32_t mask,
                 hipStream_t s) {
