@@ -176,7 +176,11 @@ typedef struct trspmm_plan_opts {
 } trspmm_plan_opts;
 
 /* Builds the device plan (AMD lane layouts) from reference-format metadata.
- * m_out: output rows (mOri); ncols: rows of B (kOri); dtype: TRSPMM_F16/BF16. */
+ * m_out: output rows (mOri); ncols: rows of B (kOri); dtype: TRSPMM_F16/BF16.
+ * view->window in [1, 16]: at 16 the routes keep their tiles; a narrower
+ * window's nonzeros run as residual records of 16-row windows (2:4 rows
+ * window_index * window + r, TC rows t_window_row + r, as the reference's
+ * kernels read them: mGCNkernel_online.cu:291-299). */
 int trspmm_plan_create(const trspmm_meta_view* view, int64_t m_out, int64_t ncols, int dtype,
                        void* stream, trspmm_plan** out);
 /* Same, from the router's own output. */
@@ -187,7 +191,8 @@ int trspmm_plan_create_ex(const trspmm_meta_view* view, int64_t m_out, int64_t n
                           const trspmm_plan_opts* opts, void* stream, trspmm_plan** out);
 int trspmm_plan_from_meta_ex(const trspmm_meta* meta, int64_t m_out, int64_t ncols, int dtype,
                              const trspmm_plan_opts* opts, void* stream, trspmm_plan** out);
-/* 2:4 route only, from the mma.sp packed tiles (fp16 values). */
+/* 2:4 route only, from the mma.sp packed tiles (fp16 values); tile row r of
+ * window w is output row w * window + r (r < window), window in [1, 16]. */
 int trspmm_plan_create_packed(const int32_t* s_tile_column, const int32_t* s_packed_a,
                               const int32_t* s_packed_meta, const int32_t* s_window_tile_offset,
                               int64_t n_tiles, int64_t n_windows, int32_t window, int64_t m_out,

@@ -40,7 +40,8 @@ def _parallel(a):
 
 @pytest.mark.parametrize("over, msg", [
     ({"t_value": None}, "unexpected dtype"),
-    ({"window": 8}, "window == 16"),
+    ({"window": 8}, "s_value must have shape"),       # window 8 runs, but not on 16-row metadata
+    ({"window": 17}, "0 < window <= 16"),
     ({"window": 0}, "window"),
     ({"epoches": 0}, "epoches must be positive"),
     ({"dimN": 0}, "dimN must be positive"),

@@ -86,7 +86,7 @@ def _ref_product(arrays, x, mOri, window=16, routes="all"):
     return oracle.spmm64(arr, x.float().numpy(), mOri, window)
 
 
-PARALLEL = [p for p in GOLDEN if "w8" not in p]
+PARALLEL = GOLDEN          # every fixture, the window-8 one included
 
 
 @pytest.mark.parametrize("path", PARALLEL, ids=[os.path.basename(p)[7:-4] for p in PARALLEL])
@@ -102,6 +102,7 @@ def test_libra6_parallel_op(cuda, path, dimN, monkeypatch):
     mOri = z["rowptr"].size - 1
     kOri = int(z["col"].max()) + 1
     x = _rhs(kOri, dimN)
+    window = kw.get("window", 16)
     res = Libra6SpMMOnline.forward_fp16_tcu_cuda_sptc_mma_parallel_online(
         d["t_row_offset"], d["t_block_offset"], d["t_column"], d["t_value"], d["t_window_row"], d["t_atomic"],
         d["t_binary"],
@@ -109,19 +110,19 @@ def test_libra6_parallel_op(cuda, path, dimN, monkeypatch):
         d["c_row_offset_short"], d["c_row_short"], d["c_atomic_short"], d["c_column_short"], d["c_value_short"],
         d["s_column"], d["s_value"], d["s_pos"], d["s_offsets"], d["s_tile_window"], d["s_tile_column"],
         d["s_packed_a"], d["s_packed_meta"], d["s_window_tile_offset"], x,
-        d["parts_t"], d["parts_c"], kw.get("partsize_c", 32), d["parts_c_short"], 16, dimN, mOri, kOri, 3)
+        d["parts_t"], d["parts_c"], kw.get("partsize_c", 32), d["parts_c_short"], window, dimN, mOri, kOri, 3)
     assert len(res) == 6
     out = res[0]
     assert out.device.type == "cpu" and out.dtype == torch.float32 and tuple(out.shape) == (mOri, dimN)
     for ms in res[1:]:
         assert ms.dtype == torch.float32 and ms.dim() == 0 and float(ms) >= 0
-    ref = _ref_product(d, x, mOri)
+    ref = _ref_product(d, x, mOri, window)
     err = np.linalg.norm(out.numpy() - ref) / max(np.linalg.norm(ref), 1e-30)
     assert err < TOL, err
 
 
 @pytest.mark.parametrize("op", ["sptc", "sptc_mma", "sptc_mma_packed"])
-@pytest.mark.parametrize("path", [p for p in PARALLEL if "er512" in p or "hub" in p])
+@pytest.mark.parametrize("path", [p for p in PARALLEL if "er512" in p or "hub" in p or "w8" in p])
 def test_libra6_sptc_ops(cuda, op, path):
     import Libra6SpMMOnline
     z, kw, env, _ = _fixture(path)
@@ -131,18 +132,19 @@ def test_libra6_sptc_ops(cuda, op, path):
     kOri = int(z["col"].max()) + 1
     dimN = 32
     x = _rhs(kOri, dimN, seed=7)
+    window = kw.get("window", 16)
     if op == "sptc":
         out, ms = Libra6SpMMOnline.forward_fp16_sptc_online(d["s_column"], d["s_value"], d["s_pos"], d["s_window"],
-                                                             x, 16, dimN, mOri, kOri, 2)
+                                                             x, window, dimN, mOri, kOri, 2)
     elif op == "sptc_mma":
         out, ms = Libra6SpMMOnline.forward_fp16_sptc_mma_online(d["s_column"], d["s_value"], d["s_pos"],
-                                                                 d["s_offsets"], x, 16, dimN, mOri, kOri, 2)
+                                                                 d["s_offsets"], x, window, dimN, mOri, kOri, 2)
     else:
         out, ms = Libra6SpMMOnline.forward_fp16_sptc_mma_packed_online(
-            d["s_tile_column"], d["s_packed_a"], d["s_packed_meta"], d["s_window_tile_offset"], x, 16, dimN, mOri,
-            kOri, 2)
+            d["s_tile_column"], d["s_packed_a"], d["s_packed_meta"], d["s_window_tile_offset"], x, window, dimN,
+            mOri, kOri, 2)
     assert tuple(out.shape) == (mOri, dimN) and out.dtype == torch.float32 and float(ms) >= 0
-    ref = _ref_product(d, x, mOri, routes="sptc")
+    ref = _ref_product(d, x, mOri, window, routes="sptc")
     err = np.linalg.norm(out.numpy() - ref) / max(np.linalg.norm(ref), 1e-30)
     assert err < TOL, err
 

@@ -119,3 +119,31 @@ def test_exact_mode_vs_fp64_csr(cuda):
                             B.float().numpy(), csr.m)
     err = np.linalg.norm(C - ref) / np.linalg.norm(ref)
     assert err < TOL_FRO, err
+
+
+@pytest.mark.parametrize("window", [1, 5, 8, 12])
+@pytest.mark.parametrize("case", ["dense", "ragged", "rmat10", "hub"])
+@pytest.mark.parametrize("mode", ["compat", "exact"])
+def test_narrow_windows(cuda, window, case, mode):
+    """Windows narrower than 16 rows (the reference takes any window in (0, 16]):
+    the metadata's nonzeros, TC bits included (TC rows t_window_row + r), run
+    as residual records of 16-row windows; windows of 5 and 12 rows straddle
+    the 16-row tiles.  Against the FP64 product of the oracle's decode of the
+    same window's metadata.  The reference codes a TC part's rows from
+    16 * window_id whatever the window (block_online.cpp:494, read so by
+    mGCNkernel_online.cu:207-209), so the CSR product itself is expected only
+    where no TC part was emitted."""
+    csr = CASES[case]()
+    print(f"[narrow] {case} window={window} mode={mode}", flush=True)
+    K = 64
+    A = RoutedSparse(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val), csr.n,
+                     m=csr.m, mode=mode, window=window)
+    Bt = torch.from_numpy(np.random.default_rng(window).standard_normal((csr.n, K)).astype(np.float32)).half()
+    got = A.matmul(Bt.cuda()).cpu().numpy()
+    meta = oracle.route(csr.rowptr, csr.col, csr.val, window=window, exact=mode != "compat")
+    ref = oracle.spmm64(meta, Bt.float().numpy(), csr.m, window)
+    assert np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-30) < TOL_FRO
+    if mode == "exact" and meta["t_window_row"].size == 0:     # every nonzero kept where it is
+        vq = torch.from_numpy(csr.val).half().double().numpy()
+        full = oracle.csr_spmm64(csr.rowptr, csr.col, vq, Bt.double().numpy(), csr.m)
+        assert np.linalg.norm(got - full) / max(np.linalg.norm(full), 1e-30) < TOL_FRO

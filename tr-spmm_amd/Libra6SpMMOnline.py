@@ -22,10 +22,16 @@ ms_cuda_* time the same launch restricted to one route; the residual route
 serves the long and short part lists together, so ms_cuda_long and
 ms_cuda_short report the same residual-route time.
 
+Any window in (0, 16] runs, as in the reference.  At window 16 the routes
+keep their tiles (16-row MFMA tiles); a narrower window's nonzeros (TC bits,
+kept 2:4 slots, residual lists, packed tiles) run as residual records of
+16-row windows, each at the row the reference's kernels give it (2:4 and
+packed rows window_index * window + r, TC rows t_window_row + r).
+
 Stricter than the reference (RuntimeError instead of undefined behaviour):
-window must be 16 (16-row MFMA tiles), part counts must match the tensors they
-describe, offsets must be monotone and in range, TC parts and residual parts
-must be in window / row order (as the router emits them).
+part counts must match the tensors they describe, offsets must be monotone
+and in range, TC parts and residual parts must be in window / row order (as
+the router emits them).
 """
 from __future__ import annotations
 
@@ -56,8 +62,6 @@ def _dtype(t, dt, name):
 def _common_scalars(window, dimN, mOri, kOri, epoches, rhs):
     if not (0 < window <= 16):
         raise RuntimeError("parallel mma path requires 0 < window <= 16")
-    if window != 16:
-        raise RuntimeError("the routed multiply requires window == 16 (16-row MFMA tiles)")
     for v, n in ((dimN, "dimN"), (mOri, "mOri"), (kOri, "kOri"), (epoches, "epoches")):
         if int(v) <= 0:
             raise RuntimeError(f"{n} must be positive")
@@ -97,12 +101,14 @@ def _run(plan: Plan, B, k8, mOri, dimN, epoches, masks):
     return out[:, : int(dimN)].cpu().contiguous(), times
 
 
-def _plan_from_view(view_tensors, n_windows, mOri, kOri, keep, dimN):
+def _plan_from_view(view_tensors, n_windows, mOri, kOri, keep, dimN, window=16):
+    """Windows narrower than 16 rows run as residual records of 16-row windows
+    (the plan builder decodes the metadata's rows; see trspmm_plan_create)."""
     v = _lib.MetaView()
     for k, val in view_tensors.items():
         setattr(v, k, val)
     v.n_windows = n_windows
-    v.window = 16
+    v.window = int(window)
     opts = _lib.PlanOpts()
     opts.k_hint = int(dimN)                   # the op runs at one K: size the work items for it
     h = ctypes.c_void_p()
@@ -203,9 +209,9 @@ def forward_fp16_tcu_cuda_sptc_mma_parallel_online(
         if valt.numel() != colt.numel():
             raise RuntimeError(f"{pre}: column and value counts differ")
         _nondecreasing(rows, pre)
-        if rows.numel() and (int(rows.min()) < 0 or int(rows.max()) >= 16 * W):
+        if rows.numel() and (int(rows.min()) < 0 or int(rows.max()) >= int(window) * W):
             raise RuntimeError(f"{pre} rows must lie inside the windows")
-    if 16 * W < int(mOri):
+    if int(window) * W < int(mOri):
         raise RuntimeError("the windows of s_offsets must cover mOri rows")
 
     D = {n: _dev(T[n]) for n in names if n != "rhs_matrix"}
@@ -222,7 +228,7 @@ def forward_fp16_tcu_cuda_sptc_mma_parallel_online(
         "s_column": ptr(D["s_column"]), "n_groups": G, "s_value": ptr(D["s_value"]), "s_pos": ptr(D["s_pos"]),
         "s_offsets": ptr(D["s_offsets"]), "t_value_is_f32": 0, "c_value_is_f32": 0,
     }
-    plan = _plan_from_view(view, W, mOri, kOri, keep=tuple(D.values()), dimN=dimN)
+    plan = _plan_from_view(view, W, mOri, kOri, keep=tuple(D.values()), dimN=dimN, window=window)
     B, k8 = _operand(T["rhs_matrix"], kOri, dimN)
     out, (ms, ms_tc, ms_res, ms_sptc) = _run(plan, B, k8, mOri, dimN, epoches,
                                              [_lib.ROUTE_ALL, _lib.ROUTE_TC, _lib.ROUTE_RESIDUAL, _lib.ROUTE_SPTC])
@@ -233,13 +239,13 @@ def forward_fp16_tcu_cuda_sptc_mma_parallel_online(
 def _sptc_only(sc, sv, sp, so, rhs, window, dimN, mOri, kOri, epoches):
     G = sc.size(0)
     W = so.numel() - 1
-    if 16 * W < int(mOri):
+    if int(window) * W < int(mOri):
         raise RuntimeError("the windows of s_offsets must cover mOri rows")
     z = torch.zeros(1, dtype=torch.int32, device="cuda")
     Dsc, Dsv, Dsp, Dso = _dev(sc), _dev(sv), _dev(sp), _dev(so)
     view = {"t_row_offset": ptr(z), "t_block_offset": ptr(z), "c_row_offset": ptr(z), "cs_row_offset": ptr(z),
             "s_column": ptr(Dsc), "n_groups": G, "s_value": ptr(Dsv), "s_pos": ptr(Dsp), "s_offsets": ptr(Dso)}
-    plan = _plan_from_view(view, W, mOri, kOri, keep=(z, Dsc, Dsv, Dsp, Dso), dimN=dimN)
+    plan = _plan_from_view(view, W, mOri, kOri, keep=(z, Dsc, Dsv, Dsp, Dso), dimN=dimN, window=window)
     B, k8 = _operand(rhs, kOri, dimN)
     out, (ms,) = _run(plan, B, k8, mOri, dimN, epoches, [_lib.ROUTE_ALL])
     return [out, torch.tensor(ms, dtype=torch.float32)]
@@ -271,7 +277,7 @@ def forward_fp16_sptc_online(s_column, s_value, s_pos, s_window, rhs_matrix, win
     if sw.dim() != 1 or sw.size(0) != sc.size(0):
         raise RuntimeError("s_column group count mismatch")
     # groups may come in any window order here (each is an independent product term)
-    W = max((int(mOri) + 15) // 16, int(sw.max()) + 1 if sw.numel() else 0)
+    W = max((int(mOri) + int(window) - 1) // int(window), int(sw.max()) + 1 if sw.numel() else 0)
     if sw.numel() and int(sw.min()) < 0:
         raise RuntimeError("s_window entries must be non-negative")
     order = torch.argsort(sw, stable=True)
@@ -327,11 +333,12 @@ def forward_fp16_sptc_mma_packed_online(s_tile_column, s_packed_a, s_packed_meta
         raise RuntimeError("s_packed_meta tile count mismatch")
     _offsets(wo, T, "s_window_tile_offset")
     W = wo.numel() - 1
-    if 16 * W < int(mOri):
+    if int(window) * W < int(mOri):
         raise RuntimeError("the windows of s_window_tile_offset must cover mOri rows")
     Dtc, Dpa, Dpm, Dwo = _dev(tc), _dev(pa), _dev(pm), _dev(wo)
     h = ctypes.c_void_p()
-    check(lib().trspmm_plan_create_packed(ptr(Dtc), ptr(Dpa), ptr(Dpm), ptr(Dwo), T, W, 16, int(mOri), int(kOri),
+    check(lib().trspmm_plan_create_packed(ptr(Dtc), ptr(Dpa), ptr(Dpm), ptr(Dwo), T, W, int(window), int(mOri),
+                                          int(kOri),
                                           _lib.current_stream(), ctypes.byref(h)), "trspmm_plan_create_packed")
     plan = Plan(h, int(mOri), int(kOri), torch.float16, keep=(Dtc, Dpa, Dpm, Dwo))
     B, k8 = _operand(rhs, kOri, dimN)

@@ -328,6 +328,125 @@ __global__ void k_refresh_records(const int32_t* __restrict__ edge, int64_t n, N
     }
 }
 
+// ---- windows narrower than 16 rows: every stored nonzero becomes a residual
+// record.  Key of an entry: row << 32 | its index in the concatenation
+// (TC values, 2:4 slots, long residual entries, short residual entries), so
+// sorting the keys orders each row's entries in that concatenation order;
+// an empty 2:4 slot or a row past the plan gets row = rows (sorted last).
+struct NarrowSrc {
+    trspmm_meta_view v;
+    int64_t rows, ncols, n_t, n_s;
+    int dtype;
+    // packed 2:4 tiles (trspmm_plan_create_packed), after the residual lists
+    const int32_t *p_col, *p_wto;
+    const uint32_t *p_a, *p_meta;
+    int64_t n_ptiles, n_res;
+};
+
+__device__ inline int64_t part_of(const int32_t* off, int64_t parts, int64_t e) {
+    return lower_bound(off, parts + 1, (int32_t)(e + 1)) - 1;           // last part starting at or before e
+}
+
+__device__ inline void narrow_put(const NarrowSrc& S, int64_t i, int64_t row, int32_t col, uint16_t bits,
+                                  uint64_t* keys, int32_t* cols, uint16_t* vals) {
+    const bool ok = row >= 0 && row < S.rows;
+    keys[i] = ((uint64_t)(ok ? row : S.rows) << 32) | (uint64_t)(uint32_t)i;
+    cols[i] = clean_col(col, S.ncols);
+    vals[i] = bits;
+}
+
+// TC: one thread per sub-block, its bits in the order the values are stored
+// (rows t_window_row + half 8 + b / 8, column b % 8)
+__global__ void k_narrow_tc(NarrowSrc S, uint64_t* __restrict__ keys, int32_t* __restrict__ cols,
+                            uint16_t* __restrict__ vals) {
+    const int64_t sb = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (sb >= S.v.n_tsub) return;
+    const int64_t p = part_of(S.v.t_row_offset, S.v.n_tparts, sb);
+    const int64_t base = S.v.t_window_row[p];
+    int64_t vi = S.v.t_block_offset[sb];
+    for (int half = 0; half < 2; ++half) {
+        const uint64_t bits = (uint64_t)S.v.t_binary[2 * sb + half];
+        for (int b = 0; b < 64; ++b) {
+            if (!((bits >> b) & 1ull)) continue;
+            narrow_put(S, vi, base + half * 8 + b / 8, S.v.t_column[8 * sb + b % 8],
+                       to_plan_bits(S.v.t_value, vi, S.v.t_value_is_f32, S.dtype), keys, cols, vals);
+            ++vi;
+        }
+    }
+}
+
+// 2:4: one thread per (group, row, slot); the group's window from s_offsets
+__global__ void k_narrow_sptc(NarrowSrc S, uint64_t* __restrict__ keys, int32_t* __restrict__ cols,
+                              uint16_t* __restrict__ vals) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= S.n_s) return;
+    const int64_t g = b / (2 * S.v.window), lr = (b / 2) % S.v.window;
+    const int64_t w = part_of(S.v.s_offsets, S.v.n_windows, g);
+    const int q = S.v.s_pos[b];
+    const int64_t row = q >= 0 ? w * S.v.window + lr : -1;
+    const float x = S.v.s_value[b];
+    narrow_put(S, S.n_t + b, row, q >= 0 ? S.v.s_column[4 * g + q] : -1,
+               S.dtype == TRSPMM_BF16 ? f32_to_bf16_bits(x) : f32_to_f16_bits(x), keys, cols, vals);
+}
+
+// residual lists: one thread per entry (long list, then short list)
+__global__ void k_narrow_res(NarrowSrc S, uint64_t* __restrict__ keys, int32_t* __restrict__ cols,
+                             uint16_t* __restrict__ vals) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nl = S.v.n_cval, n = nl + S.v.n_csval;
+    if (e >= n) return;
+    const bool lng = e < nl;
+    const int64_t k = lng ? e : e - nl;
+    const int64_t p = lng ? part_of(S.v.c_row_offset, S.v.n_cparts, k) : part_of(S.v.cs_row_offset, S.v.n_csparts, k);
+    const int64_t row = lng ? S.v.c_row[p] : S.v.cs_row[p];
+    const int32_t col = lng ? S.v.c_column[k] : S.v.cs_column[k];
+    const uint16_t bits = to_plan_bits(lng ? S.v.c_value : S.v.cs_value, k, S.v.c_value_is_f32, S.dtype);
+    narrow_put(S, S.n_t + S.n_s + e, row, col, bits, keys, cols, vals);
+}
+
+// packed tiles (block_online.cpp:796-841 layout): one thread per (tile, row,
+// group), its two slots; tile row r < window is output row w window + r
+__global__ void k_narrow_packed(NarrowSrc S, uint64_t* __restrict__ keys, int32_t* __restrict__ cols,
+                                uint16_t* __restrict__ vals) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S.n_ptiles * 64) return;
+    const int64_t t = i >> 6;
+    const int r = (int)((i >> 2) & 15), c = (int)(i & 3);
+    const int64_t w = part_of(S.p_wto, S.v.n_windows, t);
+    const uint32_t a = S.p_a[((t * 8 + (r & 7)) * 4 + c) * 2 + (r >> 3)];
+    const uint32_t nib = (S.p_meta[t * 8 + (r & 7)] >> (16 * (r >> 3) + 4 * c)) & 0xfu;
+    const int64_t row = r < S.v.window ? w * S.v.window + r : -1;
+    for (int j = 0; j < 2; ++j) {
+        const int q = (int)((nib >> (2 * j)) & 3u);
+        const uint16_t h = (uint16_t)(a >> (16 * j));
+        const uint16_t bits = S.dtype == TRSPMM_F16 ? h
+                                                    : f32_to_bf16_bits((float)__builtin_bit_cast(_Float16, h));
+        narrow_put(S, S.n_t + S.n_s + S.n_res + 2 * i + j, row, S.p_col[t * 16 + 4 * c + q], bits, keys, cols, vals);
+    }
+}
+
+// rowptr[r] = first sorted key of row >= r, r = 0 .. rows
+__global__ void k_narrow_rowptr(const uint64_t* __restrict__ keys, int64_t n, int64_t rows,
+                                int32_t* __restrict__ rowptr) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r <= rows) rowptr[r] = (int32_t)lower_bound(keys, n, (uint64_t)r << 32);
+}
+
+__global__ void k_low_words(const uint64_t* __restrict__ keys, int64_t n, uint32_t* __restrict__ idx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) idx[i] = (uint32_t)keys[i];
+}
+
+__global__ void k_narrow_gather(const uint32_t* __restrict__ idx, int64_t n, const int32_t* __restrict__ cols,
+                                const uint16_t* __restrict__ vals, int32_t* __restrict__ out_cols,
+                                uint16_t* __restrict__ out_vals) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        out_cols[i] = cols[idx[i]];
+        out_vals[i] = vals[idx[i]];
+    }
+}
+
 inline unsigned blocks_for(int64_t n, int threads = 256) {
     int64_t b = (n + threads - 1) / threads;
     if (b < 1) b = 1;
@@ -575,9 +694,111 @@ static int tile_slab(const trspmm_plan_opts& o) {
     return (e && std::atoi(e) == 128) ? 128 : 64;
 }
 
+// Windows narrower than 16 rows (window in [1, 16)): the plan runs the routed
+// matrix as the metadata stores it, every nonzero (TC bits, kept 2:4 slots,
+// residual lists) a residual record of 16-row windows, each row's entries in
+// that order.  The reference's parallel op takes any window in (0, 16]
+// (mGCN_online.cpp:885); its 2:4 kernel reads rows s_window * window + r
+// (mGCNkernel_online.cu:291-299) and its TC kernel rows t_window_row + r.
+static int build_plan_narrow(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtype, hipStream_t s,
+                             Plan& P, const NarrowSrc* packed = nullptr) {
+    TRSPMM_CHECK(v.window >= 1 && v.window < 16, "narrow windows: window must be in [1, 16)");
+    TRSPMM_CHECK(v.n_windows >= 0 && v.n_windows * v.window >= m_out, "windows must cover the output rows");
+    const int64_t W = (std::max<int64_t>(m_out, v.n_windows * v.window) + 15) / 16;
+    const int64_t rows = 16 * W;
+    TRSPMM_CHECK(rows < ((int64_t)1 << 31), "too many rows");
+    P.dev.windows = W;
+    P.dev.m_out = m_out;
+    P.dev.ncols = ncols;
+    P.dev.window = 16;
+    P.dev.dtype = dtype;
+    P.dev.n_tsub = 0;
+    P.dev.n_tval = 0;
+    P.dev.n_tiles = 0;
+    const int64_t n_t = v.n_tval, n_s = 2 * v.n_groups * v.window, n_r = v.n_cval + v.n_csval;
+    const int64_t n_p = packed ? 128 * packed->n_ptiles : 0;
+    const int64_t n = n_t + n_s + n_r + n_p;
+    TRSPMM_CHECK(n < ((int64_t)1 << 31), "too many nonzeros");
+    NarrowSrc S{v, rows, ncols, n_t, n_s, dtype, nullptr, nullptr, nullptr, nullptr, 0, n_r};
+    if (packed) {
+        S.p_col = packed->p_col;
+        S.p_wto = packed->p_wto;
+        S.p_a = packed->p_a;
+        S.p_meta = packed->p_meta;
+        S.n_ptiles = packed->n_ptiles;
+    }
+    int rc;
+    if ((rc = P.tc_sub_off.alloc(4 * (W + 1))) || (rc = P.sp_tile_off.alloc(4 * (W + 1))) ||
+        (rc = P.res_rowptr.alloc(4 * (rows + 1))))
+        return rc;
+    TRSPMM_HIP(hipMemsetAsync(P.tc_sub_off.p, 0, 4 * (W + 1), s));
+    TRSPMM_HIP(hipMemsetAsync(P.sp_tile_off.p, 0, 4 * (W + 1), s));
+    DevBuf keys, keys2, idx, idx2, cols, vals, tmp;
+    const int64_t n1 = n > 0 ? n : 1;
+    if ((rc = keys.alloc(8 * n1)) || (rc = keys2.alloc(8 * n1)) || (rc = idx.alloc(4 * n1)) ||
+        (rc = idx2.alloc(4 * n1)) || (rc = cols.alloc(4 * n1)) || (rc = vals.alloc(2 * n1)))
+        return rc;
+    if (v.n_tsub > 0) {
+        hipLaunchKernelGGL(k_narrow_tc, dim3(blocks_for(v.n_tsub)), dim3(256), 0, s, S, keys.as<uint64_t>(),
+                           cols.as<int32_t>(), vals.as<uint16_t>());
+        TRSPMM_HIP(hipGetLastError());
+    }
+    if (n_s > 0) {
+        hipLaunchKernelGGL(k_narrow_sptc, dim3(blocks_for(n_s)), dim3(256), 0, s, S, keys.as<uint64_t>(),
+                           cols.as<int32_t>(), vals.as<uint16_t>());
+        TRSPMM_HIP(hipGetLastError());
+    }
+    if (n_r > 0) {
+        hipLaunchKernelGGL(k_narrow_res, dim3(blocks_for(n_r)), dim3(256), 0, s, S, keys.as<uint64_t>(),
+                           cols.as<int32_t>(), vals.as<uint16_t>());
+        TRSPMM_HIP(hipGetLastError());
+    }
+    if (n_p > 0) {
+        hipLaunchKernelGGL(k_narrow_packed, dim3(blocks_for(n_p / 2)), dim3(256), 0, s, S, keys.as<uint64_t>(),
+                           cols.as<int32_t>(), vals.as<uint16_t>());
+        TRSPMM_HIP(hipGetLastError());
+    }
+    int total = 0;
+    if (n > 0) {
+        int rbits = 1;
+        while (rbits < 31 && ((int64_t)1 << rbits) <= rows) ++rbits;
+        size_t tmp_bytes = 0;
+        TRSPMM_HIP(rocprim::radix_sort_keys(nullptr, tmp_bytes, keys.as<uint64_t>(), keys2.as<uint64_t>(), (size_t)n,
+                                            0, 32 + rbits, s));
+        if ((rc = tmp.alloc(tmp_bytes ? tmp_bytes : 1))) return rc;
+        TRSPMM_HIP(rocprim::radix_sort_keys(tmp.p, tmp_bytes, keys.as<uint64_t>(), keys2.as<uint64_t>(), (size_t)n,
+                                            0, 32 + rbits, s));
+    }
+    hipLaunchKernelGGL(k_narrow_rowptr, dim3(blocks_for(rows + 1)), dim3(256), 0, s, keys2.as<uint64_t>(), n, rows,
+                       P.res_rowptr.as<int32_t>());
+    TRSPMM_HIP(hipGetLastError());
+    TRSPMM_HIP(hipMemcpyAsync(&total, P.res_rowptr.as<int32_t>() + rows, 4, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipStreamSynchronize(s));
+    P.dev.n_rval = total;
+    if (total > 0) {
+        // the sorted keys' low words are the entries' indices
+        hipLaunchKernelGGL(k_low_words, dim3(blocks_for(total)), dim3(256), 0, s, keys2.as<uint64_t>(), (int64_t)total,
+                           idx.as<uint32_t>());
+        TRSPMM_HIP(hipGetLastError());
+        if ((rc = P.res_cols.alloc(4 * (int64_t)total)) || (rc = P.res_vals.alloc(2 * (int64_t)total)) ||
+            (rc = P.res_rec.alloc(8 * (int64_t)total)))
+            return rc;
+        hipLaunchKernelGGL(k_narrow_gather, dim3(blocks_for(total)), dim3(256), 0, s, idx.as<uint32_t>(),
+                           (int64_t)total, cols.as<int32_t>(), vals.as<uint16_t>(), P.res_cols.as<int32_t>(),
+                           P.res_vals.as<uint16_t>());
+        TRSPMM_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_res_records, dim3(blocks_for(rows, 4)), dim3(256), 0, s, P.res_rowptr.as<int32_t>(),
+                           rows, P.res_cols.as<int32_t>(), P.res_vals.as<uint16_t>(), P.res_rec.as<uint64_t>());
+        TRSPMM_HIP(hipGetLastError());
+        if (P.opts.res_order == 0 && (rc = sort_records_by_column(P, rows, total, s))) return rc;
+    }
+    TRSPMM_HIP(hipStreamSynchronize(s));   // temporaries are freed on return
+    return P.build_items(s);
+}
+
 int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtype, hipStream_t s, Plan& P,
                const EdgeMaps* em) {
-    TRSPMM_CHECK(v.window == 16, "the routed multiply requires window == 16 (16-row MFMA tiles)");
+    TRSPMM_CHECK(v.window >= 1 && v.window <= 16, "window must be in [1, 16]");
     TRSPMM_CHECK(P.opts.residual_engine >= 0 && P.opts.residual_engine <= TRSPMM_ENGINE_MFMA,
                  "plan options: residual_engine must be TRSPMM_ENGINE_DEFAULT, _VALU or _MFMA");
     TRSPMM_CHECK(P.opts.slab == 0 || P.opts.slab == 64 || P.opts.slab == 128, "plan options: slab must be 0, 64 or 128");
@@ -605,6 +826,7 @@ int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtyp
     P.dev.item_order = P.opts.item_order;
     TRSPMM_CHECK(dtype == TRSPMM_F16 || dtype == TRSPMM_BF16, "dtype must be TRSPMM_F16 or TRSPMM_BF16");
     TRSPMM_CHECK(m_out > 0 && ncols > 0, "m_out and ncols must be positive");
+    if (v.window != 16) return build_plan_narrow(v, m_out, ncols, dtype, s, P);     // (no edge maps)
     TRSPMM_CHECK(v.n_windows >= 0 && v.n_windows * 16 >= m_out, "windows must cover the output rows");
     const int64_t W = v.n_windows;
     P.dev.windows = W;
@@ -681,7 +903,23 @@ int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtyp
 int build_plan_packed(const int32_t* s_tile_column, const int32_t* s_packed_a, const int32_t* s_packed_meta,
                       const int32_t* s_window_tile_offset, int64_t T, int64_t W, int window, int64_t m_out,
                       int64_t ncols, hipStream_t s, Plan& P) {
-    TRSPMM_CHECK(window == 16, "the routed multiply requires window == 16 (16-row MFMA tiles)");
+    TRSPMM_CHECK(window >= 1 && window <= 16, "window must be in [1, 16]");
+    if (window != 16) {
+        // narrower windows: the tiles' rows w window + r (r < window) as residual records
+        TRSPMM_CHECK(m_out > 0 && ncols > 0, "m_out and ncols must be positive");
+        trspmm_meta_view v{};
+        v.window = window;
+        v.n_windows = W;
+        NarrowSrc pk{};
+        pk.p_col = s_tile_column;
+        pk.p_wto = s_window_tile_offset;
+        pk.p_a = (const uint32_t*)s_packed_a;
+        pk.p_meta = (const uint32_t*)s_packed_meta;
+        pk.n_ptiles = T;
+        P.dev.res_mfma = residual_on_matrix_cores(P.opts);
+        P.dev.slab = tile_slab(P.opts);
+        return build_plan_narrow(v, m_out, ncols, TRSPMM_F16, s, P, &pk);
+    }
     TRSPMM_CHECK(m_out > 0 && ncols > 0, "m_out and ncols must be positive");
     TRSPMM_CHECK(W * 16 >= m_out, "windows must cover the output rows");
     P.dev.windows = W;
