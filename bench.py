@@ -626,7 +626,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_record(a.workload, a.mode, a.dtype), "algorithmic_bytes": int(byts),
-                     "kernel": "k_tile_routes", "copy_gbs": round(copy_gbs, 1)},
+                     "kernel": "k_routes", "copy_gbs": round(copy_gbs, 1)},
     }
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(csr, K, B16)
