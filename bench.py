@@ -570,6 +570,11 @@ def main():
           file=sys.stderr, flush=True)
     C = torch.empty((csr.m, K), device=dev, dtype=torch.float32)
     stream = torch.cuda.current_stream()
+    # measured before the timed steps: the multiply after a MALL flush, and the
+    # device's plain copy bandwidth (the first few dozen multiplies after routing
+    # run ~5 % slower than later ones, and these runs cover that phase)
+    ms_cold = time_cold(lambda: A.matmul(Bt, C), max(5, min(a.steps, 20)))
+    copy_gbs = copy_bandwidth()
     for _ in range(a.warmup):
         A.matmul(Bt, C)
     torch.cuda.synchronize()
@@ -604,8 +609,6 @@ def main():
         ref = np.asarray(Af @ B16.astype(np.float64))[: csr.m]
         got = C.cpu().numpy().astype(np.float64)
         err = float(f"{np.linalg.norm(got - ref) / np.linalg.norm(ref):.3e}")
-    ms_cold = time_cold(lambda: A.matmul(Bt, C), max(5, min(a.steps, 20)))
-    copy_gbs = copy_bandwidth()
     detail = {"routes": A.plan.stats, "plan_opts": A.plan_opts}
     exact_ms = None
     if a.mode != "exact" and not a.no_extras:
