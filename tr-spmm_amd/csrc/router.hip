@@ -183,9 +183,26 @@ __device__ inline int run_count(const RunArrays& R, int32_t run) { return R.star
 // strictly larger keys, so it is a stable sort by key: a counting sort
 // through `tmp`.  A range whose depth budget runs out is heap-sorted by lane
 // 0, as lsort::sort does.  `tmp` needs n entries of global memory.
+struct PartLds {
+    int32_t kp, cut_l;
+};
 struct WaveSortLds {
     int32_t base[256];     // per-key output offsets of the final pass
-    int32_t kp, cut_l;
+    PartLds part;
+};
+
+// Barriers of the sort: a one-wave workgroup uses the workgroup barrier; a
+// wave of a larger workgroup working on a range of its own syncs only itself
+// (its memory operations complete and become visible to its own lanes).
+struct BlockSync {
+    __device__ void operator()() const { __syncthreads(); }
+};
+struct WaveSync {
+    __device__ void operator()() const {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
 };
 
 __device__ inline int32_t wave_max_i32(int32_t v) {
@@ -193,9 +210,11 @@ __device__ inline int32_t wave_max_i32(int32_t v) {
     return v;
 }
 
+template <typename Sync = BlockSync>
 __device__ int32_t wave_partition(uint32_t* a, int32_t first, int32_t last, int32_t* __restrict__ tmp,
-                                  WaveSortLds& sl) {
-    const int lane = threadIdx.x;
+                                  PartLds& sl) {
+    const Sync sync{};
+    const int lane = threadIdx.x & (kWave - 1);
     if (lane == 0) {
         // median of (first + 1, mid, last - 1) moved to first (lsort::partition_pivot)
         const lsort::KeyDesc less{};
@@ -209,7 +228,7 @@ __device__ int32_t wave_partition(uint32_t* a, int32_t first, int32_t last, int3
         sl.kp = (int32_t)(a[first] >> 24);
         sl.cut_l = INT32_MAX;
     }
-    __syncthreads();
+    sync();
     const uint32_t kp = (uint32_t)sl.kp;
     const uint64_t le = lanemask_lt() | (1ull << lane);
     // #r over the range
@@ -247,7 +266,7 @@ __device__ int32_t wave_partition(uint32_t* a, int32_t first, int32_t last, int3
         PL += __popcll(mL);
         PR += __popcll(mR);
     }
-    __syncthreads();
+    sync();
     // swap l_k with r_k, k <= S.  Every swapped r lies above l_S, so the scan
     // ends after the chunk holding l_S, before reaching a swapped position
     // from an earlier chunk; within a chunk all reads precede the writes.
@@ -268,7 +287,7 @@ __device__ int32_t wave_partition(uint32_t* a, int32_t first, int32_t last, int3
     }
     const int32_t r_s = S > 0 ? tmp[S - 1] : INT32_MAX;
     int32_t cut = min(sl.cut_l, r_s);
-    __syncthreads();
+    sync();
     return cut > last ? last : cut;     // a left stop always exists (libstdc++'s median-of-3 invariant)
 }
 
@@ -330,7 +349,7 @@ __device__ void wave_sort_desc(uint32_t* a, int32_t n, int32_t* __restrict__ tmp
                 break;
             }
             --depth;
-            const int32_t cut = wave_partition(a, first, last, tmp, sl);
+            const int32_t cut = wave_partition(a, first, last, tmp, sl.part);
             if (lane == 0) {
                 stack[3 * sp] = cut;
                 stack[3 * sp + 1] = last;
@@ -513,6 +532,391 @@ __global__ __launch_bounds__(64) void k_window_decide(RunArrays R, WinOut O, int
     }
     // the lowest unused unit, if any: indices <= hw are used, above it the unmarked ones
     const int32_t left_k = 2 * np < nu ? next_free(hw, -1) : nu;
+    if (lane == 0) {
+        const int32_t left = left_k < nu ? left_k : -1;
+        int32_t left_nnz = 0;
+        if (left >= 0) {
+            left_nnz = run_count(R, R.rest[rb + 2 * left]);
+            if (2 * left + 1 < nrest) left_nnz += run_count(R, R.rest[rb + 2 * left + 1]);
+        }
+        const bool keep_left = left >= 0 && left_nnz >= unit_min_nnz;
+        O.n_tcsub[w] = kept_total;
+        O.n_tcval[w] = tcval;
+        O.n_pairs[w] = np;
+        O.n_groups[w] = np + (keep_left ? 1 : 0);
+        O.leftover[w] = left;
+        O.fallback[w] = (left >= 0 && !keep_left) ? 1 : 0;
+        O.n_rest[w] = nrest;
+    }
+}
+
+// ---------------------------------------------------------------- 3b. large windows, one workgroup each
+//
+// The decisions of k_window_decide with NW waves per window, for the windows
+// with thousands of columns (R-MAT's hub windows: up to ~49 000 runs and
+// ~23 000 units, 10.5 ms on one wave):
+//   a-e  counts, the (count desc, col asc) order, TC chunks, the non-TC list and
+//        the units run over the whole workgroup (stable counting sorts and
+//        compactions with per-wave segments and workgroup prefix sums);
+//   f    the libstdc++ introsort of the units: its top log2(NW) levels of
+//        partitions run as a tree (level l on 2^l waves), then each wave sorts
+//        one of the NW ranges alone; ranges are disjoint, so the result is the
+//        sequential one.  The final insertion pass is a workgroup stable
+//        counting sort by key;
+//   g    the pairing on one wave, in scalar registers, with the active-unit
+//        flags as a bitmap: the 64-flag word that holds the high-water mark
+//        stays in two SGPRs, so a partner search is a masked find-first-zero
+//        and a word load only when the mark passes the word.
+
+constexpr int kKeyBuckets = 64;      // unit keys (nnz of two runs of a <= 16-row window) are < 33
+
+// exclusive prefix sum of v over the workgroup (thread order); `red` holds
+// NW + 1 ints; every thread gets the block total
+template <int NW>
+__device__ inline int32_t block_excl_scan(int32_t v, int32_t* red, int32_t& total) {
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    int32_t x = v;
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == kWave - 1) red[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t acc = 0;
+        for (int w = 0; w < NW; ++w) {
+            const int32_t c = red[w];
+            red[w] = acc;
+            acc += c;
+        }
+        red[NW] = acc;
+    }
+    __syncthreads();
+    const int32_t r = red[wid] + x - v;
+    total = red[NW];
+    __syncthreads();
+    return r;
+}
+
+// one wave: libstdc++'s introsort loop on [first, last) with depth budget `depth`
+// (lsort::sort's steps, the partitions in closed form), without the final pass
+__device__ void wave_introsort_range(uint32_t* a, int32_t first, int32_t last, int depth, int32_t* __restrict__ tmp,
+                                     int32_t* stack, PartLds& pl) {
+    const WaveSync sync{};
+    const int lane = threadIdx.x & (kWave - 1);
+    if (lane == 0) {
+        stack[0] = first;
+        stack[1] = last;
+        stack[2] = depth;
+    }
+    sync();
+    int sp = 1;
+    while (sp > 0) {
+        --sp;
+        int32_t f = stack[3 * sp], l = stack[3 * sp + 1], d = stack[3 * sp + 2];
+        sync();
+        while (l - f > lsort::kThreshold) {
+            if (d == 0) {
+                if (lane == 0) lsort::heap_sort(a + f, l - f, lsort::KeyDesc{});
+                sync();
+                break;
+            }
+            --d;
+            const int32_t cut = wave_partition<WaveSync>(a, f, l, tmp + f, pl);
+            if (lane == 0) {
+                stack[3 * sp] = cut;
+                stack[3 * sp + 1] = l;
+                stack[3 * sp + 2] = d;
+            }
+            sync();
+            ++sp;
+            l = cut;
+        }
+    }
+}
+
+// bitmap of active units: LDS (plain loads, in wave order) or global memory
+// (atomic loads: the marks are atomic ORs at the L2)
+struct UnitFlags {
+    uint64_t* w;
+    bool global;
+    __device__ uint64_t load(int32_t i) const {
+        return global ? __hip_atomic_load(w + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : w[i];
+    }
+    __device__ void mark(int32_t k) const {
+        if (global) atomicOr((unsigned long long*)(w + (k >> 6)), 1ull << (k & 63));
+        else atomicOr((unsigned long long*)(w + (k >> 6)), 1ull << (k & 63));
+    }
+};
+
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_window_decide_wg(RunArrays R, WinOut O, int64_t W, int window, int density,
+                                                             float tc_min_util, int skip_tc, int unit_min_nnz,
+                                                             const int32_t* __restrict__ wlist, int lds_cap) {
+    constexpr int T = kWave * NW;
+    extern __shared__ __align__(16) uint32_t dyn_wg[];
+    __shared__ int32_t hist[17], base[17];
+    __shared__ int32_t wcnt[NW][17];                 // per wave and run count: runs, then write offsets
+    __shared__ int32_t kcnt[NW][kKeyBuckets];        // per wave and unit key: units, then write offsets
+    __shared__ int32_t red[NW + 1];
+    __shared__ int32_t s_ncand;
+    __shared__ int32_t sort_stack[NW][3 * 66];
+    __shared__ PartLds part[NW];
+    __shared__ int32_t rng[NW][3], rng2[NW][3];      // the sort's ranges after each tree level
+    const int64_t w = wlist[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const uint64_t lt = lanemask_lt();
+    const int32_t rb = R.win_rb[w], re = R.win_rb[w + 1];
+    const int32_t nc = re - rb;
+
+    // a. histogram of run counts (1..16), per wave
+    for (int i = tid; i < 17; i += T) hist[i] = 0;
+    for (int i = tid; i < NW * 17; i += T) wcnt[i / 17][i % 17] = 0;
+    __syncthreads();
+    const int32_t seg = (nc + NW - 1) / NW;
+    const int32_t s0 = rb + wid * seg, s1 = min(re, s0 + seg);
+    for (int32_t i = s0 + lane; i < s1; i += kWave) atomicAdd(&wcnt[wid][run_count(R, i)], 1);
+    __syncthreads();
+    if (tid == 0) {
+        for (int c = 1; c <= 16; ++c) {
+            int32_t t = 0;
+            for (int v = 0; v < NW; ++v) t += wcnt[v][c];
+            hist[c] = t;
+        }
+        int32_t acc = 0;
+        for (int c = 16; c >= 1; --c) {
+            base[c] = acc;
+            acc += hist[c];
+        }
+        int32_t ncand = 0;
+        if (!skip_tc)
+            for (int c = 16; c >= 1 && c >= density; --c) ncand += hist[c];
+        s_ncand = ncand;
+    }
+    __syncthreads();
+    // b. stable counting sort by count desc: wave v's runs of count c go after
+    // every wave's runs of larger counts and after waves < v's runs of count c
+    if (tid >= 1 && tid <= 16) {
+        int32_t off = base[tid];
+        for (int v = 0; v < NW; ++v) {
+            const int32_t c = wcnt[v][tid];
+            wcnt[v][tid] = off;
+            off += c;
+        }
+    }
+    __syncthreads();
+    for (int32_t i0 = s0; i0 < s1; i0 += kWave) {
+        const int32_t i = i0 + lane;
+        const bool on = i < s1;
+        const int c = on ? run_count(R, i) : 0;
+        uint64_t rem = ballot(on);
+        while (rem) {
+            const int leader = __ffsll((long long)rem) - 1;
+            const int cc = __shfl(c, leader);
+            const uint64_t grp = ballot(on && c == cc);
+            const int32_t at = wcnt[wid][cc];
+            if (on && c == cc) R.order[rb + at + __popcll(grp & lt)] = i;
+            if (lane == 0) wcnt[wid][cc] = at + __popcll(grp);
+            rem &= ~grp;
+        }
+    }
+    __syncthreads();
+
+    // c. TC chunks of 8 over the candidates (block_online.cpp:215-253)
+    const int32_t ncand = s_ncand;
+    const int32_t nchunk = (ncand + 7) / 8;
+    int32_t kept_total = 0, tcval = 0;
+    for (int32_t j0 = 0; j0 < nchunk; j0 += T) {
+        const int32_t j = j0 + tid;
+        bool keep = false;
+        int32_t nnz = 0;
+        if (j < nchunk) {
+            const int32_t e = (8 * j + 8 < ncand) ? 8 * j + 8 : ncand;
+            for (int32_t k = 8 * j; k < e; ++k) nnz += run_count(R, R.order[rb + k]);
+            const float util = (float)nnz / (float)(window * 8);
+            keep = !(util < tc_min_util);
+            for (int32_t k = 8 * j; k < e; ++k) R.in_tc[R.order[rb + k]] = keep ? 1 : 0;
+        }
+        int32_t nk = 0, sv = 0;
+        const int32_t pos = block_excl_scan<NW>(keep ? 1 : 0, red, nk);
+        if (keep) R.tc_chunk[rb + kept_total + pos] = j;
+        block_excl_scan<NW>(keep ? nnz : 0, red, sv);
+        kept_total += nk;
+        tcval += sv;
+    }
+    for (int32_t k = ncand + tid; k < nc; k += T) R.in_tc[R.order[rb + k]] = 0;
+    __syncthreads();
+
+    // d. non-TC runs in (count desc, col asc) order
+    int32_t nrest = 0;
+    for (int32_t k0 = 0; k0 < nc; k0 += T) {
+        const int32_t k = k0 + tid;
+        const int32_t run = k < nc ? R.order[rb + k] : 0;
+        const bool on = k < nc && !R.in_tc[run];
+        int32_t n_on = 0;
+        const int32_t pos = block_excl_scan<NW>(on ? 1 : 0, red, n_on);
+        if (on) R.rest[rb + nrest + pos] = run;
+        nrest += n_on;
+    }
+    __syncthreads();
+
+    // e. units and their order key; the active-unit bitmap cleared
+    const int32_t nu = (nrest + 1) / 2;
+    const bool in_lds = lds_cap > 0 && nu <= lds_cap;
+    uint32_t* units = in_lds ? dyn_wg : R.units_g + rb;
+    for (int32_t k = tid; k < nu; k += T) {
+        int32_t nnz = run_count(R, R.rest[rb + 2 * k]);
+        if (2 * k + 1 < nrest) nnz += run_count(R, R.rest[rb + 2 * k + 1]);
+        units[k] = ((uint32_t)nnz << 24) | (uint32_t)k;
+    }
+    // bitmap: in LDS after the keys (8-byte aligned: lds_cap is even), else in
+    // used_g's bytes of this window rounded up to 8 bytes (nc >= nu / 8 + 8)
+    UnitFlags fl;
+    fl.global = !in_lds;
+    fl.w = in_lds ? (uint64_t*)(dyn_wg + lds_cap)
+                  : (uint64_t*)(((uintptr_t)(R.used_g + rb) + 7) & ~(uintptr_t)7);
+    const int32_t nwords = (nu + 63) / 64;
+    for (int32_t k = tid; k < nwords; k += T) fl.w[k] = 0;
+    __syncthreads();
+
+    // f. libstdc++ order of the units
+    uint32_t* a = units;
+    int32_t* tmp = R.pair_a + rb;                    // nc >= nu entries of scratch
+    if (nu <= lsort::kThreshold) {
+        if (tid == 0) lsort::sort(a, nu, lsort::KeyDesc{}, sort_stack[0]);
+        __syncthreads();
+    } else {
+        if (tid == 0) {
+            rng[0][0] = 0;
+            rng[0][1] = nu;
+            rng[0][2] = 2 * lsort::floor_log2(nu);
+            for (int v = 1; v < NW; ++v) rng[v][0] = rng[v][1] = rng[v][2] = 0;
+        }
+        __syncthreads();
+        // tree levels: wave v < 2^l partitions range v into ranges 2v and 2v + 1
+        for (int nr = 1; 2 * nr <= NW; nr *= 2) {
+            if (wid < nr) {
+                const int32_t f = rng[wid][0], l = rng[wid][1], d = rng[wid][2];
+                int32_t c0[3] = {f, l, d}, c1[3] = {0, 0, 0};
+                if (l - f > lsort::kThreshold && d > 0) {
+                    const int32_t cut = wave_partition<WaveSync>(a, f, l, tmp + f, part[wid]);
+                    c0[1] = cut;
+                    c0[2] = d - 1;
+                    c1[0] = cut;
+                    c1[1] = l;
+                    c1[2] = d - 1;
+                }
+                if (lane == 0)
+                    for (int q = 0; q < 3; ++q) {
+                        rng2[2 * wid][q] = c0[q];
+                        rng2[2 * wid + 1][q] = c1[q];
+                    }
+            }
+            __syncthreads();
+            if (tid < 2 * nr * 3) rng[tid / 3][tid % 3] = rng2[tid / 3][tid % 3];
+            __syncthreads();
+        }
+        wave_introsort_range(a, rng[wid][0], rng[wid][1], rng[wid][2], tmp, sort_stack[wid], part[wid]);
+        __syncthreads();
+        // the final insertion pass = a stable sort by key (descending), per-wave
+        // segments in order
+        for (int i = tid; i < NW * kKeyBuckets; i += T) kcnt[i / kKeyBuckets][i % kKeyBuckets] = 0;
+        __syncthreads();
+        const int32_t useg = (nu + NW - 1) / NW;
+        const int32_t u0 = wid * useg, u1 = min(nu, u0 + useg);
+        for (int32_t p = u0 + lane; p < u1; p += kWave) atomicAdd(&kcnt[wid][a[p] >> 24], 1);
+        __syncthreads();
+        if (tid == 0) {
+            int32_t acc = 0;
+            for (int k = kKeyBuckets - 1; k >= 0; --k)
+                for (int v = 0; v < NW; ++v) {
+                    const int32_t c = kcnt[v][k];
+                    kcnt[v][k] = acc;
+                    acc += c;
+                }
+        }
+        __syncthreads();
+        uint32_t* out = (uint32_t*)tmp;
+        for (int32_t b0 = u0; b0 < u1; b0 += kWave) {
+            const int32_t p = b0 + lane;
+            const bool on = p < u1;
+            const uint32_t v = on ? a[p] : 0u;
+            const uint32_t k = v >> 24;
+            uint64_t rem = ballot(on);
+            while (rem) {
+                const int leader = __ffsll((long long)rem) - 1;
+                const uint32_t kk = (uint32_t)__shfl((int)k, leader);
+                const uint64_t grp = ballot(on && k == kk);
+                const int32_t at = kcnt[wid][kk];
+                if (on && k == kk) out[at + __popcll(grp & lt)] = v;
+                if (lane == 0) kcnt[wid][kk] = at + __popcll(grp);
+                rem &= ~grp;
+            }
+        }
+        __syncthreads();
+        for (int32_t p = tid; p < nu; p += T) a[p] = out[p];
+        __syncthreads();
+    }
+
+    // g. pairing (block_online.cpp:365-391) on wave 0: in sorted order, a unit
+    // not yet taken (index > hw) takes the first index above hw that no earlier
+    // active unit holds, other than itself.  hw only grows, so the search keeps
+    // one 64-flag word of the bitmap in scalar registers (indices sb .. sb + 63)
+    // and loads the next word when hw passes it; a mark sets the bit there when
+    // it falls in that word, and always in the bitmap
+    if (wid != 0) return;
+    auto load_word = [&](int32_t wi) -> uint64_t {   // word wi, indices >= nu read as taken
+        const int32_t i0 = 64 * wi;
+        if (i0 >= nu) return ~0ull;
+        const uint64_t x = fl.load(wi);
+        uint64_t u = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32)) << 32) |
+                     (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+        if (nu - i0 < 64) u |= ~0ull << (nu - i0);
+        return u;
+    };
+    int32_t sb = 0;
+    uint64_t sw = load_word(0);
+    // first index >= from whose flag is clear, other than `except` (nu if none)
+    auto find_free = [&](int32_t from, int32_t except) -> int32_t {
+        while (from < nu) {
+            if ((from >> 6) != (sb >> 6)) {
+                sb = from & ~63;
+                sw = load_word(sb >> 6);
+            }
+            uint64_t m = ~sw & (~0ull << (from - sb));
+            if (except >= sb && except < sb + 64) m &= ~(1ull << (except - sb));
+            if (m) return sb + (int32_t)__builtin_ctzll(m);
+            from = sb + 64;
+        }
+        return nu;
+    };
+    int32_t hw = -1, np = 0, my_a = 0, my_b = 0;
+    for (int32_t i0 = 0; i0 < nu; i0 += kWave) {
+        const int32_t mine = i0 + lane < nu ? (int32_t)(a[i0 + lane] & 0xffffffu) : 0;
+        const int32_t cnt = nu - i0 < kWave ? nu - i0 : kWave;
+        for (int32_t j = 0; j < cnt; ++j) {
+            const int32_t v = __builtin_amdgcn_readlane(mine, j);
+            if (v <= hw) continue;                   // already a partner
+            const int32_t p = find_free(hw + 1, v);
+            if (p >= nu) continue;                   // no partner: v stays unused
+            if (lane == 0) fl.mark(v);
+            if (v >= sb && v < sb + 64) sw |= 1ull << (v - sb);
+            if (lane == (np & (kWave - 1))) {
+                my_a = v;
+                my_b = p;
+            }
+            hw = p;
+            if ((++np & (kWave - 1)) == 0) {
+                R.pair_a[rb + np - kWave + lane] = my_a;
+                R.pair_b[rb + np - kWave + lane] = my_b;
+            }
+        }
+    }
+    if (lane < (np & (kWave - 1))) {
+        R.pair_a[rb + (np & ~(kWave - 1)) + lane] = my_a;
+        R.pair_b[rb + (np & ~(kWave - 1)) + lane] = my_b;
+    }
+    const int32_t left_k = 2 * np < nu ? find_free(hw + 1, -1) : nu;
     if (lane == 0) {
         const int32_t left = left_k < nu ? left_k : -1;
         int32_t left_nnz = 0;
@@ -1072,17 +1476,19 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
         // size classes by units-per-window: small windows share a CU, large ones take as much
         // LDS as a workgroup may hold (160 KB on gfx950), a window beyond that sorts in global
         // memory.  A unit takes 4 bytes (key) + 1 (used flag).
+        // classes 1-3 run on whole workgroups (k_window_decide_wg: 4 waves for the
+        // medium class, 16 for the large ones)
         int big_cap = 12288;
         {
             int dev = 0, max_lds = 0;
             hipFuncAttributes fa{};
             if (hipGetDevice(&dev) == hipSuccess &&
                 hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
-                hipFuncGetAttributes(&fa, (const void*)k_window_decide) == hipSuccess) {
-                const int cap = (int)((max_lds - (int64_t)fa.sharedSizeBytes - 64) / 5);
+                hipFuncGetAttributes(&fa, (const void*)k_window_decide_wg<16>) == hipSuccess) {
+                const int cap = (int)((max_lds - (int64_t)fa.sharedSizeBytes - 64) / 5) & ~1;
                 if (cap > big_cap &&
-                    hipFuncSetAttribute((const void*)k_window_decide, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        cap * 5 + 16) == hipSuccess)
+                    hipFuncSetAttribute((const void*)k_window_decide_wg<16>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, cap * 5 + 16) == hipSuccess)
                     big_cap = cap;
             }
             (void)hipGetLastError();      // a refused query leaves the 64 KB classes
@@ -1116,8 +1522,17 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
             const size_t n = at[c + 1] - at[c];
             if (n == 0) continue;
             const size_t lds = caps[c] > 0 ? (size_t)caps[c] * 5 + 16 : 0;
-            hipLaunchKernelGGL(k_window_decide, dim3((unsigned)n), dim3(64), lds, s, R, O, W, window, cfg.density,
-                               cfg.tc_min_util, cfg.skip_tc, cfg.unit_min_nnz, wl.as<int32_t>() + at[c], caps[c]);
+            if (c == 0)
+                hipLaunchKernelGGL(k_window_decide, dim3((unsigned)n), dim3(64), lds, s, R, O, W, window, cfg.density,
+                                   cfg.tc_min_util, cfg.skip_tc, cfg.unit_min_nnz, wl.as<int32_t>() + at[c], caps[c]);
+            else if (c == 1)
+                hipLaunchKernelGGL(k_window_decide_wg<4>, dim3((unsigned)n), dim3(4 * kWave), lds, s, R, O, W, window,
+                                   cfg.density, cfg.tc_min_util, cfg.skip_tc, cfg.unit_min_nnz,
+                                   wl.as<int32_t>() + at[c], caps[c]);
+            else
+                hipLaunchKernelGGL(k_window_decide_wg<16>, dim3((unsigned)n), dim3(16 * kWave), lds, s, R, O, W,
+                                   window, cfg.density, cfg.tc_min_util, cfg.skip_tc, cfg.unit_min_nnz,
+                                   wl.as<int32_t>() + at[c], caps[c]);
             TRSPMM_HIP(hipGetLastError());
         }
         TRSPMM_HIP(hipStreamSynchronize(s));    // the window list is released on return from this block
