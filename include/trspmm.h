@@ -66,6 +66,13 @@ enum {
 const char* trspmm_last_error(void);
 int trspmm_version(void);
 int trspmm_device_arch(char* buf, int len);   /* gcnArchName of the current device */
+/* No counterpart in the reference.  Router results and router scratch take
+ * their device blocks from a per-device cache that keeps released blocks for
+ * the next call (hipMalloc / hipFree cost ~0.3 ms each and hipFree waits for
+ * the device); at most TRSPMM_CACHE_MB (environment, default 4096) idle MiB per
+ * device are kept.  This frees every idle block and stores the bytes freed in
+ * *freed (may be NULL). */
+int trspmm_cache_trim(int64_t* freed);
 
 /* ------------------------------------------------------------------ router */
 

@@ -83,3 +83,10 @@ def test_check_raises_with_message():
     rc = L.trspmm_spmm(None, None, 8, 8, None, 8, 7, None)
     with pytest.raises(_lib.TrspmmError, match="plan"):
         _lib.check(rc, "trspmm_spmm")
+
+
+def test_cache_trim_without_blocks():
+    freed = ctypes.c_int64(-1)
+    assert _lib.lib().trspmm_cache_trim(ctypes.byref(freed)) == 0
+    assert freed.value == 0
+    assert _lib.lib().trspmm_cache_trim(None) == 0

@@ -1,5 +1,8 @@
 // extern "C" entry points of libtrspmm.so (declared in include/trspmm.h).
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -24,6 +27,156 @@ static thread_local std::string g_error;
 void set_error(const std::string& msg) { g_error = msg; }
 const char* last_error() { return g_error.c_str(); }
 
+// ---------------------------------------------------------------- block cache
+namespace {
+struct CacheRec {
+    size_t n;
+    int dev;
+};
+struct BlockCache {
+    std::mutex mu;
+    std::map<void*, CacheRec> live;                  // every block of the cache, handed out or idle
+    std::map<int, std::multimap<size_t, void*>> idle;  // per device, by size
+    std::map<int, int64_t> idle_bytes;
+    int64_t limit = -1;
+    int64_t bound() {
+        if (limit < 0) {
+            const char* e = std::getenv("TRSPMM_CACHE_MB");
+            const long long mb = e ? std::atoll(e) : 4096;
+            limit = (int64_t)(mb > 0 ? mb : 0) << 20;
+        }
+        return limit;
+    }
+};
+BlockCache& block_cache() {
+    static BlockCache* c = new BlockCache();         // never destroyed: blocks may be put back at exit
+    return *c;
+}
+}  // namespace
+
+void* cache_get(size_t n) {
+    n = (n + 255) & ~(size_t)255;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("cache_get: no current device");
+        return nullptr;
+    }
+    BlockCache& c = block_cache();
+    {
+        std::lock_guard<std::mutex> g(c.mu);
+        auto& m = c.idle[dev];
+        auto it = m.lower_bound(n);
+        if (it != m.end() && it->first <= 2 * n) {
+            void* p = it->second;
+            c.idle_bytes[dev] -= (int64_t)it->first;
+            m.erase(it);
+            return p;
+        }
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, n) != hipSuccess) {
+        (void)hipGetLastError();
+        // idle blocks of this device may be what stands in the way
+        {
+            std::lock_guard<std::mutex> g(c.mu);
+            for (auto& kv : c.idle[dev]) {
+                (void)hipFree(kv.second);
+                c.live.erase(kv.second);
+            }
+            c.idle[dev].clear();
+            c.idle_bytes[dev] = 0;
+        }
+        if (hipMalloc(&p, n) != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("hipMalloc failed (" + std::to_string(n) + " bytes)");
+            return nullptr;
+        }
+    }
+    std::lock_guard<std::mutex> g(c.mu);
+    c.live[p] = CacheRec{n, dev};
+    return p;
+}
+
+void cache_put(void* p) {
+    if (!p) return;
+    BlockCache& c = block_cache();
+    std::unique_lock<std::mutex> g(c.mu);
+    auto it = c.live.find(p);
+    if (it == c.live.end()) {          // not a cache block
+        g.unlock();
+        (void)hipFree(p);
+        return;
+    }
+    const CacheRec r = it->second;
+    if (c.idle_bytes[r.dev] + (int64_t)r.n > c.bound()) {
+        c.live.erase(it);
+        g.unlock();
+        (void)hipFree(p);
+        return;
+    }
+    c.idle[r.dev].emplace(r.n, p);
+    c.idle_bytes[r.dev] += (int64_t)r.n;
+}
+
+int64_t cache_trim() {
+    BlockCache& c = block_cache();
+    std::lock_guard<std::mutex> g(c.mu);
+    int64_t freed = 0;
+    for (auto& d : c.idle) {
+        for (auto& kv : d.second) {
+            (void)hipFree(kv.second);
+            c.live.erase(kv.second);
+            freed += (int64_t)kv.first;
+        }
+        d.second.clear();
+        c.idle_bytes[d.first] = 0;
+    }
+    return freed;
+}
+
+void* StreamArena::get(size_t n) {
+    int best = -1;
+    for (int i = 0; i < n_idle; ++i)
+        if (idle[i].n >= n && idle[i].n <= 2 * n && (best < 0 || idle[i].n < idle[best].n)) best = i;
+    if (best >= 0) {
+        void* p = idle[best].p;
+        idle[best] = idle[--n_idle];
+        return p;
+    }
+    if (n_held == cap_held) {
+        const int nc = cap_held ? 2 * cap_held : 64;
+        void** h = (void**)std::realloc(held, sizeof(void*) * nc);
+        if (!h) {
+            set_error("out of host memory");
+            return nullptr;
+        }
+        held = h;
+        cap_held = nc;
+    }
+    void* p = cache_get(n);
+    if (p) held[n_held++] = p;
+    return p;
+}
+
+void StreamArena::put(void* p, size_t n) {
+    if (n_idle == cap_idle) {
+        const int nc = cap_idle ? 2 * cap_idle : 64;
+        Block* b = (Block*)std::realloc(idle, sizeof(Block) * nc);
+        if (!b) return;                // the block stays held; it goes back at the end
+        idle = b;
+        cap_idle = nc;
+    }
+    idle[n_idle++] = Block{p, n};
+}
+
+StreamArena::~StreamArena() {
+    (void)hipStreamSynchronize(s);
+    for (int i = 0; i < n_held; ++i) cache_put(held[i]);
+    std::free(held);
+    std::free(idle);
+}
+
 }  // namespace trspmm
 
 using namespace trspmm;
@@ -35,6 +188,12 @@ extern "C" {
 const char* trspmm_last_error(void) { return trspmm::last_error(); }
 
 int trspmm_version(void) { return 2; }
+
+int trspmm_cache_trim(int64_t* freed) {
+    const int64_t n = cache_trim();
+    if (freed) *freed = n;
+    return TRSPMM_OK;
+}
 
 int trspmm_device_arch(char* buf, int len) {
     int dev = 0;

@@ -15,6 +15,8 @@ inline size_t meta_elem_size(int i) {
     }
 }
 
+// The arrays are blocks of the device block cache: the destructor waits for the
+// device (as hipFree would) before they go back.
 struct Meta {
     DevBuf buf[29];
     void* data[29] = {};
@@ -25,7 +27,7 @@ struct Meta {
     double seconds = 0.0;
     int alloc(int i, int64_t n) {
         count[i] = n;
-        int rc = buf[i].alloc(meta_elem_size(i) * (size_t)(n > 0 ? n : 0));
+        int rc = buf[i].alloc_cached(meta_elem_size(i) * (size_t)(n > 0 ? n : 0));
         data[i] = buf[i].p;
         return rc;
     }
@@ -38,10 +40,14 @@ struct Meta {
     int edge_alloc(int i, int64_t n) {
         has_edges = true;
         edge_count[i] = n;
-        int rc = edge_buf[i].alloc(4 * (size_t)(n > 0 ? n : 0));
+        int rc = edge_buf[i].alloc_cached(4 * (size_t)(n > 0 ? n : 0));
         edge_data[i] = edge_buf[i].p;
         return rc;
     }
+    Meta() = default;
+    Meta(const Meta&) = delete;
+    Meta& operator=(const Meta&) = delete;
+    ~Meta() { (void)hipDeviceSynchronize(); }
 };
 
 int route_device(const int32_t* rowptr, const int32_t* col, const float* val, int64_t rows, int64_t nnz,

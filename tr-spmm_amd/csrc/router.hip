@@ -30,6 +30,8 @@
 #include <chrono>
 #include <cstring>   // rocPRIM's texture iterator uses memset without including it
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include <rocprim/rocprim.hpp>
@@ -983,6 +985,25 @@ struct Emit {
 };
 
 // TC: one thread per (window, kept chunk) = sub-block
+// values of each TC sub-block (the runs of its chunk), in sub-block order
+__global__ void k_tc_counts(RunArrays R, Emit E, int64_t W, int64_t S, int32_t* __restrict__ cnt) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const int64_t w = upper_bound_window(E.tc_sub_off, W, (int32_t)s);
+    const int32_t rb = R.win_rb[w];
+    const int32_t j = R.tc_chunk[rb + (int32_t)(s - E.tc_sub_off[w])];
+    const int32_t nw = R.win_rb[w + 1] - rb;
+    int32_t c = 0;
+    for (int q = 0; q < 8; ++q) {   // chunk j = order[8j, 8j+8) cut at the first non-candidate
+        const int32_t pos = 8 * j + q;
+        if (pos >= nw) break;
+        const int32_t run = R.order[rb + pos];
+        if (!R.in_tc[run]) break;
+        c += run_count(R, run);
+    }
+    cnt[s] = c;
+}
+
 __global__ void k_emit_tc(RunArrays R, WinOut O, Emit E, int64_t W, int window, int part_t, int64_t S) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
@@ -1015,19 +1036,9 @@ __global__ void k_emit_tc(RunArrays R, WinOut O, Emit E, int64_t W, int window, 
         cols[b + 1] = cc;
     }
     for (int q = 0; q < 8; ++q) E.t_column[8 * s + q] = q < n ? cols[q] : -1;
-    // value offset of this sub-block: window offset + values of earlier kept chunks
-    int32_t voff = E.tc_val_off[w];
-    for (int32_t kk = 0; kk < k; ++kk) {
-        const int32_t jj = R.tc_chunk[rb + kk];
-        for (int q = 0; q < 8; ++q) {
-            const int32_t pos = 8 * jj + q;
-            if (pos >= R.win_rb[w + 1] - rb) break;
-            const int32_t run = R.order[rb + pos];
-            if (!R.in_tc[run]) break;
-            voff += run_count(R, run);
-        }
-    }
-    E.t_block_offset[s] = voff;
+    // value offset of this sub-block: the prefix sum of k_tc_counts, which equals
+    // the window's offset plus the values of its earlier kept chunks
+    int32_t voff = E.t_block_offset[s];
     // bits and values in (row, slot) order
     int32_t ptr[8];
     for (int q = 0; q < n; ++q) ptr[q] = R.start[runs[q]];
@@ -1371,12 +1382,57 @@ int bit_width(int64_t x) {
     return b < 1 ? 1 : b;
 }
 
+// two streams per device (default priority, created once) on which the smaller
+// size classes of the window decisions run beside the largest one
+int side_streams(hipStream_t (&out)[2]) {
+    static std::mutex mu;
+    static std::map<int, std::pair<hipStream_t, hipStream_t>> made;
+    int dev = 0;
+    TRSPMM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    auto it = made.find(dev);
+    if (it == made.end()) {
+        hipStream_t a = nullptr, b = nullptr;
+        TRSPMM_HIP(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
+        if (hipStreamCreateWithFlags(&b, hipStreamNonBlocking) != hipSuccess) {
+            (void)hipStreamDestroy(a);
+            set_error("hipStreamCreateWithFlags failed");
+            return TRSPMM_EHIP;
+        }
+        it = made.emplace(dev, std::make_pair(a, b)).first;
+    }
+    out[0] = it->second.first;
+    out[1] = it->second.second;
+    return TRSPMM_OK;
+}
+
+// events of one fork/join, destroyed with the scope (HIP releases a recorded
+// event once its work is done); a scope left before the join (an error) waits
+// for the side streams, so that no scratch is handed back under their work
+struct EventSet {
+    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+    hipStream_t side[2] = {nullptr, nullptr};
+    bool joined = false;
+    int make() {
+        for (auto& x : e) TRSPMM_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+        return TRSPMM_OK;
+    }
+    ~EventSet() {
+        if (!joined)
+            for (auto& x : side)
+                if (x) (void)hipStreamSynchronize(x);
+        for (auto& x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------- host driver
 int route_device(const int32_t* rowptr, const int32_t* col, const float* val, int64_t rows, int64_t nnz,
                  int64_t ncols, const trspmm_route_cfg& cfg, hipStream_t s, Meta& M) {
     const auto t_begin = std::chrono::steady_clock::now();
+    StreamArena A(s);      // scratch: declared first, so that every buffer below is back before it waits
     TRSPMM_CHECK(cfg.window > 0 && cfg.window <= 16, "window must be in [1, 16]");
     TRSPMM_CHECK(cfg.wide > 0 && cfg.wide <= 32, "wide must be in [1, 32]");
     TRSPMM_CHECK(cfg.part_t > 0 && cfg.part_c > 0, "partSize_t and partSize_c must be positive");
@@ -1399,7 +1455,7 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
 
     // 1. validate + keys
     DevBuf err;
-    if ((rc = err.alloc(4))) return rc;
+    if ((rc = err.alloc(A, 4))) return rc;
     TRSPMM_HIP(hipMemsetAsync(err.p, 0, 4, s));
     if (rows > 0)
         hipLaunchKernelGGL(k_validate, dim3(nblk(rows, 4)), dim3(256), 0, s, rowptr, col, rows, nnz, ncols,
@@ -1413,9 +1469,9 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
 
     DevBuf keys, keys2, vals, svals, erow, flag, run_of, run_start, run_col, run_win, win_rb;
     const int64_t n1 = nnz > 0 ? nnz : 1;
-    if ((rc = keys.alloc(8 * n1)) || (rc = keys2.alloc(8 * n1)) || (rc = vals.alloc(4 * n1)) ||
-        (rc = svals.alloc(4 * n1)) || (rc = erow.alloc(4 * n1)) || (rc = flag.alloc(4 * n1)) ||
-        (rc = run_of.alloc(4 * (n1 + 1))) || (rc = win_rb.alloc(4 * (W + 1))))
+    if ((rc = keys.alloc(A, 8 * n1)) || (rc = keys2.alloc(A, 8 * n1)) || (rc = vals.alloc(A, 4 * n1)) ||
+        (rc = svals.alloc(A, 4 * n1)) || (rc = erow.alloc(A, 4 * n1)) || (rc = flag.alloc(A, 4 * n1)) ||
+        (rc = run_of.alloc(A, 4 * (n1 + 1))) || (rc = win_rb.alloc(A, 4 * (W + 1))))
         return rc;
     if (nnz > 0) {
         hipLaunchKernelGGL(k_keys, dim3(nblk_gs(nnz)), dim3(256), 0, s, rowptr, col, rows, nnz, window, colbits,
@@ -1427,7 +1483,7 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
                                              vals.as<int32_t>(), svals.as<int32_t>(), (size_t)nnz, 0,
                                              colbits + winbits, s));
         DevBuf tmp;
-        if ((rc = tmp.alloc(tmp_bytes ? tmp_bytes : 1))) return rc;
+        if ((rc = tmp.alloc(A, tmp_bytes ? tmp_bytes : 1))) return rc;
         TRSPMM_HIP(rocprim::radix_sort_pairs(tmp.p, tmp_bytes, keys.as<uint64_t>(), keys2.as<uint64_t>(),
                                              vals.as<int32_t>(), svals.as<int32_t>(), (size_t)nnz, 0,
                                              colbits + winbits, s));
@@ -1435,13 +1491,13 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
                            flag.as<int32_t>());
         TRSPMM_HIP(hipGetLastError());
     }
-    if ((rc = exclusive_scan<int32_t>(nnz > 0 ? flag.as<int32_t>() : nullptr, run_of.as<int32_t>(), nnz, s)))
+    if ((rc = exclusive_scan<int32_t>(nnz > 0 ? flag.as<int32_t>() : nullptr, run_of.as<int32_t>(), nnz, s, &A)))
         return rc;
     int32_t nruns = 0;
     TRSPMM_HIP(hipMemcpyAsync(&nruns, run_of.as<int32_t>() + nnz, 4, hipMemcpyDeviceToHost, s));
     TRSPMM_HIP(hipStreamSynchronize(s));
     const int64_t nr1 = nruns > 0 ? nruns : 1;
-    if ((rc = run_start.alloc(4 * (nr1 + 1))) || (rc = run_col.alloc(4 * nr1)) || (rc = run_win.alloc(4 * nr1)))
+    if ((rc = run_start.alloc(A, 4 * (nr1 + 1))) || (rc = run_col.alloc(A, 4 * nr1)) || (rc = run_win.alloc(A, 4 * nr1)))
         return rc;
     if (nnz > 0) {
         hipLaunchKernelGGL(k_runs, dim3(nblk_gs(nnz)), dim3(256), 0, s, keys2.as<uint64_t>(), flag.as<int32_t>(),
@@ -1461,11 +1517,11 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
     // 3. per-window decisions
     DevBuf order, rest, tc_chunk, in_tc, units_g, used_g, pair_a, pair_b;
     DevBuf n_tcsub, n_tcval, n_pairs, n_groups, leftover, fallback, n_rest;
-    if ((rc = order.alloc(4 * nr1)) || (rc = rest.alloc(4 * nr1)) || (rc = tc_chunk.alloc(4 * nr1)) ||
-        (rc = in_tc.alloc(nr1)) || (rc = units_g.alloc(4 * nr1)) || (rc = used_g.alloc(nr1)) ||
-        (rc = pair_a.alloc(4 * nr1)) || (rc = pair_b.alloc(4 * nr1)) || (rc = n_tcsub.alloc(4 * (W + 1))) ||
-        (rc = n_tcval.alloc(4 * (W + 1))) || (rc = n_pairs.alloc(4 * (W + 1))) || (rc = n_groups.alloc(4 * (W + 1))) ||
-        (rc = leftover.alloc(4 * (W + 1))) || (rc = fallback.alloc(4 * (W + 1))) || (rc = n_rest.alloc(4 * (W + 1))))
+    if ((rc = order.alloc(A, 4 * nr1)) || (rc = rest.alloc(A, 4 * nr1)) || (rc = tc_chunk.alloc(A, 4 * nr1)) ||
+        (rc = in_tc.alloc(A, nr1)) || (rc = units_g.alloc(A, 4 * nr1)) || (rc = used_g.alloc(A, nr1)) ||
+        (rc = pair_a.alloc(A, 4 * nr1)) || (rc = pair_b.alloc(A, 4 * nr1)) || (rc = n_tcsub.alloc(A, 4 * (W + 1))) ||
+        (rc = n_tcval.alloc(A, 4 * (W + 1))) || (rc = n_pairs.alloc(A, 4 * (W + 1))) || (rc = n_groups.alloc(A, 4 * (W + 1))) ||
+        (rc = leftover.alloc(A, 4 * (W + 1))) || (rc = fallback.alloc(A, 4 * (W + 1))) || (rc = n_rest.alloc(A, 4 * (W + 1))))
         return rc;
     RunArrays R{run_start.as<int32_t>(), run_col.as<int32_t>(), win_rb.as<int32_t>(), order.as<int32_t>(),
                 rest.as<int32_t>(), tc_chunk.as<int32_t>(), in_tc.as<uint8_t>(), units_g.as<uint32_t>(),
@@ -1516,18 +1572,32 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
             at[c + 1] = all.size();
         }
         DevBuf wl;
-        if ((rc = wl.alloc(4 * all.size()))) return rc;
+        if ((rc = wl.alloc(A, 4 * all.size()))) return rc;
         TRSPMM_HIP(hipMemcpyAsync(wl.p, all.data(), 4 * all.size(), hipMemcpyHostToDevice, s));
-        for (int c = 0; c < 4; ++c) {
+        // the classes are independent: the two smaller ones run on side streams
+        // while the largest (the longest windows) runs on s
+        hipStream_t side[2] = {s, s};
+        EventSet ev;
+        const bool fork = (at[1] > at[0]) + (at[2] > at[1]) + (at[4] > at[2]) > 1;
+        if (fork) {
+            if ((rc = side_streams(side)) || (rc = ev.make())) return rc;
+            ev.side[0] = side[0];
+            ev.side[1] = side[1];
+            TRSPMM_HIP(hipEventRecord(ev.e[0], s));
+            TRSPMM_HIP(hipStreamWaitEvent(side[0], ev.e[0], 0));
+            TRSPMM_HIP(hipStreamWaitEvent(side[1], ev.e[0], 0));
+        }
+        for (int c = 3; c >= 0; --c) {
             const size_t n = at[c + 1] - at[c];
             if (n == 0) continue;
             const size_t lds = caps[c] > 0 ? (size_t)caps[c] * 5 + 16 : 0;
             if (c == 0)
-                hipLaunchKernelGGL(k_window_decide, dim3((unsigned)n), dim3(64), lds, s, R, O, W, window, cfg.density,
-                                   cfg.tc_min_util, cfg.skip_tc, cfg.unit_min_nnz, wl.as<int32_t>() + at[c], caps[c]);
-            else if (c == 1)
-                hipLaunchKernelGGL(k_window_decide_wg<4>, dim3((unsigned)n), dim3(4 * kWave), lds, s, R, O, W, window,
+                hipLaunchKernelGGL(k_window_decide, dim3((unsigned)n), dim3(64), lds, side[0], R, O, W, window,
                                    cfg.density, cfg.tc_min_util, cfg.skip_tc, cfg.unit_min_nnz,
+                                   wl.as<int32_t>() + at[c], caps[c]);
+            else if (c == 1)
+                hipLaunchKernelGGL(k_window_decide_wg<4>, dim3((unsigned)n), dim3(4 * kWave), lds, side[1], R, O, W,
+                                   window, cfg.density, cfg.tc_min_util, cfg.skip_tc, cfg.unit_min_nnz,
                                    wl.as<int32_t>() + at[c], caps[c]);
             else
                 hipLaunchKernelGGL(k_window_decide_wg<16>, dim3((unsigned)n), dim3(16 * kWave), lds, s, R, O, W,
@@ -1535,14 +1605,21 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
                                    wl.as<int32_t>() + at[c], caps[c]);
             TRSPMM_HIP(hipGetLastError());
         }
+        if (fork) {
+            TRSPMM_HIP(hipEventRecord(ev.e[1], side[0]));
+            TRSPMM_HIP(hipEventRecord(ev.e[2], side[1]));
+            TRSPMM_HIP(hipStreamWaitEvent(s, ev.e[1], 0));
+            TRSPMM_HIP(hipStreamWaitEvent(s, ev.e[2], 0));
+            ev.joined = true;
+        }
         TRSPMM_HIP(hipStreamSynchronize(s));    // the window list is released on return from this block
     }
 
     // 4. window prefix sums -> output sizes
     DevBuf n_tparts, n_tiles, tc_sub_off, tc_val_off, tc_part_off, grp_all_off;
-    if ((rc = n_tparts.alloc(4 * (W + 1))) || (rc = n_tiles.alloc(4 * (W + 1))) ||
-        (rc = tc_sub_off.alloc(4 * (W + 1))) || (rc = tc_val_off.alloc(4 * (W + 1))) ||
-        (rc = tc_part_off.alloc(4 * (W + 1))) || (rc = grp_all_off.alloc(4 * (W + 1))))
+    if ((rc = n_tparts.alloc(A, 4 * (W + 1))) || (rc = n_tiles.alloc(A, 4 * (W + 1))) ||
+        (rc = tc_sub_off.alloc(A, 4 * (W + 1))) || (rc = tc_val_off.alloc(A, 4 * (W + 1))) ||
+        (rc = tc_part_off.alloc(A, 4 * (W + 1))) || (rc = grp_all_off.alloc(A, 4 * (W + 1))))
         return rc;
     if ((rc = M.alloc(22, W + 1)) || (rc = M.alloc(28, W + 1))) return rc;   // s_offsets, s_window_tile_offset
     if (W > 0) {
@@ -1551,10 +1628,10 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
         TRSPMM_HIP(hipGetLastError());
     }
     const int32_t* nz = nullptr;
-    if ((rc = exclusive_scan<int32_t>(W ? n_tcsub.as<int32_t>() : nz, tc_sub_off.as<int32_t>(), W, s)) ||
-        (rc = exclusive_scan<int32_t>(W ? n_tcval.as<int32_t>() : nz, tc_val_off.as<int32_t>(), W, s)) ||
-        (rc = exclusive_scan<int32_t>(W ? n_tparts.as<int32_t>() : nz, tc_part_off.as<int32_t>(), W, s)) ||
-        (rc = exclusive_scan<int32_t>(W ? n_groups.as<int32_t>() : nz, grp_all_off.as<int32_t>(), W, s)))
+    if ((rc = exclusive_scan<int32_t>(W ? n_tcsub.as<int32_t>() : nz, tc_sub_off.as<int32_t>(), W, s, &A)) ||
+        (rc = exclusive_scan<int32_t>(W ? n_tcval.as<int32_t>() : nz, tc_val_off.as<int32_t>(), W, s, &A)) ||
+        (rc = exclusive_scan<int32_t>(W ? n_tparts.as<int32_t>() : nz, tc_part_off.as<int32_t>(), W, s, &A)) ||
+        (rc = exclusive_scan<int32_t>(W ? n_groups.as<int32_t>() : nz, grp_all_off.as<int32_t>(), W, s, &A)))
         return rc;
     int32_t tot[5];
     TRSPMM_HIP(hipMemcpyAsync(&tot[0], tc_sub_off.as<int32_t>() + W, 4, hipMemcpyDeviceToHost, s));
@@ -1566,7 +1643,7 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
 
     // 2:4 groups that stay on the 2:4 route (all but sparse ones in TUNED mode)
     DevBuf res_flag, keep, gmap;
-    if ((rc = res_flag.alloc(n1)) || (rc = keep.alloc(GA > 0 ? GA : 1)) || (rc = gmap.alloc(4 * (GA + 1))))
+    if ((rc = res_flag.alloc(A, n1)) || (rc = keep.alloc(A, GA > 0 ? GA : 1)) || (rc = gmap.alloc(A, 4 * (GA + 1))))
         return rc;
     TRSPMM_HIP(hipMemsetAsync(res_flag.p, 0, n1, s));
     Emit E{svals.as<int32_t>(), erow.as<int32_t>(), val, tc_sub_off.as<int32_t>(), tc_val_off.as<int32_t>(),
@@ -1580,17 +1657,17 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
     // gmap = exclusive scan of keep (bytes widened through the int32 counts buffer)
     {
         DevBuf keep32;
-        if ((rc = keep32.alloc(4 * (GA > 0 ? GA : 1)))) return rc;
+        if ((rc = keep32.alloc(A, 4 * (GA > 0 ? GA : 1)))) return rc;
         if (GA > 0) {
             hipLaunchKernelGGL(k_widen, dim3(nblk(GA)), dim3(256), 0, s, keep.as<uint8_t>(), GA, keep32.as<int32_t>());
             TRSPMM_HIP(hipGetLastError());
         }
-        if ((rc = exclusive_scan<int32_t>(GA ? keep32.as<int32_t>() : nz, gmap.as<int32_t>(), GA, s))) return rc;
+        if ((rc = exclusive_scan<int32_t>(GA ? keep32.as<int32_t>() : nz, gmap.as<int32_t>(), GA, s, &A))) return rc;
     }
     hipLaunchKernelGGL(k_group_offsets, dim3(nblk(W + 1)), dim3(256), 0, s, grp_all_off.as<int32_t>(),
                        gmap.as<int32_t>(), W, (int32_t*)M.data[22], n_tiles.as<int32_t>());
     TRSPMM_HIP(hipGetLastError());
-    if ((rc = exclusive_scan<int32_t>(W ? n_tiles.as<int32_t>() : nz, (int32_t*)M.data[28], W, s))) return rc;
+    if ((rc = exclusive_scan<int32_t>(W ? n_tiles.as<int32_t>() : nz, (int32_t*)M.data[28], W, s, &A))) return rc;
     TRSPMM_HIP(hipMemcpyAsync(&tot[3], (int32_t*)M.data[22] + W, 4, hipMemcpyDeviceToHost, s));
     TRSPMM_HIP(hipMemcpyAsync(&tot[4], (int32_t*)M.data[28] + W, 4, hipMemcpyDeviceToHost, s));
     TRSPMM_HIP(hipStreamSynchronize(s));
@@ -1630,6 +1707,11 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
         if (TV > 0) TRSPMM_HIP(hipMemsetAsync(E.t_edge, 0xff, 4 * TV, s));   // -1: no edge
     }
     if (S > 0) {
+        DevBuf tcn;
+        if ((rc = tcn.alloc(A, 4 * (size_t)S))) return rc;
+        hipLaunchKernelGGL(k_tc_counts, dim3(nblk(S)), dim3(256), 0, s, R, E, W, S, tcn.as<int32_t>());
+        TRSPMM_HIP(hipGetLastError());
+        if ((rc = exclusive_scan<int32_t>(tcn.as<int32_t>(), E.t_block_offset, S, s, &A))) return rc;
         hipLaunchKernelGGL(k_emit_tc, dim3(nblk(S)), dim3(256), 0, s, R, O, E, W, window, cfg.part_t, S);
         TRSPMM_HIP(hipGetLastError());
     }
@@ -1651,8 +1733,8 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
     // residual parts per row
     DevBuf nl, ns, el, es, pl, ps, ql, qs;
     const int64_t rw1 = rows + 1;
-    if ((rc = nl.alloc(4 * rw1)) || (rc = ns.alloc(4 * rw1)) || (rc = el.alloc(4 * rw1)) || (rc = es.alloc(4 * rw1)) ||
-        (rc = pl.alloc(4 * rw1)) || (rc = ps.alloc(4 * rw1)) || (rc = ql.alloc(4 * rw1)) || (rc = qs.alloc(4 * rw1)))
+    if ((rc = nl.alloc(A, 4 * rw1)) || (rc = ns.alloc(A, 4 * rw1)) || (rc = el.alloc(A, 4 * rw1)) || (rc = es.alloc(A, 4 * rw1)) ||
+        (rc = pl.alloc(A, 4 * rw1)) || (rc = ps.alloc(A, 4 * rw1)) || (rc = ql.alloc(A, 4 * rw1)) || (rc = qs.alloc(A, 4 * rw1)))
         return rc;
     ResCounts RCn{nl.as<int32_t>(), ns.as<int32_t>(), el.as<int32_t>(), es.as<int32_t>()};
     if (rows > 0) {
@@ -1661,10 +1743,10 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
         TRSPMM_HIP(hipGetLastError());
     }
     const int32_t* z = nullptr;
-    if ((rc = exclusive_scan<int32_t>(rows ? nl.as<int32_t>() : z, pl.as<int32_t>(), rows, s)) ||
-        (rc = exclusive_scan<int32_t>(rows ? ns.as<int32_t>() : z, ps.as<int32_t>(), rows, s)) ||
-        (rc = exclusive_scan<int32_t>(rows ? el.as<int32_t>() : z, ql.as<int32_t>(), rows, s)) ||
-        (rc = exclusive_scan<int32_t>(rows ? es.as<int32_t>() : z, qs.as<int32_t>(), rows, s)))
+    if ((rc = exclusive_scan<int32_t>(rows ? nl.as<int32_t>() : z, pl.as<int32_t>(), rows, s, &A)) ||
+        (rc = exclusive_scan<int32_t>(rows ? ns.as<int32_t>() : z, ps.as<int32_t>(), rows, s, &A)) ||
+        (rc = exclusive_scan<int32_t>(rows ? el.as<int32_t>() : z, ql.as<int32_t>(), rows, s, &A)) ||
+        (rc = exclusive_scan<int32_t>(rows ? es.as<int32_t>() : z, qs.as<int32_t>(), rows, s, &A)))
         return rc;
     int32_t rt[4];
     TRSPMM_HIP(hipMemcpyAsync(&rt[0], pl.as<int32_t>() + rows, 4, hipMemcpyDeviceToHost, s));

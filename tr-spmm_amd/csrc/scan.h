@@ -93,7 +93,7 @@ __global__ __launch_bounds__(1024) void k_scan_chunks(const T* __restrict__ in, 
 }
 
 template <typename T>
-inline int exclusive_scan(const T* in, T* out, int64_t n, hipStream_t s) {
+inline int exclusive_scan(const T* in, T* out, int64_t n, hipStream_t s, StreamArena* arena = nullptr) {
     if (n <= kScanSingle) {
         hipLaunchKernelGGL(k_exclusive_scan<T>, dim3(1), dim3(1024), 0, s, in, out, n);
         TRSPMM_HIP(hipGetLastError());
@@ -101,6 +101,16 @@ inline int exclusive_scan(const T* in, T* out, int64_t n, hipStream_t s) {
     }
     const int64_t G = (n + kScanChunk - 1) / kScanChunk;
     T* chunk_sums = nullptr;
+    if (arena) {
+        DevBuf sums;          // back to the arena at once: later work on s reuses it in stream order
+        if (int rc = sums.alloc(*arena, sizeof(T) * (size_t)(2 * G + 1))) return rc;
+        T* cs = sums.as<T>();
+        hipLaunchKernelGGL(k_scan_chunk_sums<T>, dim3((unsigned)G), dim3(1024), 0, s, in, n, cs);
+        hipLaunchKernelGGL(k_exclusive_scan<T>, dim3(1), dim3(1024), 0, s, cs, cs + G, G);
+        hipLaunchKernelGGL(k_scan_chunks<T>, dim3((unsigned)G), dim3(1024), 0, s, in, out, n, cs + G);
+        TRSPMM_HIP(hipGetLastError());
+        return TRSPMM_OK;
+    }
     if (hipMallocAsync((void**)&chunk_sums, sizeof(T) * (size_t)(2 * G + 1), s) != hipSuccess) {
         (void)hipGetLastError();
         set_error("exclusive_scan: stream-ordered allocation of " + std::to_string(2 * G + 1) + " chunk sums failed");

@@ -28,7 +28,7 @@ EXPORTS = [
     "trspmm_spmm_t_sddmm", "trspmm_plan_create_ex", "trspmm_plan_from_meta_ex", "trspmm_match_2to4",
     "trspmm_match_2to4_hash", "trspmm_match_2to4_bucket", "trspmm_match_2to4_online", "trspmm_match_2to4_online_noalloc",
     "trspmm_minhash", "trspmm_minhash_coefficients", "trspmm_tca_order", "trspmm_plan_set_values",
-    "trspmm_bwd_plan_set_values",
+    "trspmm_bwd_plan_set_values", "trspmm_cache_trim",
 ]
 
 ENGINE_DEFAULT, ENGINE_VALU, ENGINE_MFMA = 0, 1, 2
@@ -102,6 +102,7 @@ def lib():
     L = ctypes.CDLL(LIB_PATH)
     L.trspmm_last_error.restype = ctypes.c_char_p
     L.trspmm_version.restype = ctypes.c_int
+    L.trspmm_cache_trim.argtypes = [ctypes.POINTER(_I64)]
     L.trspmm_device_arch.argtypes = [ctypes.c_char_p, ctypes.c_int]
     L.trspmm_route.argtypes = [_P, _P, _P, _I64, _I64, _I64, ctypes.POINTER(RouteCfg), _P, ctypes.POINTER(_P)]
     L.trspmm_meta_count.restype = _I64
