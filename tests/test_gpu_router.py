@@ -109,3 +109,28 @@ def test_router_rejects_bad_row_pointers(cuda):
     rp = torch.tensor([0, 3, 2], dtype=torch.int32)
     with pytest.raises(RuntimeError, match="row pointers"):
         route(rp, torch.tensor([1, 2], dtype=torch.int32), torch.ones(2), 8)
+
+
+def test_router_block_cache_reuse_and_trim(cuda):
+    """Router scratch and results come from the device block cache: results
+    that stay alive keep their blocks while other routes reuse the idle ones,
+    trimming frees the idle blocks, and routing after a trim is unchanged."""
+    import ctypes
+
+    from trspmm import _lib
+    from trspmm import graphs as g
+    a, b = CASES["rmat10"](), g.rmat_gcn(12, seed=9)
+    metas = []
+    for csr in (a, b, a, b):
+        metas.append(route(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val),
+                           csr.n, mode="exact"))
+    for csr, meta in zip((a, b, a, b), metas):
+        ref = oracle.route(csr.rowptr, csr.col, csr.val, exact=True)
+        got = meta.tensors("cpu")
+        assert all(np.array_equal(gt.numpy(), ref[n]) for n, gt in zip(META_NAMES, got))
+    del metas
+    freed = ctypes.c_int64(-1)
+    assert _lib.lib().trspmm_cache_trim(ctypes.byref(freed)) == 0
+    assert freed.value > 0                      # the scratch of four routes and the freed results
+    assert _lib.lib().trspmm_cache_trim(ctypes.byref(freed)) == 0 and freed.value == 0
+    _compare(b, {}, "exact")

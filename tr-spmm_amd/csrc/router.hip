@@ -863,62 +863,107 @@ __global__ __launch_bounds__(64 * NW) void k_window_decide_wg(RunArrays R, WinOu
     // g. pairing (block_online.cpp:365-391) on wave 0: in sorted order, a unit
     // not yet taken (index > hw) takes the first index above hw that no earlier
     // active unit holds, other than itself.  hw only grows, so the search keeps
-    // one 64-flag word of the bitmap in scalar registers (indices sb .. sb + 63)
-    // and loads the next word when hw passes it; a mark sets the bit there when
-    // it falls in that word, and always in the bitmap
+    // the 64-flag word that holds hw + 1 in scalar registers (indices sb ..
+    // sb + 63): a unit marked inside that word sets its bit there only (the
+    // word is never read again once hw passes it), one marked beyond it sets
+    // its bit in the bitmap, from which the word is loaded when hw reaches it.
+    // A ballot per 64 units skips those already at or below hw.  Pairs are
+    // collected in two registers (pair k in lane k % 64) and stored 64 at a time.
     if (wid != 0) return;
-    auto load_word = [&](int32_t wi) -> uint64_t {   // word wi, indices >= nu read as taken
-        const int32_t i0 = 64 * wi;
-        if (i0 >= nu) return ~0ull;
-        const uint64_t x = fl.load(wi);
-        uint64_t u = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32)) << 32) |
-                     (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
-        if (nu - i0 < 64) u |= ~0ull << (nu - i0);
-        return u;
-    };
-    int32_t sb = 0;
-    uint64_t sw = load_word(0);
-    // first index >= from whose flag is clear, other than `except` (nu if none)
-    auto find_free = [&](int32_t from, int32_t except) -> int32_t {
-        while (from < nu) {
-            if ((from >> 6) != (sb >> 6)) {
-                sb = from & ~63;
+    int32_t hw = -1, np = 0, left_k = nu;
+    auto pairing = [&](auto bits) {
+        auto load_word = [&](int32_t wi) -> uint64_t {   // word wi, indices >= nu read as taken
+            const int32_t i0 = 64 * wi;
+            if (i0 >= nu) return ~0ull;
+            uint64_t u = bits.load(wi);
+            if (nu - i0 < 64) u |= ~0ull << (nu - i0);
+            return u;
+        };
+        int32_t sb = 0;
+        uint64_t sw = load_word(0);
+        // first index > hw whose flag is clear, other than `except` (nu if none);
+        // leaves sb / sw on the word searched last
+        auto search = [&](int32_t except) -> int32_t {
+            uint64_t m = ~sw & (~0ull << (hw + 1 - sb));
+            const uint32_t de = (uint32_t)(except - sb);
+            if (de < 64u) m &= ~(1ull << de);
+            while (m == 0) {
+                sb += 64;
+                if (sb >= nu) return nu;
                 sw = load_word(sb >> 6);
+                m = ~sw;
+                const uint32_t d2 = (uint32_t)(except - sb);
+                if (d2 < 64u) m &= ~(1ull << d2);
             }
-            uint64_t m = ~sw & (~0ull << (from - sb));
-            if (except >= sb && except < sb + 64) m &= ~(1ull << (except - sb));
-            if (m) return sb + (int32_t)__builtin_ctzll(m);
-            from = sb + 64;
+            return sb + (int32_t)__builtin_ctzll(m);
+        };
+        uint32_t va = 0, vb = 0;
+        int32_t last = -1;                           // a unit that found no partner (then no later one can)
+        for (int32_t i0 = 0; i0 < nu && last < 0; i0 += kWave) {
+            const int32_t mine = i0 + lane < nu ? (int32_t)(a[i0 + lane] & 0xffffffu) : -1;
+            uint64_t cand = __ballot(mine > hw);
+            while (cand) {
+                const int j = (int)__builtin_ctzll(cand);
+                cand &= cand - 1;
+                const int32_t v = __builtin_amdgcn_readlane(mine, j);
+                if (v <= hw) continue;                   // already a partner
+                const int32_t p = search(v);
+                if (p >= nu) {                           // every index above hw but v is taken
+                    last = v;
+                    break;
+                }
+                const uint32_t dv = (uint32_t)(v - sb);
+                if (dv < 64u) sw |= 1ull << dv;
+                else bits.mark(v, lane);
+                if (lane == (np & (kWave - 1))) {
+                    va = (uint32_t)v;
+                    vb = (uint32_t)p;
+                }
+                hw = p;
+                if (((hw + 1) >> 6) != (sb >> 6)) {      // keep the word of hw + 1 at hand
+                    sb = (hw + 1) & ~63;
+                    sw = load_word(sb >> 6);
+                }
+                if ((++np & (kWave - 1)) == 0) {
+                    R.pair_a[rb + np - kWave + lane] = (int32_t)va;
+                    R.pair_b[rb + np - kWave + lane] = (int32_t)vb;
+                }
+            }
         }
-        return nu;
+        if (lane < (np & (kWave - 1))) {
+            R.pair_a[rb + (np & ~(kWave - 1)) + lane] = (int32_t)va;
+            R.pair_b[rb + (np & ~(kWave - 1)) + lane] = (int32_t)vb;
+        }
+        if (last >= 0) left_k = last;            // the one index above hw still free
+        else if (2 * np < nu) left_k = search(-1);
     };
-    int32_t hw = -1, np = 0, my_a = 0, my_b = 0;
-    for (int32_t i0 = 0; i0 < nu; i0 += kWave) {
-        const int32_t mine = i0 + lane < nu ? (int32_t)(a[i0 + lane] & 0xffffffu) : 0;
-        const int32_t cnt = nu - i0 < kWave ? nu - i0 : kWave;
-        for (int32_t j = 0; j < cnt; ++j) {
-            const int32_t v = __builtin_amdgcn_readlane(mine, j);
-            if (v <= hw) continue;                   // already a partner
-            const int32_t p = find_free(hw + 1, v);
-            if (p >= nu) continue;                   // no partner: v stays unused
-            if (lane == 0) fl.mark(v);
-            if (v >= sb && v < sb + 64) sw |= 1ull << (v - sb);
-            if (lane == (np & (kWave - 1))) {
-                my_a = v;
-                my_b = p;
+    if (in_lds) {
+        struct {
+            uint64_t* w;
+            __device__ uint64_t load(int32_t i) const {
+                const uint64_t x = w[i];
+                return (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x) |
+                       ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32)) << 32);
             }
-            hw = p;
-            if ((++np & (kWave - 1)) == 0) {
-                R.pair_a[rb + np - kWave + lane] = my_a;
-                R.pair_b[rb + np - kWave + lane] = my_b;
+            __device__ void mark(int32_t k, int ln) const {
+                if (ln == 0) atomicOr((unsigned long long*)(w + (k >> 6)), 1ull << (k & 63));
             }
-        }
+        } bits{(uint64_t*)(dyn_wg + lds_cap)};
+        pairing(bits);
+    } else {
+        struct {
+            uint64_t* w;
+            __device__ uint64_t load(int32_t i) const {
+                const uint64_t x = __hip_atomic_load(w + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x) |
+                       ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32)) << 32);
+            }
+            __device__ void mark(int32_t k, int ln) const {
+                if (ln == 0) atomicOr((unsigned long long*)(w + (k >> 6)), 1ull << (k & 63));
+            }
+        } bits{fl.w};
+        pairing(bits);
     }
-    if (lane < (np & (kWave - 1))) {
-        R.pair_a[rb + (np & ~(kWave - 1)) + lane] = my_a;
-        R.pair_b[rb + (np & ~(kWave - 1)) + lane] = my_b;
-    }
-    const int32_t left_k = 2 * np < nu ? find_free(hw + 1, -1) : nu;
     if (lane == 0) {
         const int32_t left = left_k < nu ? left_k : -1;
         int32_t left_nnz = 0;
