@@ -20,6 +20,12 @@ struct BwdPlan {
     Plan at;
     DevBuf pos, perm, D;
     bool routed = false;
+    // the buffers are blocks of the device block cache: wait for the device
+    // (as hipFree would) before they go back
+    BwdPlan() = default;
+    BwdPlan(const BwdPlan&) = delete;
+    BwdPlan& operator=(const BwdPlan&) = delete;
+    ~BwdPlan() { (void)hipDeviceSynchronize(); }
 };
 
 int bwd_plan_build(const int32_t* t_rowptr, const int32_t* t_col, const float* t_val, const int32_t* perm,

@@ -506,6 +506,7 @@ int launch_bwd_typed(const BwdPlan& P, const void* dY, int64_t lddy, const void*
 
 int csr_transpose(const int32_t* rowptr, const int32_t* col, const float* val, int64_t rows, int64_t nnz,
                   int64_t ncols, int32_t* t_rowptr, int32_t* t_col, float* t_val, int32_t* perm, hipStream_t s) {
+    StreamArena A(s);      // temporaries (declared first: back to the cache after the stream is idle)
     TRSPMM_CHECK(rows >= 0 && nnz >= 0 && ncols >= 0, "csr_transpose: negative size");
     TRSPMM_CHECK(nnz < (int64_t)1 << 31 && ncols < (int64_t)1 << 31 && rows < (int64_t)1 << 31,
                  "csr_transpose: sizes must fit int32");
@@ -518,7 +519,7 @@ int csr_transpose(const int32_t* rowptr, const int32_t* col, const float* val, i
         return TRSPMM_OK;
     }
     DevBuf err, keys_sorted, ids;
-    if ((rc = err.alloc(4)) || (rc = keys_sorted.alloc(4 * nnz)) || (rc = ids.alloc(4 * nnz))) return rc;
+    if ((rc = err.alloc(A, 4)) || (rc = keys_sorted.alloc(A, 4 * nnz)) || (rc = ids.alloc(A, 4 * nnz))) return rc;
     TRSPMM_HIP(hipMemsetAsync(err.p, 0, 4, s));
     hipLaunchKernelGGL(k_check_cols, dim3(nblk(nnz)), dim3(256), 0, s, col, nnz, ncols, err.as<int32_t>());
     TRSPMM_HIP(hipGetLastError());
@@ -534,7 +535,7 @@ int csr_transpose(const int32_t* rowptr, const int32_t* col, const float* val, i
     TRSPMM_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, col, keys_sorted.as<int32_t>(), ids.as<int32_t>(), perm,
                                          (size_t)nnz, 0, bits, s));
     DevBuf tmp;
-    if ((rc = tmp.alloc(tmp_bytes ? tmp_bytes : 1))) return rc;
+    if ((rc = tmp.alloc(A, tmp_bytes ? tmp_bytes : 1))) return rc;
     TRSPMM_HIP(rocprim::radix_sort_pairs(tmp.p, tmp_bytes, col, keys_sorted.as<int32_t>(), ids.as<int32_t>(), perm,
                                          (size_t)nnz, 0, bits, s));
     hipLaunchKernelGGL(k_transpose_fill, dim3(nblk(nnz)), dim3(256), 0, s, rowptr, rows, val, perm, nnz, t_col, t_val);
@@ -571,6 +572,7 @@ static int build_routed(const int32_t* t_rowptr, const int32_t* t_col, const flo
 
 int bwd_plan_build(const int32_t* t_rowptr, const int32_t* t_col, const float* t_val, const int32_t* perm,
                    int64_t rows, int64_t nnz, int64_t ncols, int dtype, hipStream_t s, BwdPlan& P) {
+    StreamArena A(s);      // temporaries (declared first: back to the cache after the stream is idle)
     TRSPMM_CHECK(rows >= 0 && nnz >= 0 && ncols >= 0, "bwd plan: negative size");
     TRSPMM_CHECK(nnz < (int64_t)1 << 31 && rows < (int64_t)1 << 31, "bwd plan: sizes must fit int32");
     TRSPMM_CHECK(dtype == TRSPMM_F16 || dtype == TRSPMM_BF16, "bwd plan: dtype must be TRSPMM_F16 or TRSPMM_BF16");
@@ -631,10 +633,10 @@ int bwd_plan_build(const int32_t* t_rowptr, const int32_t* t_col, const float* t
     P.n_zero = (int64_t)zero.size();
     int rc;
     const int64_t n1 = nnz > 0 ? nnz : 1;
-    if ((rc = P.rowptr.alloc(4 * (rows + 1))) || (rc = P.col.alloc(4 * n1)) || (rc = P.val.alloc(2 * n1)) ||
-        (rc = P.invperm.alloc(4 * n1)) || (rc = P.dvals_t.alloc(4 * n1)) ||
-        (rc = P.items.alloc(sizeof(int4) * (items.size() ? items.size() : 1))) ||
-        (rc = P.zero_rows.alloc(4 * (zero.size() ? zero.size() : 1))))
+    if ((rc = P.rowptr.alloc_cached(4 * (rows + 1))) || (rc = P.col.alloc_cached(4 * n1)) || (rc = P.val.alloc_cached(2 * n1)) ||
+        (rc = P.invperm.alloc_cached(4 * n1)) || (rc = P.dvals_t.alloc_cached(4 * n1)) ||
+        (rc = P.items.alloc_cached(sizeof(int4) * (items.size() ? items.size() : 1))) ||
+        (rc = P.zero_rows.alloc_cached(4 * (zero.size() ? zero.size() : 1))))
         return rc;
     TRSPMM_HIP(hipMemcpyAsync(P.rowptr.p, t_rowptr, 4 * (rows + 1), hipMemcpyDeviceToDevice, s));
     if (!items.empty())
@@ -646,7 +648,7 @@ int bwd_plan_build(const int32_t* t_rowptr, const int32_t* t_col, const float* t
         // permutation of [0, nnz) (entries in range, and the inverse built from
         // it maps every entry back to its own position)
         DevBuf err;
-        if ((rc = err.alloc(8))) return rc;
+        if ((rc = err.alloc(A, 8))) return rc;
         TRSPMM_HIP(hipMemsetAsync(err.p, 0, 8, s));
         hipLaunchKernelGGL(k_check_cols, dim3(nblk(nnz)), dim3(256), 0, s, t_col, nnz, ncols, err.as<int32_t>());
         TRSPMM_HIP(hipGetLastError());
@@ -679,6 +681,7 @@ int bwd_plan_build(const int32_t* t_rowptr, const int32_t* t_col, const float* t
 // TRSPMM_BACKWARD=lanes in the environment keeps the lane-group kernel instead.
 static int build_routed(const int32_t* t_rowptr, const int32_t* t_col, const float* t_val, const int32_t* perm,
                         int64_t rows, int64_t nnz, int64_t ncols, int dtype, hipStream_t s, BwdPlan& P) {
+    StreamArena A(s);      // temporaries (declared first: back to the cache after the stream is idle)
     const char* env = std::getenv("TRSPMM_BACKWARD");
     if ((env && std::strcmp(env, "lanes") == 0) || nnz == 0 || rows == 0 || ncols == 0) return TRSPMM_OK;
     trspmm_route_cfg cfg{};
@@ -704,8 +707,8 @@ static int build_routed(const int32_t* t_rowptr, const int32_t* t_col, const flo
     }
     if (!fused_sddmm_supported(P.at.dev) || P.at.dev.n_rval != nnz) return TRSPMM_OK;
     DevBuf cnt;
-    if ((rc = P.pos.alloc(4 * nnz)) || (rc = P.perm.alloc(4 * nnz)) || (rc = P.D.alloc(4 * nnz)) ||
-        (rc = cnt.alloc(4)))
+    if ((rc = P.pos.alloc_cached(4 * nnz)) || (rc = P.perm.alloc_cached(4 * nnz)) || (rc = P.D.alloc_cached(4 * nnz)) ||
+        (rc = cnt.alloc(A, 4)))
         return rc;
     TRSPMM_HIP(hipMemsetAsync(P.pos.p, 0xff, 4 * nnz, s));
     TRSPMM_HIP(hipMemsetAsync(cnt.p, 0, 4, s));

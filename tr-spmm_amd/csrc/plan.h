@@ -28,6 +28,12 @@ struct Plan {
     trspmm_plan_opts opts{};   // requested engine / slab (0 = default)
     int finalize();
     int build_items(hipStream_t s);   // work-item list from the per-window ranges
+    // the buffers are blocks of the device block cache: wait for the device
+    // (as hipFree would) before they go back
+    Plan() = default;
+    Plan(const Plan&) = delete;
+    Plan& operator=(const Plan&) = delete;
+    ~Plan() { (void)hipDeviceSynchronize(); }
 };
 
 int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtype, hipStream_t s, Plan& P,

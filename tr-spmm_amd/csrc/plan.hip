@@ -68,36 +68,48 @@ __device__ inline void row_spans(const ResSpans& R, int32_t r, int32_t& l0, int3
     s1 = R.n_csparts ? R.cs_row_offset[d] : 0;
 }
 
-__global__ void k_res_row_counts(ResSpans R, int64_t rows, int32_t* __restrict__ cnt) {
+// per row: its long entries [l0, l1) and short entries [s0, s1) in the part
+// lists, and their count
+__global__ void k_res_row_counts(ResSpans R, int64_t rows, int32_t* __restrict__ cnt, int4* __restrict__ spans) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows) return;
     int32_t l0, l1, s0, s1;
     row_spans(R, (int32_t)r, l0, l1, s0, s1);
     cnt[r] = (l1 - l0) + (s1 - s0);
+    spans[r] = int4{l0, l1, s0, s1};
 }
 
-// one wave per row: copies the row's long then short entries
-__global__ void k_res_row_fill(ResSpans R, int64_t rows, const int32_t* __restrict__ rowptr,
-                               const int32_t* __restrict__ c_column, const void* __restrict__ c_value,
-                               const int32_t* __restrict__ cs_column, const void* __restrict__ cs_value,
-                               int value_is_f32, int dtype, int64_t ncols, int32_t* __restrict__ cols,
-                               uint16_t* __restrict__ vals, const int32_t* __restrict__ c_edge,
-                               const int32_t* __restrict__ cs_edge, int32_t* __restrict__ edge) {
-    const int64_t r = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-    if (r >= rows) return;
-    const int lane = threadIdx.x & 63;
-    int32_t l0, l1, s0, s1;
-    row_spans(R, (int32_t)r, l0, l1, s0, s1);
-    const int32_t base = rowptr[r], nl = l1 - l0;
-    for (int32_t i = lane; i < nl; i += 64) {
-        cols[base + i] = clean_col(c_column[l0 + i], ncols);
-        vals[base + i] = to_plan_bits(c_value, l0 + i, value_is_f32, dtype);
-        if (edge) edge[base + i] = c_edge[l0 + i];
-    }
-    for (int32_t i = lane; i < s1 - s0; i += 64) {
-        cols[base + nl + i] = clean_col(cs_column[s0 + i], ncols);
-        vals[base + nl + i] = to_plan_bits(cs_value, s0 + i, value_is_f32, dtype);
-        if (edge) edge[base + nl + i] = cs_edge[s0 + i];
+// 32 lanes per part: a part's entries go to its row at the part's place in the
+// row's long list (SHORT = 0) or after the long entries (SHORT = 1).  A part
+// outside its row's span (part rows not sorted, or out of range) is skipped
+// and reported in *bad.
+template <int SHORT>
+__global__ void k_res_part_fill(const int32_t* __restrict__ prow, const int32_t* __restrict__ poff, int64_t nparts,
+                                int64_t rows, const int4* __restrict__ spans, const int32_t* __restrict__ rowptr,
+                                const int32_t* __restrict__ column, const void* __restrict__ value,
+                                int value_is_f32, int dtype, int64_t ncols, int32_t* __restrict__ cols,
+                                uint16_t* __restrict__ vals, const int32_t* __restrict__ src_edge,
+                                int32_t* __restrict__ edge, int32_t* __restrict__ bad) {
+    const int lane = threadIdx.x & 31;
+    const int64_t stride = ((int64_t)gridDim.x * blockDim.x) >> 5;
+    for (int64_t pt = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 5; pt < nparts; pt += stride) {
+        const int32_t r = prow[pt], b = poff[pt], e = poff[pt + 1];
+        if (r < 0 || r >= rows) {
+            if (lane == 0) atomicOr(bad, 1);
+            continue;
+        }
+        const int4 sp = spans[r];
+        const int32_t lo = SHORT ? sp.z : sp.x, hi = SHORT ? sp.w : sp.y;
+        if (b < lo || e > hi || b > e) {
+            if (lane == 0) atomicOr(bad, 1);
+            continue;
+        }
+        const int32_t base = rowptr[r] + (SHORT ? (sp.y - sp.x) + (b - sp.z) : (b - sp.x));
+        for (int32_t i = lane; i < e - b; i += 32) {
+            cols[base + i] = clean_col(column[b + i], ncols);
+            vals[base + i] = to_plan_bits(value, b + i, value_is_f32, dtype);
+            if (edge) edge[base + i] = src_edge[b + i];
+        }
     }
 }
 
@@ -146,21 +158,20 @@ __device__ inline void slot_pair(const float* s_value, const int8_t* s_pos, cons
     }
 }
 
-// One workgroup per window; thread (tile j, row r) fills the row's 4 A words,
-// its index word and (row 0) the tile's columns.
+// Thread (tile t, row r), tile t being tile j of its window, fills the row's
+// 4 A words, its index word and (row 0) the tile's columns.
 __global__ void k_tiles_from_groups(const int32_t* __restrict__ s_offsets, const int32_t* __restrict__ s_column,
                                     const float* __restrict__ s_value, const int8_t* __restrict__ s_pos,
                                     const int32_t* __restrict__ s_edge, const int32_t* __restrict__ tile_off,
-                                    int64_t W, int window, int64_t ncols, int dtype, int32_t* __restrict__ sp_cols,
-                                    uint32_t* __restrict__ sp_a, uint16_t* __restrict__ sp_idx,
-                                    int2* __restrict__ sp_edge) {
-    const int64_t w = blockIdx.x;
-    if (w >= W) return;
-    const int32_t g0 = s_offsets[w], g1 = s_offsets[w + 1];
-    const int32_t t0 = tile_off[w], nt = tile_off[w + 1] - t0;
-    for (int32_t i = threadIdx.x; i < nt * 16; i += blockDim.x) {
-        const int32_t j = i >> 4, r = i & 15;
-        const int64_t t = t0 + j;
+                                    int64_t W, int64_t T, int window, int64_t ncols, int dtype,
+                                    int32_t* __restrict__ sp_cols, uint32_t* __restrict__ sp_a,
+                                    uint16_t* __restrict__ sp_idx, int2* __restrict__ sp_edge) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < T * 16; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = i >> 4;
+        const int r = (int)(i & 15);
+        const int64_t w = lower_bound(tile_off, W + 1, (int32_t)t + 1) - 1;   // last window starting at or before t
+        const int32_t g0 = s_offsets[w], g1 = s_offsets[w + 1];
+        const int32_t j = (int32_t)(t - tile_off[w]);
         uint32_t idx = 0;
         for (int c = 0; c < 4; ++c) {
             const int32_t g = g0 + 4 * j + c;
@@ -553,6 +564,7 @@ static int sort_items_heaviest_first(Plan& P, int64_t n, hipStream_t s) {
 }
 
 int Plan::build_items(hipStream_t s) {
+    StreamArena A(s);      // temporaries (declared first: back to the cache after the stream is idle)
     const int64_t W = dev.windows;
     if (W == 0) {
         dev.n_titems = dev.n_ritems = dev.n_split = 0;
@@ -560,24 +572,24 @@ int Plan::build_items(hipStream_t s) {
     }
     DevBuf nt, nr, sp, toff, roff, soff;
     int rc;
-    if ((rc = nt.alloc(4 * (W + 1))) || (rc = nr.alloc(4 * (W + 1))) || (rc = sp.alloc(4 * (W + 1))) ||
-        (rc = toff.alloc(4 * (W + 1))) || (rc = roff.alloc(4 * (W + 1))) || (rc = soff.alloc(4 * (W + 1))))
+    if ((rc = nt.alloc(A, 4 * (W + 1))) || (rc = nr.alloc(A, 4 * (W + 1))) || (rc = sp.alloc(A, 4 * (W + 1))) ||
+        (rc = toff.alloc(A, 4 * (W + 1))) || (rc = roff.alloc(A, 4 * (W + 1))) || (rc = soff.alloc(A, 4 * (W + 1))))
         return rc;
     hipLaunchKernelGGL(k_item_counts, dim3(blocks_for(W)), dim3(256), 0, s, tc_sub_off.as<int32_t>(),
                        sp_tile_off.as<int32_t>(), res_rowptr.as<int32_t>(), W, caps, (int)dev.res_mfma,
                        nt.as<int32_t>(), nr.as<int32_t>(), sp.as<int32_t>());
     TRSPMM_HIP(hipGetLastError());
-    if ((rc = exclusive_scan<int32_t>(nt.as<int32_t>(), toff.as<int32_t>(), W, s)) ||
-        (rc = exclusive_scan<int32_t>(nr.as<int32_t>(), roff.as<int32_t>(), W, s)) ||
-        (rc = exclusive_scan<int32_t>(sp.as<int32_t>(), soff.as<int32_t>(), W, s)))
+    if ((rc = exclusive_scan<int32_t>(nt.as<int32_t>(), toff.as<int32_t>(), W, s, &A)) ||
+        (rc = exclusive_scan<int32_t>(nr.as<int32_t>(), roff.as<int32_t>(), W, s, &A)) ||
+        (rc = exclusive_scan<int32_t>(sp.as<int32_t>(), soff.as<int32_t>(), W, s, &A)))
         return rc;
     int32_t tot[3] = {0, 0, 0};
     TRSPMM_HIP(hipMemcpyAsync(&tot[0], toff.as<int32_t>() + W, 4, hipMemcpyDeviceToHost, s));
     TRSPMM_HIP(hipMemcpyAsync(&tot[1], roff.as<int32_t>() + W, 4, hipMemcpyDeviceToHost, s));
     TRSPMM_HIP(hipMemcpyAsync(&tot[2], soff.as<int32_t>() + W, 4, hipMemcpyDeviceToHost, s));
     TRSPMM_HIP(hipStreamSynchronize(s));
-    if ((rc = titems.alloc(16 * (int64_t)(tot[0] ? tot[0] : 1))) || (rc = ritems.alloc(16 * (int64_t)(tot[1] ? tot[1] : 1))) ||
-        (rc = split_windows.alloc(4 * (int64_t)(tot[2] ? tot[2] : 1))) || (rc = queue.alloc(4 * kQueueInts)))
+    if ((rc = titems.alloc_cached(16 * (int64_t)(tot[0] ? tot[0] : 1))) || (rc = ritems.alloc_cached(16 * (int64_t)(tot[1] ? tot[1] : 1))) ||
+        (rc = split_windows.alloc_cached(4 * (int64_t)(tot[2] ? tot[2] : 1))) || (rc = queue.alloc_cached(4 * kQueueInts)))
         return rc;
     TRSPMM_HIP(hipMemsetAsync(queue.as<int32_t>(), 0, 4 * kQueueInts, s));      // the persistent tile queue starts empty
     hipLaunchKernelGGL(k_fill_items, dim3(blocks_for(W)), dim3(256), 0, s, tc_sub_off.as<int32_t>(),
@@ -603,32 +615,33 @@ int Plan::build_items(hipStream_t s) {
 // windows, one wave per window) took 0.139 ms with the entries by column
 // against 0.172 ms in row order.
 static int sort_records_by_column(Plan& P, int64_t rows, int64_t total, hipStream_t s) {
+    StreamArena A(s);      // temporaries (declared first: back to the cache after the stream is idle)
     const int64_t W = P.dev.windows;
     int wbits = 1;
     while (wbits < 31 && ((int64_t)1 << wbits) < W) ++wbits;
     DevBuf keys, keys2, rec2, tmp;
     int rc;
-    if ((rc = keys.alloc(8 * total)) || (rc = keys2.alloc(8 * total)) || (rc = rec2.alloc(8 * total))) return rc;
+    if ((rc = keys.alloc(A, 8 * total)) || (rc = keys2.alloc(A, 8 * total)) || (rc = rec2.alloc(A, 8 * total))) return rc;
     hipLaunchKernelGGL(k_res_sort_keys, dim3(blocks_for(rows, 4)), dim3(256), 0, s, P.res_rowptr.as<int32_t>(), rows,
                        P.res_cols.as<int32_t>(), keys.as<uint64_t>());
     TRSPMM_HIP(hipGetLastError());
     size_t tmp_bytes = 0;
     TRSPMM_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys.as<uint64_t>(), keys2.as<uint64_t>(),
                                          P.res_rec.as<uint64_t>(), rec2.as<uint64_t>(), (size_t)total, 0, 32 + wbits, s));
-    if ((rc = tmp.alloc(tmp_bytes ? tmp_bytes : 1))) return rc;
+    if ((rc = tmp.alloc(A, tmp_bytes ? tmp_bytes : 1))) return rc;
     TRSPMM_HIP(rocprim::radix_sort_pairs(tmp.p, tmp_bytes, keys.as<uint64_t>(), keys2.as<uint64_t>(),
                                          P.res_rec.as<uint64_t>(), rec2.as<uint64_t>(), (size_t)total, 0, 32 + wbits, s));
     TRSPMM_HIP(hipMemcpyAsync(P.res_rec.p, rec2.p, 8 * total, hipMemcpyDeviceToDevice, s));
     if (P.has_edges) {
         // the same keys through the same stable sort: the edges land beside their records
-        if ((rc = P.rec_edge.alloc(4 * total))) return rc;
+        if ((rc = P.rec_edge.alloc_cached(4 * total))) return rc;
         size_t eb = 0;
         TRSPMM_HIP(rocprim::radix_sort_pairs(nullptr, eb, keys.as<uint64_t>(), keys2.as<uint64_t>(),
                                              P.res_edge.as<int32_t>(), P.rec_edge.as<int32_t>(), (size_t)total, 0,
                                              32 + wbits, s));
         if (eb > tmp_bytes) {
             tmp_bytes = eb;
-            if ((rc = tmp.alloc(tmp_bytes))) return rc;
+            if ((rc = tmp.alloc(A, tmp_bytes))) return rc;
         }
         TRSPMM_HIP(rocprim::radix_sort_pairs(tmp.p, tmp_bytes, keys.as<uint64_t>(), keys2.as<uint64_t>(),
                                              P.res_edge.as<int32_t>(), P.rec_edge.as<int32_t>(), (size_t)total, 0,
@@ -641,16 +654,20 @@ static int sort_records_by_column(Plan& P, int64_t rows, int64_t total, hipStrea
 // residual row CSR from the long/short part lists
 static int build_residual(const trspmm_meta_view& v, int64_t ncols, int dtype, hipStream_t s, Plan& P,
                           const EdgeMaps* em) {
+    StreamArena A(s);      // temporaries (declared first: back to the cache after the stream is idle)
     const int64_t rows = 16 * P.dev.windows;
     int rc;
-    DevBuf cnt;
-    if ((rc = cnt.alloc(4 * (rows + 1))) || (rc = P.res_rowptr.alloc(4 * (rows + 1)))) return rc;
+    DevBuf cnt, spans;
+    if ((rc = cnt.alloc(A, 4 * (rows + 1))) || (rc = spans.alloc(A, 16 * (rows + 1))) ||
+        (rc = P.res_rowptr.alloc_cached(4 * (rows + 1))))
+        return rc;
     ResSpans RS{v.c_row, v.c_row_offset, v.cs_row, v.cs_row_offset, v.n_cparts, v.n_csparts};
     if (rows > 0) {
-        hipLaunchKernelGGL(k_res_row_counts, dim3(blocks_for(rows)), dim3(256), 0, s, RS, rows, cnt.as<int32_t>());
+        hipLaunchKernelGGL(k_res_row_counts, dim3(blocks_for(rows)), dim3(256), 0, s, RS, rows, cnt.as<int32_t>(),
+                           spans.as<int4>());
         TRSPMM_HIP(hipGetLastError());
     }
-    if ((rc = exclusive_scan<int32_t>(rows ? cnt.as<int32_t>() : nullptr, P.res_rowptr.as<int32_t>(), rows, s)))
+    if ((rc = exclusive_scan<int32_t>(rows ? cnt.as<int32_t>() : nullptr, P.res_rowptr.as<int32_t>(), rows, s, &A)))
         return rc;
     int32_t total = 0;
     TRSPMM_HIP(hipMemcpyAsync(&total, P.res_rowptr.as<int32_t>() + rows, 4, hipMemcpyDeviceToHost, s));
@@ -658,15 +675,29 @@ static int build_residual(const trspmm_meta_view& v, int64_t ncols, int dtype, h
     TRSPMM_CHECK(total == v.n_cval + v.n_csval, "residual parts must lie inside the windows' rows");
     P.dev.n_rval = total;
     if (total > 0) {
-        if ((rc = P.res_cols.alloc(4 * (int64_t)total)) || (rc = P.res_vals.alloc(2 * (int64_t)total))) return rc;
-        if (em && (rc = P.res_edge.alloc(4 * (int64_t)total))) return rc;
-        hipLaunchKernelGGL(k_res_row_fill, dim3(blocks_for(rows, 4)), dim3(256), 0, s, RS, rows,
-                           P.res_rowptr.as<int32_t>(), v.c_column, v.c_value, v.cs_column, v.cs_value,
-                           v.c_value_is_f32, dtype, ncols, P.res_cols.as<int32_t>(), P.res_vals.as<uint16_t>(),
-                           em ? em->c_edge : nullptr, em ? em->cs_edge : nullptr,
-                           em ? P.res_edge.as<int32_t>() : nullptr);
+        if ((rc = P.res_cols.alloc_cached(4 * (int64_t)total)) || (rc = P.res_vals.alloc_cached(2 * (int64_t)total))) return rc;
+        if (em && (rc = P.res_edge.alloc_cached(4 * (int64_t)total))) return rc;
+        DevBuf bad;
+        if ((rc = bad.alloc(A, 4))) return rc;
+        TRSPMM_HIP(hipMemsetAsync(bad.p, 0, 4, s));
+        if (v.n_cparts > 0)
+            hipLaunchKernelGGL(k_res_part_fill<0>, dim3(blocks_for(v.n_cparts, 8)), dim3(256), 0, s, v.c_row,
+                               v.c_row_offset, v.n_cparts, rows, spans.as<int4>(), P.res_rowptr.as<int32_t>(),
+                               v.c_column, v.c_value, v.c_value_is_f32, dtype, ncols, P.res_cols.as<int32_t>(),
+                               P.res_vals.as<uint16_t>(), em ? em->c_edge : nullptr,
+                               em ? P.res_edge.as<int32_t>() : nullptr, bad.as<int32_t>());
+        if (v.n_csparts > 0)
+            hipLaunchKernelGGL(k_res_part_fill<1>, dim3(blocks_for(v.n_csparts, 8)), dim3(256), 0, s, v.cs_row,
+                               v.cs_row_offset, v.n_csparts, rows, spans.as<int4>(), P.res_rowptr.as<int32_t>(),
+                               v.cs_column, v.cs_value, v.c_value_is_f32, dtype, ncols, P.res_cols.as<int32_t>(),
+                               P.res_vals.as<uint16_t>(), em ? em->cs_edge : nullptr,
+                               em ? P.res_edge.as<int32_t>() : nullptr, bad.as<int32_t>());
         TRSPMM_HIP(hipGetLastError());
-        if ((rc = P.res_rec.alloc(8 * (int64_t)total))) return rc;
+        int32_t hbad = 0;
+        TRSPMM_HIP(hipMemcpyAsync(&hbad, bad.p, 4, hipMemcpyDeviceToHost, s));
+        TRSPMM_HIP(hipStreamSynchronize(s));
+        TRSPMM_CHECK(hbad == 0, "residual parts must list rows in non-decreasing order, inside the windows");
+        if ((rc = P.res_rec.alloc_cached(8 * (int64_t)total))) return rc;
         hipLaunchKernelGGL(k_res_records, dim3(blocks_for(rows, 4)), dim3(256), 0, s, P.res_rowptr.as<int32_t>(),
                            rows, P.res_cols.as<int32_t>(), P.res_vals.as<uint16_t>(), P.res_rec.as<uint64_t>());
         TRSPMM_HIP(hipGetLastError());
@@ -702,6 +733,7 @@ static int tile_slab(const trspmm_plan_opts& o) {
 // (mGCNkernel_online.cu:291-299) and its TC kernel rows t_window_row + r.
 static int build_plan_narrow(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtype, hipStream_t s,
                              Plan& P, const NarrowSrc* packed = nullptr) {
+    StreamArena A(s);      // temporaries (declared first: back to the cache after the stream is idle)
     TRSPMM_CHECK(v.window >= 1 && v.window < 16, "narrow windows: window must be in [1, 16)");
     TRSPMM_CHECK(v.n_windows >= 0 && v.n_windows * v.window >= m_out, "windows must cover the output rows");
     const int64_t W = (std::max<int64_t>(m_out, v.n_windows * v.window) + 15) / 16;
@@ -728,15 +760,15 @@ static int build_plan_narrow(const trspmm_meta_view& v, int64_t m_out, int64_t n
         S.n_ptiles = packed->n_ptiles;
     }
     int rc;
-    if ((rc = P.tc_sub_off.alloc(4 * (W + 1))) || (rc = P.sp_tile_off.alloc(4 * (W + 1))) ||
-        (rc = P.res_rowptr.alloc(4 * (rows + 1))))
+    if ((rc = P.tc_sub_off.alloc_cached(4 * (W + 1))) || (rc = P.sp_tile_off.alloc_cached(4 * (W + 1))) ||
+        (rc = P.res_rowptr.alloc_cached(4 * (rows + 1))))
         return rc;
     TRSPMM_HIP(hipMemsetAsync(P.tc_sub_off.p, 0, 4 * (W + 1), s));
     TRSPMM_HIP(hipMemsetAsync(P.sp_tile_off.p, 0, 4 * (W + 1), s));
     DevBuf keys, keys2, idx, idx2, cols, vals, tmp;
     const int64_t n1 = n > 0 ? n : 1;
-    if ((rc = keys.alloc(8 * n1)) || (rc = keys2.alloc(8 * n1)) || (rc = idx.alloc(4 * n1)) ||
-        (rc = idx2.alloc(4 * n1)) || (rc = cols.alloc(4 * n1)) || (rc = vals.alloc(2 * n1)))
+    if ((rc = keys.alloc(A, 8 * n1)) || (rc = keys2.alloc(A, 8 * n1)) || (rc = idx.alloc(A, 4 * n1)) ||
+        (rc = idx2.alloc(A, 4 * n1)) || (rc = cols.alloc(A, 4 * n1)) || (rc = vals.alloc(A, 2 * n1)))
         return rc;
     if (v.n_tsub > 0) {
         hipLaunchKernelGGL(k_narrow_tc, dim3(blocks_for(v.n_tsub)), dim3(256), 0, s, S, keys.as<uint64_t>(),
@@ -765,7 +797,7 @@ static int build_plan_narrow(const trspmm_meta_view& v, int64_t m_out, int64_t n
         size_t tmp_bytes = 0;
         TRSPMM_HIP(rocprim::radix_sort_keys(nullptr, tmp_bytes, keys.as<uint64_t>(), keys2.as<uint64_t>(), (size_t)n,
                                             0, 32 + rbits, s));
-        if ((rc = tmp.alloc(tmp_bytes ? tmp_bytes : 1))) return rc;
+        if ((rc = tmp.alloc(A, tmp_bytes ? tmp_bytes : 1))) return rc;
         TRSPMM_HIP(rocprim::radix_sort_keys(tmp.p, tmp_bytes, keys.as<uint64_t>(), keys2.as<uint64_t>(), (size_t)n,
                                             0, 32 + rbits, s));
     }
@@ -780,8 +812,8 @@ static int build_plan_narrow(const trspmm_meta_view& v, int64_t m_out, int64_t n
         hipLaunchKernelGGL(k_low_words, dim3(blocks_for(total)), dim3(256), 0, s, keys2.as<uint64_t>(), (int64_t)total,
                            idx.as<uint32_t>());
         TRSPMM_HIP(hipGetLastError());
-        if ((rc = P.res_cols.alloc(4 * (int64_t)total)) || (rc = P.res_vals.alloc(2 * (int64_t)total)) ||
-            (rc = P.res_rec.alloc(8 * (int64_t)total)))
+        if ((rc = P.res_cols.alloc_cached(4 * (int64_t)total)) || (rc = P.res_vals.alloc_cached(2 * (int64_t)total)) ||
+            (rc = P.res_rec.alloc_cached(8 * (int64_t)total)))
             return rc;
         hipLaunchKernelGGL(k_narrow_gather, dim3(blocks_for(total)), dim3(256), 0, s, idx.as<uint32_t>(),
                            (int64_t)total, cols.as<int32_t>(), vals.as<uint16_t>(), P.res_cols.as<int32_t>(),
@@ -798,6 +830,7 @@ static int build_plan_narrow(const trspmm_meta_view& v, int64_t m_out, int64_t n
 
 int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtype, hipStream_t s, Plan& P,
                const EdgeMaps* em) {
+    StreamArena A(s);      // temporaries (declared first: back to the cache after the stream is idle)
     TRSPMM_CHECK(v.window >= 1 && v.window <= 16, "window must be in [1, 16]");
     TRSPMM_CHECK(P.opts.residual_engine >= 0 && P.opts.residual_engine <= TRSPMM_ENGINE_MFMA,
                  "plan options: residual_engine must be TRSPMM_ENGINE_DEFAULT, _VALU or _MFMA");
@@ -841,8 +874,8 @@ int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtyp
 
     int rc;
     DevBuf tile_cnt;
-    if ((rc = P.tc_sub_off.alloc(4 * (W + 1))) || (rc = P.sp_tile_off.alloc(4 * (W + 1))) ||
-        (rc = tile_cnt.alloc(4 * (W + 1))))
+    if ((rc = P.tc_sub_off.alloc_cached(4 * (W + 1))) || (rc = P.sp_tile_off.alloc_cached(4 * (W + 1))) ||
+        (rc = tile_cnt.alloc(A, 4 * (W + 1))))
         return rc;
     hipLaunchKernelGGL(k_tc_bounds, dim3(blocks_for(W + 1)), dim3(256), 0, s, v.t_window_row, v.n_tparts,
                        v.t_row_offset, W, P.tc_sub_off.as<int32_t>());
@@ -850,8 +883,8 @@ int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtyp
 
     // TC sub-blocks
     if (v.n_tsub > 0) {
-        if ((rc = P.tc_cols.alloc(4 * 8 * v.n_tsub)) || (rc = P.tc_bits.alloc(8 * 2 * v.n_tsub)) ||
-            (rc = P.tc_valoff.alloc(4 * (v.n_tsub + 1))) || (rc = P.tc_vals.alloc(2 * (v.n_tval > 0 ? v.n_tval : 1))))
+        if ((rc = P.tc_cols.alloc_cached(4 * 8 * v.n_tsub)) || (rc = P.tc_bits.alloc_cached(8 * 2 * v.n_tsub)) ||
+            (rc = P.tc_valoff.alloc_cached(4 * (v.n_tsub + 1))) || (rc = P.tc_vals.alloc_cached(2 * (v.n_tval > 0 ? v.n_tval : 1))))
             return rc;
         hipLaunchKernelGGL(k_copy_cols, dim3(blocks_for(8 * v.n_tsub)), dim3(256), 0, s, v.t_column, 8 * v.n_tsub,
                            ncols, P.tc_cols.as<int32_t>());
@@ -862,10 +895,10 @@ int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtyp
                                v.t_value_is_f32, dtype, P.tc_vals.as<uint16_t>());
         TRSPMM_HIP(hipGetLastError());
         if (em && v.n_tval > 0) {
-            if ((rc = P.tc_edge.alloc(4 * v.n_tval))) return rc;
+            if ((rc = P.tc_edge.alloc_cached(4 * v.n_tval))) return rc;
             TRSPMM_HIP(hipMemcpyAsync(P.tc_edge.p, em->t_edge, 4 * v.n_tval, hipMemcpyDeviceToDevice, s));
         }
-        if ((rc = P.tc_dense.alloc(2 * 16 * 8 * v.n_tsub))) return rc;
+        if ((rc = P.tc_dense.alloc_cached(2 * 16 * 8 * v.n_tsub))) return rc;
         hipLaunchKernelGGL(k_tc_dense, dim3(blocks_for(16 * v.n_tsub)), dim3(256), 0, s, P.tc_bits.as<uint64_t>(),
                            P.tc_valoff.as<int32_t>(), P.tc_vals.as<uint16_t>(), v.n_tsub, P.tc_dense.as<uint16_t>());
         TRSPMM_HIP(hipGetLastError());
@@ -879,20 +912,21 @@ int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtyp
         hipLaunchKernelGGL(k_tile_counts, dim3(blocks_for(W)), dim3(256), 0, s, v.s_offsets, W, tile_cnt.as<int32_t>());
         TRSPMM_HIP(hipGetLastError());
     }
-    if ((rc = exclusive_scan<int32_t>(W > 0 ? tile_cnt.as<int32_t>() : nullptr, P.sp_tile_off.as<int32_t>(), W, s)))
+    if ((rc = exclusive_scan<int32_t>(W > 0 ? tile_cnt.as<int32_t>() : nullptr, P.sp_tile_off.as<int32_t>(), W, s, &A)))
         return rc;
     int32_t n_tiles = 0;
     TRSPMM_HIP(hipMemcpyAsync(&n_tiles, P.sp_tile_off.as<int32_t>() + W, 4, hipMemcpyDeviceToHost, s));
     TRSPMM_HIP(hipStreamSynchronize(s));
     P.dev.n_tiles = n_tiles;
     if (n_tiles > 0) {
-        if ((rc = P.sp_cols.alloc(4 * 16 * (int64_t)n_tiles)) || (rc = P.sp_a.alloc(4 * 64 * (int64_t)n_tiles)) ||
-            (rc = P.sp_idx.alloc(2 * 16 * (int64_t)n_tiles)))
+        if ((rc = P.sp_cols.alloc_cached(4 * 16 * (int64_t)n_tiles)) || (rc = P.sp_a.alloc_cached(4 * 64 * (int64_t)n_tiles)) ||
+            (rc = P.sp_idx.alloc_cached(2 * 16 * (int64_t)n_tiles)))
             return rc;
-        if (em && (rc = P.sp_edge.alloc(8 * 64 * (int64_t)n_tiles))) return rc;
-        hipLaunchKernelGGL(k_tiles_from_groups, dim3((unsigned)W), dim3(256), 0, s, v.s_offsets, v.s_column,
-                           v.s_value, v.s_pos, em ? em->s_edge : nullptr, P.sp_tile_off.as<int32_t>(), W, v.window,
-                           ncols, dtype, P.sp_cols.as<int32_t>(), P.sp_a.as<uint32_t>(), P.sp_idx.as<uint16_t>(),
+        if (em && (rc = P.sp_edge.alloc_cached(8 * 64 * (int64_t)n_tiles))) return rc;
+        hipLaunchKernelGGL(k_tiles_from_groups, dim3(blocks_for(16 * (int64_t)n_tiles)), dim3(256), 0, s,
+                           v.s_offsets, v.s_column, v.s_value, v.s_pos, em ? em->s_edge : nullptr,
+                           P.sp_tile_off.as<int32_t>(), W, (int64_t)n_tiles, v.window, ncols, dtype,
+                           P.sp_cols.as<int32_t>(), P.sp_a.as<uint32_t>(), P.sp_idx.as<uint16_t>(),
                            em ? P.sp_edge.as<int2>() : nullptr);
         TRSPMM_HIP(hipGetLastError());
     }
@@ -928,14 +962,14 @@ int build_plan_packed(const int32_t* s_tile_column, const int32_t* s_packed_a, c
     P.dev.dtype = TRSPMM_F16;
     P.dev.n_tiles = T;
     int rc;
-    if ((rc = P.tc_sub_off.alloc(4 * (W + 1))) || (rc = P.res_rowptr.alloc(4 * (16 * W + 1))) ||
-        (rc = P.sp_tile_off.alloc(4 * (W + 1))))
+    if ((rc = P.tc_sub_off.alloc_cached(4 * (W + 1))) || (rc = P.res_rowptr.alloc_cached(4 * (16 * W + 1))) ||
+        (rc = P.sp_tile_off.alloc_cached(4 * (W + 1))))
         return rc;
     TRSPMM_HIP(hipMemsetAsync(P.tc_sub_off.p, 0, 4 * (W + 1), s));
     TRSPMM_HIP(hipMemsetAsync(P.res_rowptr.p, 0, 4 * (16 * W + 1), s));
     TRSPMM_HIP(hipMemcpyAsync(P.sp_tile_off.p, s_window_tile_offset, 4 * (W + 1), hipMemcpyDeviceToDevice, s));
     if (T > 0) {
-        if ((rc = P.sp_cols.alloc(4 * 16 * T)) || (rc = P.sp_a.alloc(4 * 64 * T)) || (rc = P.sp_idx.alloc(2 * 16 * T)))
+        if ((rc = P.sp_cols.alloc_cached(4 * 16 * T)) || (rc = P.sp_a.alloc_cached(4 * 64 * T)) || (rc = P.sp_idx.alloc_cached(2 * 16 * T)))
             return rc;
         hipLaunchKernelGGL(k_tiles_from_packed, dim3(blocks_for(T * 64)), dim3(256), 0, s, s_tile_column,
                            (const uint32_t*)s_packed_a, (const uint32_t*)s_packed_meta, T, ncols,
