@@ -624,6 +624,7 @@ def main():
         "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": a.dtype, "data": "synthetic (seeded generators, trspmm.graphs)",
         "config": {"workload": a.workload, "rows": csr.m, "nnz": csr.nnz, "K": K, "mode": a.mode,
+                   "rows_reordered": A.meta.reordered,
                    "rel_err_fp64": err, "router_s": round(A.meta.seconds, 4), "exact_ms": exact_ms,
                    "ms_cold": round(ms_cold, 5), "wall_s_timed": round(wall, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

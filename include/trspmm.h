@@ -96,7 +96,17 @@ typedef struct trspmm_route_cfg {
  * (no reference counterpart).  Plans built from such a router result carry
  * the maps and accept new values with trspmm_plan_set_values; the 29 arrays
  * are unchanged. */
-enum { TRSPMM_ROUTE_EDGES = 1 };
+enum { TRSPMM_ROUTE_EDGES = 1, TRSPMM_ROUTE_REORDER = 2 };
+/* TRSPMM_ROUTE_REORDER (EXACT or TUNED mode, window 16; no reference
+ * counterpart, in the spirit of the reference caller's row reordering,
+ * spmm/tr_csr_fp16/tr_tca_reorder.py): the router also routes the rows in
+ * order of (first column, nonzeros descending, row index), empty rows last,
+ * and keeps that routing when its routes gather fewer B rows per multiply
+ * (TC sub-block columns + 2:4 tile columns + residual entries; the input
+ * order on a tie).  The arrays then describe the reordered rows,
+ * trspmm_meta_row_order gives the order, edge maps still name the input's
+ * edges, and plans built from the result write every row to its input
+ * index. */
 
 typedef struct trspmm_meta trspmm_meta;
 
@@ -111,6 +121,11 @@ int trspmm_route(const int32_t* d_rowptr, const int32_t* d_col, const float* d_v
 int64_t trspmm_meta_count(const trspmm_meta* meta, int index);  /* elements of array `index` */
 void* trspmm_meta_data(const trspmm_meta* meta, int index);       /* device pointer */
 double trspmm_meta_seconds(const trspmm_meta* meta);             /* router wall time */
+/* Device pointer to the routed rows' input indices (routed row i is input row
+ * order[i], `rows` entries) when the result kept a reordered routing
+ * (TRSPMM_ROUTE_REORDER); NULL when the rows kept their order.  No reference
+ * counterpart. */
+const int32_t* trspmm_meta_row_order(const trspmm_meta* meta);
 void trspmm_meta_free(trspmm_meta* meta);
 
 /* ------------------------------------------------------------------ plans */

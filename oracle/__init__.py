@@ -13,6 +13,10 @@ spmm64(meta, B)   -> FP64 C = A_routed @ B
 csr_spmm64(...)   -> FP64 C = A @ B for a plain CSR (torch.sparse / scipy)
 sort_order(keys)  -> libstdc++ std::sort order of iota(n) by key desc
 f32_to_f16_bits   -> IEEE RNE binary16 bits (the router's packing conversion)
+reorder_rows(...) -> row order of the reordered routing (first column, degree desc, row)
+permute_rows(...) -> the CSR with its rows in that order, and each entry's input edge
+gathered_rows(m)  -> B rows one multiply over a routing gathers (the reorder decision)
+route_reordered   -> the routing the device router keeps under TRSPMM_ROUTE_REORDER
 """
 from __future__ import annotations
 
@@ -209,3 +213,54 @@ def f32_to_f16_bits(x):
     out = np.empty(x.size, np.uint16)
     L.oracle_f32_to_f16(_ptr(x), _ptr(out), x.size)
     return out
+
+
+# ---------------------------------------------------------------- row reordering
+# No reference counterpart: the device router's TRSPMM_ROUTE_REORDER option
+# (tr-spmm_amd/csrc/router.hip, route_device).  Restated here from its
+# definition: rows by (first column, nonzeros descending, row index), empty
+# rows last; the reordered routing is kept when it gathers fewer B rows.
+
+
+def reorder_rows(rowptr, col):
+    """order[i] = input row of routed row i."""
+    rowptr = np.asarray(rowptr, np.int64)
+    col = np.asarray(col, np.int64)
+    rows = rowptr.size - 1
+    deg = np.diff(rowptr)
+    first = np.full(rows, np.iinfo(np.int64).max, np.int64)
+    nz = deg > 0
+    first[nz] = col[rowptr[:-1][nz]]
+    return np.lexsort((np.arange(rows), -deg, first)).astype(np.int32)
+
+
+def permute_rows(rowptr, col, val, order):
+    """(rowptr', col', val', edge'): the CSR with routed row i = input row
+    order[i]; edge'[k] = the input edge of entry k."""
+    rowptr = np.asarray(rowptr, np.int64)
+    deg = np.diff(rowptr)[order]
+    prow = np.concatenate([[0], np.cumsum(deg)]).astype(np.int32)
+    edge = np.concatenate([np.arange(rowptr[r], rowptr[r + 1]) for r in order]).astype(np.int32) \
+        if len(order) else np.zeros(0, np.int32)
+    return prow, np.asarray(col)[edge].astype(np.int32), np.asarray(val)[edge].astype(np.float32), edge
+
+
+def gathered_rows(meta):
+    """TC sub-block columns + 2:4 tile columns + residual entries of a routing."""
+    a = meta.arrays if isinstance(meta, RoutedMeta) else meta
+    return int((a["t_column"] >= 0).sum() + (a["s_tile_column"] >= 0).sum()
+               + a["c_column"].size + a["c_column_short"].size)
+
+
+def route_reordered(rowptr, col, val, **kw):
+    """(meta, order): the routing the device router keeps with row reordering
+    (order None when the input order is kept), routed by this oracle."""
+    natural = route(rowptr, col, val, **kw)
+    if len(rowptr) - 1 < 2 or len(col) == 0:
+        return natural, None
+    order = reorder_rows(rowptr, col)
+    prow, pcol, pval, _ = permute_rows(rowptr, col, val, order)
+    other = route(prow, pcol, pval, **kw)
+    if gathered_rows(other) < gathered_rows(natural):
+        return other, order
+    return natural, None

@@ -241,6 +241,8 @@ void* trspmm_meta_data(const trspmm_meta* meta, int index) {
 
 double trspmm_meta_seconds(const trspmm_meta* meta) { return meta ? meta->meta.seconds : 0.0; }
 
+const int32_t* trspmm_meta_row_order(const trspmm_meta* meta) { return meta ? meta->meta.row_order : nullptr; }
+
 void trspmm_meta_free(trspmm_meta* meta) { delete meta; }
 
 // ---------------------------------------------------------------- plans
@@ -279,7 +281,7 @@ int trspmm_plan_from_meta_ex(const trspmm_meta* meta, int64_t m_out, int64_t nco
     TRSPMM_NOTNULL(meta, "meta");
     const Meta& M = meta->meta;
     const trspmm_meta_view v = meta_view(M);
-    if (!M.has_edges) return trspmm_plan_create_ex(&v, m_out, ncols, dtype, opts, stream, out);
+    if (!M.has_edges && !M.row_order) return trspmm_plan_create_ex(&v, m_out, ncols, dtype, opts, stream, out);
     TRSPMM_NOTNULL(out, "out");
     *out = nullptr;
     trspmm_plan* p = new (std::nothrow) trspmm_plan();
@@ -289,7 +291,8 @@ int trspmm_plan_from_meta_ex(const trspmm_meta* meta, int64_t m_out, int64_t nco
     }
     if (opts) p->plan.opts = *opts;
     const EdgeMaps em = meta_edges(M);
-    const int rc = build_plan(v, m_out, ncols, dtype, (hipStream_t)stream, p->plan, &em);
+    const int rc = build_plan(v, m_out, ncols, dtype, (hipStream_t)stream, p->plan, M.has_edges ? &em : nullptr,
+                              M.row_order, M.rows);
     if (rc != TRSPMM_OK) {
         delete p;
         return rc;

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <utility>
 
 #include "../../include/trspmm.h"
 
@@ -118,6 +119,12 @@ struct DevBuf {
         bytes = 0;
         arena = nullptr;
         cached = false;
+    }
+    void swap(DevBuf& o) {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+        std::swap(arena, o.arena);
+        std::swap(cached, o.cached);
     }
     template <typename T> T* as() const { return (T*)p; }
     ~DevBuf() { release(); }
@@ -263,6 +270,9 @@ struct PlanDev {
     int64_t n_ritems = 0;
     const int32_t* split_windows = nullptr;
     int64_t n_split = 0;
+    // plan row 16 w + r -> output row (-1: none), for a reordered routing; null:
+    // plan row i is output row i (rows at or beyond m_out dropped)
+    const int32_t* row_map = nullptr;
 };
 
 // work-item caps: a window is split when it has more TC sub-blocks, 2:4

@@ -17,7 +17,7 @@ struct Plan {
     DevBuf tc_sub_off, tc_cols, tc_bits, tc_valoff, tc_vals, tc_dense;
     DevBuf sp_tile_off, sp_cols, sp_a, sp_idx;
     DevBuf res_rowptr, res_cols, res_vals, res_rec;
-    DevBuf titems, ritems, split_windows, queue;
+    DevBuf titems, ritems, split_windows, queue, row_map;
     // edge maps (plans built with EdgeMaps): the CSR edge behind tc_vals[i],
     // the (low, high) halves of sp_a[i], res_vals[i] and res_rec[i]
     // (rec_edge empty when the records are in res_vals order)
@@ -36,8 +36,10 @@ struct Plan {
     ~Plan() { (void)hipDeviceSynchronize(); }
 };
 
+// row_order (device, order_rows entries, may be null): the routed rows' output
+// rows, from a reordered routing (routed row i is output row row_order[i])
 int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtype, hipStream_t s, Plan& P,
-               const EdgeMaps* edges = nullptr);
+               const EdgeMaps* edges = nullptr, const int32_t* row_order = nullptr, int64_t order_rows = 0);
 // new values for a plan with edge maps: value of CSR edge e = vals[index ? index[e] : e]
 // (0 outside [0, n_vals)), converted to the plan's type as at build time
 int plan_set_values(Plan& P, const float* vals, int64_t n_vals, const int32_t* index, int64_t nnz, hipStream_t s);

@@ -828,8 +828,17 @@ static int build_plan_narrow(const trspmm_meta_view& v, int64_t m_out, int64_t n
     return P.build_items(s);
 }
 
+// plan row i -> output row: row_order[i] when it is an output row, else -1
+__global__ void k_row_map(const int32_t* __restrict__ order, int64_t n_order, int64_t m_out, int64_t n,
+                          int32_t* __restrict__ dst) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t r = i < n_order ? order[i] : -1;
+        dst[i] = (r >= 0 && (int64_t)r < m_out) ? r : -1;
+    }
+}
+
 int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtype, hipStream_t s, Plan& P,
-               const EdgeMaps* em) {
+               const EdgeMaps* em, const int32_t* row_order, int64_t order_rows) {
     StreamArena A(s);      // temporaries (declared first: back to the cache after the stream is idle)
     TRSPMM_CHECK(v.window >= 1 && v.window <= 16, "window must be in [1, 16]");
     TRSPMM_CHECK(P.opts.residual_engine >= 0 && P.opts.residual_engine <= TRSPMM_ENGINE_MFMA,
@@ -873,6 +882,19 @@ int build_plan(const trspmm_meta_view& v, int64_t m_out, int64_t ncols, int dtyp
     P.src_nnz = em ? em->nnz : 0;
 
     int rc;
+    if (row_order) {
+        // the row map is applied by the deep tile kernel's stores only
+        if (P.dev.tile_kernel != 0 || !P.dev.res_mfma || P.dev.item_order == 2) {
+            set_error("a reordered routing runs on the deep tile kernel with the matrix-core residual engine "
+                      "(plan options tile_kernel 0, residual engine default or MFMA, item_order 0, 1 or 3)");
+            return TRSPMM_EUNSUPPORTED;
+        }
+        if ((rc = P.row_map.alloc_cached(4 * 16 * (W > 0 ? W : 1)))) return rc;
+        hipLaunchKernelGGL(k_row_map, dim3(blocks_for(16 * W)), dim3(256), 0, s, row_order, order_rows, m_out, 16 * W,
+                           P.row_map.as<int32_t>());
+        TRSPMM_HIP(hipGetLastError());
+        P.dev.row_map = P.row_map.as<int32_t>();
+    }
     DevBuf tile_cnt;
     if ((rc = P.tc_sub_off.alloc_cached(4 * (W + 1))) || (rc = P.sp_tile_off.alloc_cached(4 * (W + 1))) ||
         (rc = tile_cnt.alloc(A, 4 * (W + 1))))

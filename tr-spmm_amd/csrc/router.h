@@ -44,6 +44,30 @@ struct Meta {
         edge_data[i] = edge_buf[i].p;
         return rc;
     }
+    // TRSPMM_ROUTE_REORDER, when the reordered routing was kept: the rows in the
+    // order they were routed (routed row i is row_order[i] of the input, `rows`
+    // entries); null when the rows kept their order
+    DevBuf row_order_buf;
+    const int32_t* row_order = nullptr;
+    void swap(Meta& o) {
+        for (int i = 0; i < 29; ++i) {
+            buf[i].swap(o.buf[i]);
+            std::swap(data[i], o.data[i]);
+            std::swap(count[i], o.count[i]);
+        }
+        for (int i = 0; i < 4; ++i) {
+            edge_buf[i].swap(o.edge_buf[i]);
+            std::swap(edge_data[i], o.edge_data[i]);
+            std::swap(edge_count[i], o.edge_count[i]);
+        }
+        std::swap(has_edges, o.has_edges);
+        std::swap(window, o.window);
+        std::swap(rows, o.rows);
+        std::swap(nnz, o.nnz);
+        std::swap(seconds, o.seconds);
+        row_order_buf.swap(o.row_order_buf);
+        std::swap(row_order, o.row_order);
+    }
     Meta() = default;
     Meta(const Meta&) = delete;
     Meta& operator=(const Meta&) = delete;

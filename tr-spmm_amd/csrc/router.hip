@@ -1471,11 +1471,70 @@ struct EventSet {
     }
 };
 
+// ---------------------------------------------------------------- row reordering
+// TRSPMM_ROUTE_REORDER: the rows in order of (first column, nonzeros
+// descending, row), so that rows whose lowest column is the same hub sit in
+// the same windows, the longest of them first; empty rows go last.  On a
+// power-law graph a window then holds columns that reach the TC density or
+// fill 2:4 groups, which one gathered B row serves for several rows.
+__global__ void k_row_keys(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, int64_t rows,
+                           uint64_t* __restrict__ keys, int32_t* __restrict__ idx) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t b = rowptr[r], e = rowptr[r + 1];
+        const uint32_t first = e > b ? (uint32_t)col[b] : 0xffffffffu;
+        keys[r] = ((uint64_t)first << 32) | (uint64_t)(0xffffffffu - (uint32_t)(e - b));
+        idx[r] = (int32_t)r;
+    }
+}
+
+__global__ void k_perm_counts(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ order, int64_t rows,
+                              int32_t* __restrict__ cnt) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t src = order[r];
+        cnt[r] = rowptr[src + 1] - rowptr[src];
+    }
+}
+
+// one wave per routed row: the entries of its input row, and the input edge of each
+__global__ __launch_bounds__(256) void k_perm_rows(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                                   const float* __restrict__ val, const int32_t* __restrict__ order,
+                                                   const int32_t* __restrict__ prowptr, int64_t rows,
+                                                   int32_t* __restrict__ pcol, float* __restrict__ pval,
+                                                   int32_t* __restrict__ pedge) {
+    const int64_t r = wave_row();
+    if (r >= rows) return;
+    const int lane = threadIdx.x & 63;
+    const int32_t src = order[r], b = rowptr[src], n = rowptr[src + 1] - b, d = prowptr[r];
+    for (int32_t i = lane; i < n; i += kWave) {
+        pcol[d + i] = col[b + i];
+        pval[d + i] = val[b + i];
+        pedge[d + i] = b + i;
+    }
+}
+
+// edge maps of the reordered routing: routed edge -> input edge (-1 stays)
+__global__ void k_map_edges(int32_t* __restrict__ e, int64_t n, const int32_t* __restrict__ pedge) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t x = e[i];
+        if (x >= 0) e[i] = pedge[x];
+    }
+}
+
+// entries >= 0 of a[n] (column lists: -1 marks an empty slot), added to *out
+__global__ __launch_bounds__(256) void k_count_valid(const int32_t* __restrict__ a, int64_t n,
+                                                     unsigned long long* __restrict__ out) {
+    unsigned long long c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        c += a[i] >= 0;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- host driver
-int route_device(const int32_t* rowptr, const int32_t* col, const float* val, int64_t rows, int64_t nnz,
-                 int64_t ncols, const trspmm_route_cfg& cfg, hipStream_t s, Meta& M) {
+static int route_core(const int32_t* rowptr, const int32_t* col, const float* val, int64_t rows, int64_t nnz,
+                      int64_t ncols, const trspmm_route_cfg& cfg, hipStream_t s, Meta& M) {
     const auto t_begin = std::chrono::steady_clock::now();
     StreamArena A(s);      // scratch: declared first, so that every buffer below is back before it waits
     TRSPMM_CHECK(cfg.window > 0 && cfg.window <= 16, "window must be in [1, 16]");
@@ -1484,7 +1543,7 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
     TRSPMM_CHECK(cfg.mode == TRSPMM_MODE_COMPAT || cfg.mode == TRSPMM_MODE_EXACT || cfg.mode == TRSPMM_MODE_TUNED,
                  "router mode must be COMPAT, EXACT or TUNED");
     TRSPMM_CHECK(cfg.sptc_min_nnz >= 0 && cfg.sptc_min_nnz <= 33, "sptc_min_nnz must be in [0, 33]");
-    TRSPMM_CHECK((cfg.flags & ~TRSPMM_ROUTE_EDGES) == 0, "unknown route flags");
+    TRSPMM_CHECK((cfg.flags & ~(TRSPMM_ROUTE_EDGES | TRSPMM_ROUTE_REORDER)) == 0, "unknown route flags");
     TRSPMM_CHECK(rows >= 0 && nnz >= 0 && nnz < ((int64_t)1 << 31), "bad rows / nnz");
     TRSPMM_CHECK(ncols > 0 && ncols < ((int64_t)1 << 31), "ncols must be in [1, 2^31)");
     const int window = cfg.window;
@@ -1823,6 +1882,87 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
     TRSPMM_HIP(hipStreamSynchronize(s));
     const auto t_end = std::chrono::steady_clock::now();
     M.seconds = std::chrono::duration<double>(t_end - t_begin).count();
+    return TRSPMM_OK;
+}
+
+namespace {
+
+// B rows one multiply over a routing gathers: TC sub-block columns, 2:4 tile
+// columns and residual entries (the quantity the reordering is judged by)
+int gathered_rows(const Meta& M, hipStream_t s, StreamArena& A, int64_t* out) {
+    DevBuf c;
+    int rc;
+    if ((rc = c.alloc(A, 8))) return rc;
+    TRSPMM_HIP(hipMemsetAsync(c.p, 0, 8, s));
+    const int idx[2] = {2, 25};          // t_column [8 S], s_tile_column [16 T]
+    for (int i : idx)
+        if (M.count[i] > 0) {
+            hipLaunchKernelGGL(k_count_valid, dim3(nblk_gs(M.count[i])), dim3(256), 0, s, (const int32_t*)M.data[i],
+                               M.count[i], c.as<unsigned long long>());
+            TRSPMM_HIP(hipGetLastError());
+        }
+    unsigned long long h = 0;
+    TRSPMM_HIP(hipMemcpyAsync(&h, c.p, 8, hipMemcpyDeviceToHost, s));
+    TRSPMM_HIP(hipStreamSynchronize(s));
+    *out = (int64_t)h + M.count[10] + M.count[15];     // + c_column, c_column_short
+    return TRSPMM_OK;
+}
+
+}  // namespace
+
+int route_device(const int32_t* rowptr, const int32_t* col, const float* val, int64_t rows, int64_t nnz,
+                 int64_t ncols, const trspmm_route_cfg& cfg, hipStream_t s, Meta& M) {
+    const auto t_begin = std::chrono::steady_clock::now();
+    const bool reorder = (cfg.flags & TRSPMM_ROUTE_REORDER) != 0;
+    TRSPMM_CHECK(!reorder || (cfg.mode != TRSPMM_MODE_COMPAT && cfg.window == 16),
+                 "row reordering needs EXACT or TUNED mode and window 16");
+    int rc = route_core(rowptr, col, val, rows, nnz, ncols, cfg, s, M);
+    if (rc || !reorder || rows < 2 || nnz == 0) return rc;
+
+    // the rows in reordered form, routed the same way; the routing that
+    // gathers fewer B rows is kept (the input order on a tie)
+    StreamArena A(s);
+    DevBuf keys, keys2, idx, order, cnt, prowptr, pcol, pval, pedge, tmp;
+    if ((rc = keys.alloc(A, 8 * rows)) || (rc = keys2.alloc(A, 8 * rows)) || (rc = idx.alloc(A, 4 * rows)) ||
+        (rc = order.alloc_cached(4 * rows)) || (rc = cnt.alloc(A, 4 * rows)) || (rc = prowptr.alloc(A, 4 * (rows + 1))) ||
+        (rc = pcol.alloc(A, 4 * nnz)) || (rc = pval.alloc(A, 4 * nnz)) || (rc = pedge.alloc(A, 4 * nnz)))
+        return rc;
+    hipLaunchKernelGGL(k_row_keys, dim3(nblk_gs(rows)), dim3(256), 0, s, rowptr, col, rows, keys.as<uint64_t>(),
+                       idx.as<int32_t>());
+    TRSPMM_HIP(hipGetLastError());
+    size_t tmp_bytes = 0;
+    TRSPMM_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys.as<uint64_t>(), keys2.as<uint64_t>(),
+                                         idx.as<int32_t>(), order.as<int32_t>(), (size_t)rows, 0, 64, s));
+    if ((rc = tmp.alloc(A, tmp_bytes ? tmp_bytes : 1))) return rc;
+    TRSPMM_HIP(rocprim::radix_sort_pairs(tmp.p, tmp_bytes, keys.as<uint64_t>(), keys2.as<uint64_t>(),
+                                         idx.as<int32_t>(), order.as<int32_t>(), (size_t)rows, 0, 64, s));
+    hipLaunchKernelGGL(k_perm_counts, dim3(nblk_gs(rows)), dim3(256), 0, s, rowptr, order.as<int32_t>(), rows,
+                       cnt.as<int32_t>());
+    TRSPMM_HIP(hipGetLastError());
+    if ((rc = exclusive_scan<int32_t>(cnt.as<int32_t>(), prowptr.as<int32_t>(), rows, s, &A))) return rc;
+    hipLaunchKernelGGL(k_perm_rows, dim3(nblk(rows, 4)), dim3(256), 0, s, rowptr, col, val, order.as<int32_t>(),
+                       prowptr.as<int32_t>(), rows, pcol.as<int32_t>(), pval.as<float>(), pedge.as<int32_t>());
+    TRSPMM_HIP(hipGetLastError());
+    Meta R;
+    trspmm_route_cfg c2 = cfg;
+    c2.flags &= ~TRSPMM_ROUTE_REORDER;
+    if ((rc = route_core(prowptr.as<int32_t>(), pcol.as<int32_t>(), pval.as<float>(), rows, nnz, ncols, c2, s, R)))
+        return rc;
+    int64_t g0 = 0, g1 = 0;
+    if ((rc = gathered_rows(M, s, A, &g0)) || (rc = gathered_rows(R, s, A, &g1))) return rc;
+    if (g1 < g0) {
+        for (int i = 0; i < 4; ++i)
+            if (R.edge_count[i] > 0) {
+                hipLaunchKernelGGL(k_map_edges, dim3(nblk_gs(R.edge_count[i])), dim3(256), 0, s,
+                                   (int32_t*)R.edge_data[i], R.edge_count[i], pedge.as<int32_t>());
+                TRSPMM_HIP(hipGetLastError());
+            }
+        R.row_order_buf.swap(order);
+        R.row_order = R.row_order_buf.as<int32_t>();
+        M.swap(R);
+    }
+    TRSPMM_HIP(hipStreamSynchronize(s));
+    M.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count();
     return TRSPMM_OK;
 }
 

@@ -897,8 +897,10 @@ struct SddArgs {
     float* D;              // one float per residual record
 };
 
-// FH: 0 = the plain multiply; 1 or 2 = with the fused SDDMM, passes of FH halves
-template <int PW, typename DT, int FH = 0>
+// FH: 0 = the plain multiply; 1 or 2 = with the fused SDDMM, passes of FH halves;
+// RM: the plan has a row map (a reordered routing): tile row r goes to output
+// row p.row_map[16 w + r]
+template <int PW, typename DT, int FH = 0, bool RM = false>
 __device__ __forceinline__ void deep_item(const PlanDev& p, int64_t it, const uint16_t* B, int64_t ldb,
                                           int K, float* __restrict__ C, int64_t ldc, uint32_t mask, uint8_t* Bs,
                                           uint8_t* Ms, int lane, const SddArgs& sa = SddArgs{}) {
@@ -931,6 +933,8 @@ __device__ __forceinline__ void deep_item(const PlanDev& p, int64_t it, const ui
     const __amdgpu_buffer_rsrc_t rsB =
         __builtin_amdgcn_make_buffer_rsrc((void*)B, (short)0, (int)(uint32_t)(2 * p.ncols * ldb), (int)kBufferWord3);
     const uint32_t ldb2 = 2u * (uint32_t)ldb;
+    int32_t rmap = -1;                                         // RM: lane l < 16 holds tile row l's output row
+    if constexpr (RM) rmap = lane < 16 ? p.row_map[row0 + lane] : -1;
     for (int fp = 0; fp < K; fp += H * PW) {
         f32x4 acc[H][NB];
 #pragma unroll
@@ -983,6 +987,16 @@ __device__ __forceinline__ void deep_item(const PlanDev& p, int64_t it, const ui
             const int prow = lane / PPR, piece = lane % PPR;
             const bool pok = piece < width / 4;
             const float* tsrc = T + prow * PW + 4 * piece;
+            if constexpr (RM) {
+#pragma unroll
+                for (int k = 0; k < 16 / RPS; ++k) {
+                    const int32_t orow = __shfl(rmap, prow + RPS * k);
+                    if (pok & (orow >= 0)) {
+                        const f32x4 x = *(const f32x4*)(tsrc + RPS * k * PW);
+                        __builtin_nontemporal_store(x, (f32x4*)(C + (int64_t)orow * ldc + fh + 4 * piece));
+                    }
+                }
+            } else {
             float* cdst = Crow + (uint32_t)prow * ldc32 + (uint32_t)(fh + 4 * piece);
             const uint32_t cstep = (uint32_t)RPS * ldc32;
 #pragma unroll
@@ -990,6 +1004,20 @@ __device__ __forceinline__ void deep_item(const PlanDev& p, int64_t it, const ui
                 if (pok & (prow + RPS * k < rows_out)) {
                     const f32x4 x = *(const f32x4*)(tsrc + RPS * k * PW);
                     __builtin_nontemporal_store(x, (f32x4*)(cdst + (uint32_t)k * cstep));
+                }
+            }
+            }
+        } else if constexpr (RM) {
+            int32_t orow[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) orow[i] = __shfl(rmap, 4 * kg + i);
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                const int f = nb * 16 + r;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if ((orow[i] >= 0) & (f < width) & (acc[hf][nb][i] != 0.f))
+                        unsafeAtomicAdd(C + (int64_t)orow[i] * ldc + fh + f, acc[hf][nb][i]);
                 }
             }
         } else {
@@ -1010,7 +1038,7 @@ __device__ __forceinline__ void deep_item(const PlanDev& p, int64_t it, const ui
 
 // one workgroup (one wave) per work item, in the plan's item order (see
 // k_tile_routes); 19 KB of LDS per wave at PW = 128, two waves per SIMD
-template <int PW, typename DT, int FH = 0>
+template <int PW, typename DT, int FH = 0, bool RM = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void k_routes(PlanDev p, const uint16_t* B, int64_t ldb, int K, float* __restrict__ C, int64_t ldc,
               uint32_t mask, int64_t grid, SddArgs sa) {
@@ -1020,7 +1048,7 @@ void k_routes(PlanDev p, const uint16_t* B, int64_t ldb, int K, float* __restric
     __shared__ __align__(16) uint8_t Ms[4 * 512];          // record ring: up to 4 slots of 64 x 8 bytes
     static_assert(16 * PW * 4 <= ROWS * DeepCfg<PW>::RS, "the output tile must fit the staging space");
     const int64_t it = p.item_order == 1 ? xcd_window(blockIdx.x, grid) : (int64_t)blockIdx.x;
-    if (it < p.n_titems) deep_item<PW, DT, FH>(p, it, B, ldb, K, C, ldc, mask, Bs, Ms, threadIdx.x, sa);
+    if (it < p.n_titems) deep_item<PW, DT, FH, RM>(p, it, B, ldb, K, C, ldc, mask, Bs, Ms, threadIdx.x, sa);
 }
 
 // ============================================================ residual route
@@ -1187,12 +1215,14 @@ __global__ __launch_bounds__(256) void k_residual(PlanDev p, const uint16_t* __r
 
 // zero the output rows of split windows (their items accumulate atomically)
 __global__ __launch_bounds__(256) void k_zero_rows(const int32_t* __restrict__ split_windows, int64_t m_out, int K,
-                                                   float* __restrict__ C, int64_t ldc) {
+                                                   float* __restrict__ C, int64_t ldc,
+                                                   const int32_t* __restrict__ row_map) {
     const int64_t w = split_windows[blockIdx.x];
     for (int i = threadIdx.x; i < 16 * (K / 4); i += blockDim.x) {
-        const int64_t row = w * 16 + i / (K / 4);
+        const int64_t prow = w * 16 + i / (K / 4);
+        const int64_t row = row_map ? (int64_t)row_map[prow] : prow;
         const int f = 4 * (i % (K / 4));
-        if (row < m_out) *(f32x4*)(C + row * ldc + f) = f32x4{0.f, 0.f, 0.f, 0.f};
+        if ((row >= 0) & (row < m_out)) *(f32x4*)(C + row * ldc + f) = f32x4{0.f, 0.f, 0.f, 0.f};
     }
 }
 
@@ -1226,7 +1256,7 @@ int launch_typed(const PlanDev& p, const void* B, int64_t ldb, int64_t K, float*
                  hipStream_t s) {
     if (p.n_split > 0) {
         hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)p.n_split), dim3(256), 0, s, p.split_windows, p.m_out, (int)K,
-                           C, ldc);
+                           C, ldc, p.row_map);
         TRSPMM_HIP(hipGetLastError());
     }
     if (p.n_titems > 0) {
@@ -1241,7 +1271,20 @@ int launch_typed(const PlanDev& p, const void* B, int64_t ldb, int64_t K, float*
         }
         TRSPMM_CHECK(grid <= 0x7fffffff, "too many work items");
         // the deep kernel addresses B as a raw buffer of fewer than 2^31 bytes
-        if (p.tile_kernel == 0 && p.item_order != 2 && p.ncols * ldb < ((int64_t)1 << 30)) {
+        const bool deep = p.tile_kernel == 0 && p.item_order != 2 && p.ncols * ldb < ((int64_t)1 << 30);
+        if (p.row_map) {
+            // a reordered routing: only the deep kernel applies the row map
+            if (!deep) {
+                set_error("a plan of a reordered routing needs B under 2^30 elements (the deep tile kernel)");
+                return TRSPMM_EUNSUPPORTED;
+            }
+            if (K <= 64)
+                hipLaunchKernelGGL((k_routes<64, DT, 0, true>), dim3((unsigned)grid), dim3(64), 0, s, p,
+                                   (const uint16_t*)B, ldb, (int)K, C, ldc, mask, grid, SddArgs{});
+            else
+                hipLaunchKernelGGL((k_routes<128, DT, 0, true>), dim3((unsigned)grid), dim3(64), 0, s, p,
+                                   (const uint16_t*)B, ldb, (int)K, C, ldc, mask, grid, SddArgs{});
+        } else if (deep) {
             if (K <= 64)
                 hipLaunchKernelGGL((k_routes<64, DT>), dim3((unsigned)grid), dim3(64), 0, s, p, (const uint16_t*)B, ldb,
                                    (int)K, C, ldc, mask, grid, SddArgs{});
@@ -1280,7 +1323,7 @@ int launch_fused_typed(const PlanDev& p, const void* dY, int64_t lddy, int64_t K
                        const SddArgs& sa, hipStream_t s) {
     if (p.n_split > 0) {
         hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)p.n_split), dim3(256), 0, s, p.split_windows, p.m_out, (int)K,
-                           dB, lddb);
+                           dB, lddb, p.row_map);
         TRSPMM_HIP(hipGetLastError());
     }
     if (p.n_titems == 0) return TRSPMM_OK;

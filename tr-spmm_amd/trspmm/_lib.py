@@ -17,6 +17,7 @@ F16, BF16 = 0, 1
 ROUTE_TC, ROUTE_SPTC, ROUTE_RESIDUAL, ROUTE_ALL = 1, 2, 4, 7
 MODE_COMPAT, MODE_EXACT, MODE_TUNED = 0, 1, 2
 ROUTE_EDGES = 1   # trspmm_route_cfg.flags: record the CSR edge behind every value slot
+ROUTE_REORDER = 2  # trspmm_route_cfg.flags: also route the rows in (first column, degree) order; keep the cheaper
 
 # every symbol include/trspmm.h declares (checked by tests/test_abi.py)
 EXPORTS = [
@@ -28,7 +29,7 @@ EXPORTS = [
     "trspmm_spmm_t_sddmm", "trspmm_plan_create_ex", "trspmm_plan_from_meta_ex", "trspmm_match_2to4",
     "trspmm_match_2to4_hash", "trspmm_match_2to4_bucket", "trspmm_match_2to4_online", "trspmm_match_2to4_online_noalloc",
     "trspmm_minhash", "trspmm_minhash_coefficients", "trspmm_tca_order", "trspmm_plan_set_values",
-    "trspmm_bwd_plan_set_values", "trspmm_cache_trim",
+    "trspmm_bwd_plan_set_values", "trspmm_cache_trim", "trspmm_meta_row_order",
 ]
 
 ENGINE_DEFAULT, ENGINE_VALU, ENGINE_MFMA = 0, 1, 2
@@ -112,6 +113,8 @@ def lib():
     L.trspmm_meta_seconds.restype = ctypes.c_double
     L.trspmm_meta_seconds.argtypes = [_P]
     L.trspmm_meta_free.argtypes = [_P]
+    L.trspmm_meta_row_order.restype = _P
+    L.trspmm_meta_row_order.argtypes = [_P]
     L.trspmm_plan_create.argtypes = [ctypes.POINTER(MetaView), _I64, _I64, ctypes.c_int, _P, ctypes.POINTER(_P)]
     L.trspmm_plan_from_meta.argtypes = [_P, _I64, _I64, ctypes.c_int, _P, ctypes.POINTER(_P)]
     L.trspmm_plan_create_packed.argtypes = [_P, _P, _P, _P, _I64, _I64, ctypes.c_int32, _I64, _I64, _P,

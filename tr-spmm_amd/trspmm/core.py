@@ -70,13 +70,31 @@ MODES = {"compat": _lib.MODE_COMPAT, "exact": _lib.MODE_EXACT, "tuned": _lib.MOD
 class RoutedMeta:
     """The router's 29 arrays, resident on the GPU (owned by the library)."""
 
-    def __init__(self, handle, window: int, nnz: int = 0, edges: bool = False):
+    def __init__(self, handle, window: int, nnz: int = 0, edges: bool = False, rows: int = 0):
         self._h = handle
         self.window = window
         self.nnz = nnz              # nonzeros of the routed CSR
         self.edges = edges          # routed with edge maps (plans accept new values)
+        self.rows = rows            # rows of the routed CSR
         self.seconds = lib().trspmm_meta_seconds(handle)
         self._views = None
+
+    @property
+    def reordered(self) -> bool:
+        """The router kept the reordered routing (route(..., reorder=True))."""
+        return bool(lib().trspmm_meta_row_order(self._h))
+
+    def row_order(self, device="cpu"):
+        """Input row of each routed row (int32 [rows]) when the routing was
+        reordered, else None."""
+        src = lib().trspmm_meta_row_order(self._h)
+        if not src:
+            return None
+        t = torch.empty(self.rows, dtype=torch.int32, device=device)
+        if self.rows:
+            check(lib().trspmm_memcpy(ptr(t), ctypes.c_void_p(src), 4 * self.rows, _lib.current_stream()),
+                  "trspmm_memcpy")
+        return t
 
     def __del__(self):
         h, self._h = getattr(self, "_h", None), None
@@ -113,7 +131,7 @@ TUNED_SPTC_MIN_NNZ = 8   # default 2:4 occupancy floor of the tuned router (kept
 
 def route(rowptr: torch.Tensor, col: torch.Tensor, val: torch.Tensor, ncols: int, *, part_t=32, part_c=32,
           short_c=3, density=8, window=16, wide=16, tc_min_util=0.0, skip_tc=None, unit_min_nnz=None,
-          mode="exact", sptc_min_nnz=None, edges=False) -> RoutedMeta:
+          mode="exact", sptc_min_nnz=None, edges=False, reorder=False) -> RoutedMeta:
     """Routes a canonical CSR on the GPU (see trspmm_route).
 
     mode "compat": the reference's metadata bit for bit; "exact": the same t_*/s_*
@@ -121,7 +139,10 @@ def route(rowptr: torch.Tensor, col: torch.Tensor, val: torch.Tensor, ncols: int
     "tuned": exact plus 2:4 groups with fewer than `sptc_min_nnz` kept slots
     moved to the residual route.  edges: also record the CSR edge behind every
     value slot, so that plans built from the result accept new values
-    (Plan.set_values) without routing again."""
+    (Plan.set_values) without routing again.  reorder (exact / tuned, window 16):
+    also route the rows in order of (first column, nonzeros descending, row)
+    and keep that routing when it gathers fewer B rows per multiply; plans built
+    from it write every row to its input index (RoutedMeta.row_order)."""
     _require_cuda()
     es, eu = env_router_knobs()
     mode_code = MODES[mode] if isinstance(mode, str) else int(mode)
@@ -129,7 +150,8 @@ def route(rowptr: torch.Tensor, col: torch.Tensor, val: torch.Tensor, ncols: int
         sptc_min_nnz = TUNED_SPTC_MIN_NNZ if mode_code == _lib.MODE_TUNED else 0
     cfg = _lib.RouteCfg(part_t, part_c, short_c, density, window, wide, float(tc_min_util),
                         int(es if skip_tc is None else bool(skip_tc)), int(eu if unit_min_nnz is None else unit_min_nnz),
-                        mode_code, int(sptc_min_nnz), _lib.ROUTE_EDGES if edges else 0)
+                        mode_code, int(sptc_min_nnz),
+                        (_lib.ROUTE_EDGES if edges else 0) | (_lib.ROUTE_REORDER if reorder else 0))
     dev = torch.device("cuda", torch.cuda.current_device())
     rp = rowptr.to(dev, torch.int32).contiguous()
     ci = col.to(dev, torch.int32).contiguous()
@@ -137,7 +159,7 @@ def route(rowptr: torch.Tensor, col: torch.Tensor, val: torch.Tensor, ncols: int
     h = ctypes.c_void_p()
     check(lib().trspmm_route(ptr(rp), ptr(ci), ptr(vv), rp.numel() - 1, ci.numel(), int(ncols), ctypes.byref(cfg),
                              _lib.current_stream(), ctypes.byref(h)), "trspmm_route")
-    return RoutedMeta(h, window, ci.numel(), bool(edges))
+    return RoutedMeta(h, window, ci.numel(), bool(edges), rp.numel() - 1)
 
 
 class Plan:

@@ -12,10 +12,18 @@ pattern (`sddmm()`); `trspmm.autograd.spmm` wires both into torch.autograd.
 The router records the CSR edge behind every value slot, so new values
 (`set_values()`) refresh the plans of A and A^T and the fused-backward plan
 in place, without routing again.
+
+With `reorder` (the default for tuned routing where the plan's kernels
+support it: window 16, the deep tile kernel with the matrix-core residual
+engine) the router also tries the rows in (first column, degree) order and
+keeps that routing when it gathers fewer B rows; the plan then writes each
+row to its own index, so results are unchanged.  Exact routing keeps the
+reference's tiles unless reorder=True is passed.
 """
 from __future__ import annotations
 
 import ctypes
+import os
 import weakref
 
 import torch
@@ -40,10 +48,20 @@ class _BwdPlan:
                 pass
 
 
+def reorder_supported(mode: str, window: int, engine: str | None, plan_opts: dict) -> bool:
+    """Whether a reordered routing can run with these options: exact or tuned
+    routing at window 16, on the deep tile kernel with the matrix-core
+    residual engine (the kernel that writes rows through the plan's row map)."""
+    if engine is None:
+        engine = "valu" if os.environ.get("TRSPMM_RESIDUAL") == "valu" else "mfma"
+    return (mode in ("exact", "tuned") and window == 16 and engine == "mfma"
+            and int(plan_opts.get("tile_kernel", 0)) == 0 and int(plan_opts.get("item_order", 0)) != 2)
+
+
 class RoutedSparse:
     def __init__(self, rowptr, col, val, ncols: int, m: int | None = None, *, mode: str = "exact",
                  dtype: torch.dtype = torch.float16, engine: str | None = None, slab: int | None = None,
-                 plan_opts: dict | None = None, **router_kwargs):
+                 plan_opts: dict | None = None, reorder: bool | None = None, **router_kwargs):
         rowptr = torch.as_tensor(rowptr)
         col = torch.as_tensor(col)
         val = torch.as_tensor(val)
@@ -57,9 +75,12 @@ class RoutedSparse:
         self.mode = mode
         self.router_kwargs = dict(router_kwargs)
         self.window = int(router_kwargs.get("window", 16))
-        self.meta = route(rowptr, col, val, self.ncols, mode=mode, edges=True, **router_kwargs)
         self.engine, self.slab = engine, slab
         self.plan_opts = dict(plan_opts or {})      # tile_cap / tc_cap / res_cap / item_order
+        if reorder is None:
+            reorder = mode == "tuned" and reorder_supported(mode, self.window, engine, self.plan_opts)
+        self.reorder = bool(reorder)
+        self.meta = route(rowptr, col, val, self.ncols, mode=mode, edges=True, reorder=self.reorder, **router_kwargs)
         self.plan = Plan.from_meta(self.meta, self.m, self.ncols, dtype, engine=engine, slab=slab, **self.plan_opts)
         # device CSR (validated by the router): the SDDMM pattern and the
         # source of the transposed plan
@@ -144,7 +165,7 @@ class RoutedSparse:
         if self._t is None:
             t_rowptr, t_col, t_val, _ = self._transposed_csr()
             self._t = RoutedSparse(t_rowptr, t_col, t_val, self.m, self.ncols, mode=self.mode, dtype=self.dtype,
-                                   **self.router_kwargs)
+                                   reorder=self.reorder, **self.router_kwargs)
         return self._t
 
     def _backward_plan(self):
