@@ -280,7 +280,9 @@ int trspmm_sddmm(const int32_t* d_rowptr, int64_t rows, const int32_t* d_col, in
 /* Fused backward over the rows of A^T: one pass computes both
  *   dB[N, K]  = A^T · dY          (fp32, ldd: every row written)
  *   dvals[e]  = <dY[row_e], B[col_e]>   (fp32, A's edge order)
- * gathering each dY row once for both.  The plan is built once from A^T as
+ * gathering each dY row once for both: A^T is routed into TC sub-blocks
+ * (every column two rows of a window share) and residual records, its rows
+ * reordered when that gathers fewer dY rows.  The plan is built once from A^T as
  * trspmm_csr_transpose returns it (t_rowptr[rows + 1], t_col, t_val, perm;
  * device memory; rows = ncols of A, ncols = rows of A) and holds its own
  * copies, the values in `dtype`, and a scratch array of nnz floats: calls on
@@ -297,6 +299,10 @@ void trspmm_bwd_plan_free(trspmm_bwd_plan* plan);
  * fused-backward plan; stream-ordered, no allocation, must not overlap a call
  * on the same plan.  No reference counterpart. */
 int trspmm_bwd_plan_set_values(trspmm_bwd_plan* plan, const float* vals, int64_t nnz, void* stream);
+/* What the fused backward runs on (no reference counterpart): counts5 =
+ * [1 if the routed fused kernel runs (else the lane-group kernels), A^T's TC
+ * sub-blocks, TC nonzeros, residual records, 1 if A^T's rows were reordered]. */
+int trspmm_bwd_plan_stats(const trspmm_bwd_plan* plan, int64_t* counts5);
 int trspmm_spmm_t_sddmm(const trspmm_bwd_plan* plan, const void* dY, int64_t lddy, const void* B,
                         int64_t ldb, int64_t K, float* dB, int64_t lddb, float* dvals, void* stream);
 

@@ -429,6 +429,18 @@ int trspmm_bwd_plan_set_values(trspmm_bwd_plan* plan, const float* vals, int64_t
     return bwd_plan_set_values(plan->plan, vals, nnz, (hipStream_t)stream);
 }
 
+int trspmm_bwd_plan_stats(const trspmm_bwd_plan* plan, int64_t* counts5) {
+    TRSPMM_NOTNULL(plan, "plan");
+    TRSPMM_NOTNULL(counts5, "counts");
+    const BwdPlan& P = plan->plan;
+    counts5[0] = P.routed ? 1 : 0;
+    counts5[1] = P.routed ? P.at.dev.n_tsub : 0;
+    counts5[2] = P.routed ? P.at.dev.n_tval : 0;
+    counts5[3] = P.routed ? P.at.dev.n_rval : 0;
+    counts5[4] = (P.routed && P.at.dev.row_map) ? 1 : 0;
+    return TRSPMM_OK;
+}
+
 int trspmm_spmm_t_sddmm(const trspmm_bwd_plan* plan, const void* dY, int64_t lddy, const void* B, int64_t ldb,
                         int64_t K, float* dB, int64_t lddb, float* dvals, void* stream) {
     TRSPMM_NOTNULL(plan, "plan");

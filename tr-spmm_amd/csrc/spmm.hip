@@ -737,7 +737,7 @@ __device__ inline void run_deep(const PlanDev& p, __amdgpu_buffer_rsrc_t rsB, ui
                                 int32_t first, int32_t end, int n, uint8_t* __restrict__ Bs, uint8_t* __restrict__ Ms,
                                 int lane, f32x4 (&acc)[H][PW / 16], const SD& sd = SD{}) {
     constexpr bool FUSED = !std::is_same<SD, NoSdd>::value;
-    static_assert(!FUSED || KIND == 2, "the fused SDDMM covers residual records");
+    static_assert(!FUSED || KIND == 0 || KIND == 2, "the fused SDDMM covers TC sub-blocks and residual records");
     static_assert(H == 1 || (FUSED && H == 2), "two halves per step: the fused SDDMM only");
     using X8 = typename DT::x8;
     using X16 = typename DT::x16;
@@ -761,8 +761,8 @@ __device__ inline void run_deep(const PlanDev& p, __amdgpu_buffer_rsrc_t rsB, ui
     // fused SDDMM: the B operand of block h, slot sdd_slot(h, r), features 32 ks + 8 kg ..
     const uint8_t* sdr0 = Bs + deep_phys(sdd_slot(0, r)) * RS + 16 * kg;
     const uint8_t* sdr1 = Bs + deep_phys(sdd_slot(1, r)) * RS + 16 * kg;
-    const float* part = nullptr;
-    if constexpr (FUSED) part = sd.rmw ? sd.D : nullptr;
+    const float* part = nullptr;                           // residual records: the partials of earlier passes
+    if constexpr (FUSED && KIND == 2) part = sd.rmw ? sd.D : nullptr;
 
     DeepRec m[MR];
     DeepFrag fr[DEP];
@@ -790,6 +790,21 @@ __device__ inline void run_deep(const PlanDev& p, __amdgpu_buffer_rsrc_t rsB, ui
         constexpr int NHALF = (PAR + DEP) % H;             // half of virtual step v + DEP
         const int s = v / H;                               // the records' step
         __builtin_amdgcn_sched_barrier(0);
+        // fused SDDMM over TC sub-blocks: lane (kg, r) writes the value gradients of
+        // sub-block r / 4 of the step (its columns sdd_slot(h, r) % 8, rows 4 kg ..
+        // 4 kg + 3), which needs the sub-block's occupancy bits and value offset.
+        // Loaded here, before this step's record loads and next gathers, so that
+        // waiting for them does not wait for loads that are meant to stay in flight.
+        [[maybe_unused]] uint64_t tcb_lo = 0, tcb_hi = 0;
+        [[maybe_unused]] int32_t tcb_off = 0;
+        [[maybe_unused]] const int32_t tcb_sb = first + 4 * s + (r >> 2);
+        if constexpr (FUSED && KIND == 0 && HALF == H - 1) {
+            const int32_t sbc = tcb_sb < end ? tcb_sb : end - 1;
+            tcb_lo = p.tc_bits[2 * sbc];
+            tcb_hi = p.tc_bits[2 * sbc + 1];
+            tcb_off = p.tc_valoff[sbc];
+            asm volatile("" ::: "memory");
+        }
         deep_park<PW, R>(g[GB], Bs, lane);
         deep_stage<KIND>(m[NXT], first, end, n, (v + DEP) / H, Ms + MS * NXT, lane, part != nullptr);
         deep_rec<KIND>(p, first, end, n, (v + 2 * DEP) / H, lane, m[(PAR + 2 * DEP) % MR], part);
@@ -845,7 +860,30 @@ __device__ inline void run_deep(const PlanDev& p, __amdgpu_buffer_rsrc_t rsB, ui
                     sacc[h] = mma_dense<DT>(__builtin_bit_cast(X8, sd.xa[HALF * (PW / 32) + ks]),
                                             *(const X8*)(sdr + 64 * ks), sacc[h]);
             }
-            if constexpr (HALF == H - 1) {
+            if constexpr (HALF == H - 1 && KIND == 0) {
+                // S[row][slot] of a set bit is the gradient of the sub-block's value
+                // of that rank (values in bit order: rows 0-7 in the low word first)
+                if (tcb_sb < end) {
+                    const uint64_t bits = kg < 2 ? tcb_lo : tcb_hi;
+                    const int32_t base = tcb_off + (kg < 2 ? 0 : __popcll(tcb_lo));
+                    float* Dt = sd.D + p.n_rval;
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const int q = sdd_slot(h, r) & 7;
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const int b = ((4 * kg + i) & 7) * 8 + q;
+                            if ((bits >> b) & 1ull) {
+                                const int32_t idx = base + __popcll(bits & ((1ull << b) - 1ull));
+                                float v = sacc[h][i];
+                                if (sd.rmw) v += Dt[idx];
+                                Dt[idx] = v;
+                            }
+                        }
+                    }
+                }
+            }
+            if constexpr (HALF == H - 1 && KIND == 2) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const f32x4 sa = sacc[h];
@@ -863,7 +901,8 @@ __device__ inline void run_deep(const PlanDev& p, __amdgpu_buffer_rsrc_t rsB, ui
             if ((lane < 32) & (rec < end)) sd.D[rec] = v;
             }
         }
-        if constexpr (KIND != 2) deep_frag<KIND>(p, first, end, n, s + DEP, lane, fr[GB]);
+        // the next step's fragment, once every half of this step has applied it
+        if constexpr (KIND != 2 && (PAR % H) == H - 1) deep_frag<KIND>(p, first, end, n, s + DEP, lane, fr[GB]);
     };
     for (int v = 0; v < nv; v += U) {
         body(v, std::integral_constant<int, 0>{});
@@ -946,20 +985,24 @@ __device__ __forceinline__ void deep_item(const PlanDev& p, int64_t it, const ui
         int32_t bs0 = s0, bs1 = s1, bt0 = t0, bt1 = t1, be0 = e0, be1 = e1;
         asm volatile("" : "+s"(bs0), "+s"(bs1), "+s"(bt0), "+s"(bt1), "+s"(be0), "+s"(be1));
         if constexpr (FUSED) {
-            // the plan holds residual records only (checked by the launcher)
+            // the plan holds TC sub-blocks and residual records only (checked by the launcher)
             Sdd<PW, H> sd;
             sd.D = sa.D;
             sd.rmw = fp > 0;
             const __amdgpu_buffer_rsrc_t rsX = __builtin_amdgcn_make_buffer_rsrc(
                 (void*)sa.X, (short)0, (int)(uint32_t)(2 * sa.x_rows * sa.ldx), (int)kBufferWord3);
-            const bool rok = row0 + r < sa.x_rows;
+            // X row of tile row r: the plan row, or its output row under a row map
+            int64_t xrow = row0 + r;
+            if constexpr (RM) xrow = __shfl(rmap, r);
+            const bool rok = (xrow >= 0) & (xrow < sa.x_rows);
 #pragma unroll
             for (int ks = 0; ks < H * PW / 32; ++ks) {
                 const int f = fp + 32 * ks + 8 * kg;
-                const uint32_t off = (rok & (f < K)) ? (uint32_t)(row0 + r) * (uint32_t)(2 * sa.ldx) + 2u * (uint32_t)f
+                const uint32_t off = (rok & (f < K)) ? (uint32_t)xrow * (uint32_t)(2 * sa.ldx) + 2u * (uint32_t)f
                                                      : kOutOfRange;
                 sd.xa[ks] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rsX, off, 0, 0));
             }
+            if (nts > 0) run_deep<0, PW, DT, H>(p, rsB, ldb2, K, fp, bs0, bs1, nts, Bs, Ms, lane, acc, sd);
             if (nte > 0) run_deep<2, PW, DT, H>(p, rsB, ldb2, K, fp, be0, be1, nte, Bs, Ms, lane, acc, sd);
         } else {
             if (nts > 0) run_deep<0, PW, DT>(p, rsB, ldb2, K, fp, bs0, bs1, nts, Bs, Ms, lane, acc);
@@ -1330,15 +1373,22 @@ int launch_fused_typed(const PlanDev& p, const void* dY, int64_t lddy, int64_t K
     const int64_t grid = xcd_grid(p.n_titems);
     TRSPMM_CHECK(grid <= 0x7fffffff, "too many work items");
     // K <= 256 is one pass: 128-feature halves, each step's rows gathered per half
-    if (K <= 64)
-        hipLaunchKernelGGL((k_routes<64, DT, 1>), dim3((unsigned)grid), dim3(64), 0, s, p, (const uint16_t*)dY, lddy,
-                           (int)K, dB, lddb, (uint32_t)TRSPMM_ROUTE_ALL, grid, sa);
+    const uint16_t* Y = (const uint16_t*)dY;
+    const uint32_t all = (uint32_t)TRSPMM_ROUTE_ALL;
+    const dim3 gr((unsigned)grid);
+    if (p.row_map) {
+        if (K <= 64)
+            hipLaunchKernelGGL((k_routes<64, DT, 1, true>), gr, dim3(64), 0, s, p, Y, lddy, (int)K, dB, lddb, all, grid, sa);
+        else if (K <= 128)
+            hipLaunchKernelGGL((k_routes<128, DT, 1, true>), gr, dim3(64), 0, s, p, Y, lddy, (int)K, dB, lddb, all, grid, sa);
+        else
+            hipLaunchKernelGGL((k_routes<128, DT, 2, true>), gr, dim3(64), 0, s, p, Y, lddy, (int)K, dB, lddb, all, grid, sa);
+    } else if (K <= 64)
+        hipLaunchKernelGGL((k_routes<64, DT, 1>), gr, dim3(64), 0, s, p, Y, lddy, (int)K, dB, lddb, all, grid, sa);
     else if (K <= 128)
-        hipLaunchKernelGGL((k_routes<128, DT, 1>), dim3((unsigned)grid), dim3(64), 0, s, p, (const uint16_t*)dY,
-                           lddy, (int)K, dB, lddb, (uint32_t)TRSPMM_ROUTE_ALL, grid, sa);
+        hipLaunchKernelGGL((k_routes<128, DT, 1>), gr, dim3(64), 0, s, p, Y, lddy, (int)K, dB, lddb, all, grid, sa);
     else
-        hipLaunchKernelGGL((k_routes<128, DT, 2>), dim3((unsigned)grid), dim3(64), 0, s, p, (const uint16_t*)dY,
-                           lddy, (int)K, dB, lddb, (uint32_t)TRSPMM_ROUTE_ALL, grid, sa);
+        hipLaunchKernelGGL((k_routes<128, DT, 2>), gr, dim3(64), 0, s, p, Y, lddy, (int)K, dB, lddb, all, grid, sa);
     TRSPMM_HIP(hipGetLastError());
     return TRSPMM_OK;
 }
@@ -1346,13 +1396,14 @@ int launch_fused_typed(const PlanDev& p, const void* dY, int64_t lddy, int64_t K
 }  // namespace
 
 int fused_sddmm_supported(const PlanDev& p) {
-    return p.n_tsub == 0 && p.n_tiles == 0 && p.res_mfma && p.tile_kernel == 0 && p.item_order != 2;
+    return p.n_tiles == 0 && p.res_mfma && p.tile_kernel == 0 && p.item_order != 2;
 }
 
 int launch_spmm_sddmm(const PlanDev& p, const void* dY, int64_t lddy, int64_t K, float* dB, int64_t lddb,
                       const void* X, int64_t ldx, int64_t x_rows, float* D, hipStream_t s) {
-    TRSPMM_CHECK(fused_sddmm_supported(p), "fused SDDMM: the plan must hold residual records only, on the matrix-core "
-                                           "engine, with the default tile kernel and item order");
+    TRSPMM_CHECK(fused_sddmm_supported(p), "fused SDDMM: the plan must hold TC sub-blocks and residual records only "
+                                           "(no 2:4 tiles), on the matrix-core engine, with the default tile kernel "
+                                           "and item order");
     TRSPMM_CHECK(K > 0 && K % 8 == 0 && K <= (int64_t)1 << 24, "K must be a positive multiple of 8");
     TRSPMM_CHECK(lddy >= K && lddy % 8 == 0 && ldx >= K && ldx % 8 == 0, "dY and X strides must be >= K, multiples of 8");
     TRSPMM_CHECK(lddb >= K && lddb % 4 == 0, "dB stride must be >= K and a multiple of 4");
@@ -1360,7 +1411,7 @@ int launch_spmm_sddmm(const PlanDev& p, const void* dY, int64_t lddy, int64_t K,
                  "dY, dB and X must be 16-byte aligned");
     TRSPMM_CHECK(p.ncols * lddy < ((int64_t)1 << 30) && x_rows * ldx < ((int64_t)1 << 30),
                  "fused SDDMM: dY and X must hold fewer than 2^30 elements");
-    TRSPMM_CHECK(p.n_rval == 0 || D != nullptr, "fused SDDMM: D must not be null");
+    TRSPMM_CHECK(p.n_rval + p.n_tval == 0 || D != nullptr, "fused SDDMM: D must not be null");
     if (p.windows == 0) return TRSPMM_OK;
     const SddArgs sa{(const uint16_t*)X, ldx, x_rows, D};
     if (p.dtype == TRSPMM_BF16) return launch_fused_typed<BF16>(p, dY, lddy, K, dB, lddb, sa, s);

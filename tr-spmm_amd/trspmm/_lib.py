@@ -29,7 +29,7 @@ EXPORTS = [
     "trspmm_spmm_t_sddmm", "trspmm_plan_create_ex", "trspmm_plan_from_meta_ex", "trspmm_match_2to4",
     "trspmm_match_2to4_hash", "trspmm_match_2to4_bucket", "trspmm_match_2to4_online", "trspmm_match_2to4_online_noalloc",
     "trspmm_minhash", "trspmm_minhash_coefficients", "trspmm_tca_order", "trspmm_plan_set_values",
-    "trspmm_bwd_plan_set_values", "trspmm_cache_trim", "trspmm_meta_row_order",
+    "trspmm_bwd_plan_set_values", "trspmm_cache_trim", "trspmm_meta_row_order", "trspmm_bwd_plan_stats",
 ]
 
 ENGINE_DEFAULT, ENGINE_VALU, ENGINE_MFMA = 0, 1, 2
@@ -135,6 +135,7 @@ def lib():
     L.trspmm_bwd_plan_create.argtypes = [_P, _P, _P, _P, _I64, _I64, _I64, ctypes.c_int, _P, ctypes.POINTER(_P)]
     L.trspmm_bwd_plan_free.argtypes = [_P]
     L.trspmm_bwd_plan_set_values.argtypes = [_P, _P, _I64, _P]
+    L.trspmm_bwd_plan_stats.argtypes = [_P, ctypes.POINTER(_I64)]
     L.trspmm_plan_set_values.argtypes = [_P, _P, _I64, _P, _I64, _P]
     L.trspmm_spmm_t_sddmm.argtypes = [_P, _P, _I64, _P, _I64, _I64, _P, _I64, _P, _P]
     L.trspmm_match_2to4.argtypes = [_P, _P, _I64, _I64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,

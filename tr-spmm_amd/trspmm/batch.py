@@ -12,6 +12,11 @@ call and one plan cover the whole batch; the plan's work-item list is the
 concatenated tile queue of every graph, so one launch (at most three
 kernels on one stream) runs all of them and the grid stays full however
 small the individual graphs are.  Rows added for padding are empty.
+
+The rows keep this order by default (reorder=False), so each graph's rows sit
+in windows of their own and its output equals its own routed product bit for
+bit; reorder=True lets the router reorder the concatenated rows (see
+RoutedSparse), which can mix graphs in a window.
 """
 from __future__ import annotations
 
@@ -69,13 +74,13 @@ def concat_block_diagonal(graphs):
 
 class RoutedBatch:
     def __init__(self, graphs, *, mode: str = "exact", dtype: torch.dtype = torch.float16, engine=None, slab=None,
-                 plan_opts: dict | None = None, **router_kwargs):
+                 plan_opts: dict | None = None, reorder: bool = False, **router_kwargs):
         rowptr, col, val, self.row_off, self.col_off, self.m_list, self.n_list = concat_block_diagonal(graphs)
         self.count = len(self.m_list)
         self.dtype = dtype
         self.A = RoutedSparse(torch.from_numpy(rowptr), torch.from_numpy(col), torch.from_numpy(val),
                               int(self.col_off[-1]), int(self.row_off[-1]), mode=mode, dtype=dtype, engine=engine,
-                              slab=slab, plan_opts=plan_opts, **router_kwargs)
+                              slab=slab, plan_opts=plan_opts, reorder=reorder, **router_kwargs)
 
     @property
     def rows(self) -> int:

@@ -178,6 +178,16 @@ class RoutedSparse:
             self._bwd = _BwdPlan(h)
         return self._bwd
 
+    @property
+    def backward_stats(self) -> dict:
+        """What the fused backward runs on (builds its plan on first use): the routed
+        fused kernel or the lane-group kernels, A^T's TC sub-blocks / TC nonzeros /
+        residual records, and whether A^T's rows were reordered."""
+        c = (ctypes.c_int64 * 5)()
+        check(lib().trspmm_bwd_plan_stats(self._backward_plan().handle, c), "trspmm_bwd_plan_stats")
+        return {"routed": bool(c[0]), "tc_subblocks": int(c[1]), "tc_nnz": int(c[2]), "residual_nnz": int(c[3]),
+                "reordered": bool(c[4])}
+
     def spmm_t_sddmm(self, dY: torch.Tensor, B: torch.Tensor, dB: torch.Tensor | None = None,
                      dvals: torch.Tensor | None = None):
         """Fused backward: (dB = A^T · dY  [ncols, K] fp32, dvals = SDDMM(dY, B)  [nnz]

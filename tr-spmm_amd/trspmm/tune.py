@@ -5,7 +5,10 @@ The reference fixes its choices at build time (TC width 8, density 8, one
 2:4 kernel).  Here the candidates are
   router    mode exact / tuned with a 2:4 occupancy floor (sptc_min_nnz),
             TC density threshold (columns with >= density rows of a window
-            become dense 16x8 sub-blocks)
+            become dense 16x8 sub-blocks), row reordering (kept by the router
+            only when it gathers fewer B rows; its plans run only on the deep
+            tile kernel with the matrix-core residual engine, and the other
+            kernel candidates are skipped for it)
   kernels   residual engine (vector ALU / matrix cores), tile-kernel
             features per pass (64 / 128), work-item cap (a window with more
             2:4 tiles / TC sub-blocks is split into chunks of this many), work
@@ -24,6 +27,7 @@ from dataclasses import dataclass, field
 import torch
 
 from .core import Plan, route
+from .sparse import reorder_supported
 
 ROUTER_CANDIDATES = [
     {"mode": "exact", "density": 8},
@@ -31,6 +35,7 @@ ROUTER_CANDIDATES = [
     {"mode": "tuned", "density": 8, "sptc_min_nnz": 8},
     {"mode": "tuned", "density": 8, "sptc_min_nnz": 16},
     {"mode": "exact", "density": 6},
+    {"mode": "tuned", "density": 8, "sptc_min_nnz": 8, "reorder": True},
 ]
 KERNEL_CANDIDATES = [{"engine": e, "slab": s, "tile_cap": c, "item_order": o}
                      for e, s, c, o in itertools.product(["valu", "mfma"], [64, 128], [32, 64, 128], [0, 2, 3])]
@@ -64,6 +69,8 @@ def autotune(rowptr, col, val, ncols: int, m: int, B: torch.Tensor, *, dtype=tor
         meta = route(rowptr, col, val, ncols, mode=mode, **kw)
         for kc in kernel_candidates:
             cap = kc.get("tile_cap", 0)
+            if meta.reordered and not reorder_supported(mode, meta.window, kc["engine"], kc):
+                continue
             plan = Plan.from_meta(meta, m, ncols, dtype, engine=kc["engine"], slab=kc["slab"], tile_cap=cap, tc_cap=cap,
                                   item_order=kc.get("item_order", 0), k_hint=B.shape[1])
             ms = plan.timed(B, C, warmup=warmup, iters=iters)
