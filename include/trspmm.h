@@ -101,9 +101,9 @@ enum { TRSPMM_ROUTE_EDGES = 1, TRSPMM_ROUTE_REORDER = 2 };
  * counterpart, in the spirit of the reference caller's row reordering,
  * spmm/tr_csr_fp16/tr_tca_reorder.py): the router also routes the rows in
  * order of (first column, nonzeros descending, row index), empty rows last,
- * and keeps that routing when its routes gather fewer B rows per multiply
- * (TC sub-block columns + 2:4 tile columns + residual entries; the input
- * order on a tie).  The arrays then describe the reordered rows,
+ * and keeps that routing when its routes gather at least a tenth fewer B
+ * rows per multiply (TC sub-block columns + 2:4 tile columns + residual
+ * entries).  The arrays then describe the reordered rows,
  * trspmm_meta_row_order gives the order, edge maps still name the input's
  * edges, and plans built from the result write every row to its input
  * index. */

@@ -219,7 +219,8 @@ def f32_to_f16_bits(x):
 # No reference counterpart: the device router's TRSPMM_ROUTE_REORDER option
 # (tr-spmm_amd/csrc/router.hip, route_device).  Restated here from its
 # definition: rows by (first column, nonzeros descending, row index), empty
-# rows last; the reordered routing is kept when it gathers fewer B rows.
+# rows last; the reordered routing is kept when it gathers at least a tenth
+# fewer B rows.
 
 
 def reorder_rows(rowptr, col):
@@ -261,6 +262,6 @@ def route_reordered(rowptr, col, val, **kw):
     order = reorder_rows(rowptr, col)
     prow, pcol, pval, _ = permute_rows(rowptr, col, val, order)
     other = route(prow, pcol, pval, **kw)
-    if gathered_rows(other) < gathered_rows(natural):
+    if 10 * gathered_rows(other) <= 9 * gathered_rows(natural):
         return other, order
     return natural, None

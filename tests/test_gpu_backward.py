@@ -281,7 +281,7 @@ def test_transpose_is_cached_and_routed(cuda):
     assert st["tc_subblocks"] + st["sptc_tiles"] + st["residual_nnz"] > 0
 
 
-@pytest.mark.parametrize("case", ["rmat10", "dense", "star", "er512"])
+@pytest.mark.parametrize("case", ["rmat10", "dense", "star", "er512", "ragged"])
 @pytest.mark.parametrize("K", [64, 256, 512])
 @pytest.mark.parametrize("dtype", [torch.float16, torch.bfloat16])
 def test_fused_backward_tc_subblocks_and_reordered_rows(cuda, case, K, dtype):
@@ -297,7 +297,7 @@ def test_fused_backward_tc_subblocks_and_reordered_rows(cuda, case, K, dtype):
     print(f"[fused-tc] {case} K={K} {dtype} {st}", flush=True)
     assert st["routed"] and st["tc_subblocks"] > 0
     assert st["tc_nnz"] + st["residual_nnz"] == csr.nnz
-    if case in ("rmat10", "er512"):
+    if case in ("rmat10", "ragged"):           # a tenth or more fewer gathered dY rows (oracle counts)
         assert st["reordered"]
     gen = torch.Generator().manual_seed(K + 7)
     B = torch.randn(csr.n, K, generator=gen).to(dtype)

@@ -12,7 +12,8 @@ route_reordered).  Pinned here by:
     TC sub-block (8 gathered rows), a lone nonzero is one residual entry, four
     columns under the TC density shared by rows two at a time form one 2:4
     tile (its 4 non-empty columns);
-  * the keep-or-not decision against the two counts it compares.
+  * the keep-or-not decision against the two counts it compares (kept at
+    a tenth fewer gathered rows or more).
 """
 import numpy as np
 import pytest
@@ -101,7 +102,7 @@ def test_route_reordered_keeps_the_cheaper_routing(case):
     prow, pcol, pval, _ = oracle.permute_rows(csr.rowptr, csr.col, csr.val, o)
     other = oracle.route(prow, pcol, pval, **kw)
     g0, g1 = oracle.gathered_rows(natural), oracle.gathered_rows(other)
-    if g1 < g0:
+    if 10 * g1 <= 9 * g0:
         assert order is not None and np.array_equal(order, o)
         assert oracle.gathered_rows(kept) == g1
     else:

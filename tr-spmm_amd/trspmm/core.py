@@ -141,7 +141,8 @@ def route(rowptr: torch.Tensor, col: torch.Tensor, val: torch.Tensor, ncols: int
     value slot, so that plans built from the result accept new values
     (Plan.set_values) without routing again.  reorder (exact / tuned, window 16):
     also route the rows in order of (first column, nonzeros descending, row)
-    and keep that routing when it gathers fewer B rows per multiply; plans built
+    and keep that routing when it gathers at least a tenth fewer B rows per
+    multiply; plans built
     from it write every row to its input index (RoutedMeta.row_order)."""
     _require_cuda()
     es, eu = env_router_knobs()

@@ -16,7 +16,7 @@ in place, without routing again.
 With `reorder` (the default for tuned routing where the plan's kernels
 support it: window 16, the deep tile kernel with the matrix-core residual
 engine) the router also tries the rows in (first column, degree) order and
-keeps that routing when it gathers fewer B rows; the plan then writes each
+keeps that routing when it gathers at least a tenth fewer B rows; the plan then writes each
 row to its own index, so results are unchanged.  Exact routing keeps the
 reference's tiles unless reorder=True is passed.
 """
