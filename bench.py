@@ -132,15 +132,16 @@ def cpu_baseline(csr, K, B16, budget_s=20.0):
                       f"{nnz / dt / 1e6:.3f} M nnz/s"}
 
 
-def traffic_record(workload, mode, dtype):
+def traffic_record(workload, mode, dtype, reordered):
     """Measured HBM bytes per routed multiply (rocprofv3 FETCH_SIZE + WRITE_SIZE,
-    scaled by a plain copy's readings), recorded for this workload, routing mode
-    and operand type; None when no such measurement is recorded."""
+    scaled by a plain copy's readings), recorded for this workload, routing mode,
+    row order and operand type; None when no such measurement is recorded."""
     path = os.path.join(ROOT, "profiles", f"spmm_{workload}_traffic.json")
     if os.path.exists(path):
         with open(path) as f:
             rec = json.load(f)
-        if rec.get("mode") == mode and rec.get("dtype", "fp16") == dtype:
+        if (rec.get("mode") == mode and rec.get("dtype", "fp16") == dtype
+                and bool(rec.get("rows_reordered", False)) == bool(reordered)):
             return rec.get("traffic_bytes")
     return None
 
@@ -629,7 +630,8 @@ def main():
                    "ms_cold": round(ms_cold, 5), "wall_s_timed": round(wall, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_record(a.workload, a.mode, a.dtype), "algorithmic_bytes": int(byts),
+                     "traffic": traffic_record(a.workload, a.mode, a.dtype, A.meta.reordered),
+                     "algorithmic_bytes": int(byts),
                      "kernel": "k_routes", "copy_gbs": round(copy_gbs, 1)},
     }
     if not a.no_cpu_baseline:

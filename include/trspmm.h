@@ -395,6 +395,97 @@ int trspmm_tca_order(const int32_t* rowptr, const int32_t* col, int64_t rows, in
                      int num_perm, int thres, int cluster_thres, double lsh_threshold, uint64_t seed,
                      int32_t* order);
 
+/* ------------------------------------------------------------------ legacy Libra kernel family
+ * Replace the host functions of TR-source/SpMM/mGCNkernel.cu that multiply
+ * with the original Libra metadata in fp16 (same parameter lists; device
+ * pointers; half -> uint16_t, long -> int64_t):
+ *   trspmm_libra_fp16_tcu_cuda           <- spmm_forward_fp16_tcu_cuda_kernel          (:1357-1566)
+ *   trspmm_libra_fp16_tcu_cuda_seq       <- spmm_forward_fp16_tcu_cuda_kernel_seq      (:1778-1935)
+ *   trspmm_libra_fp16_tcu_cuda_hybrid    <- spmm_forward_fp16_tcu_cuda_hybrid_kernel   (:1936-2100)
+ *   trspmm_libra_fp16_tcu_cuda_hybrid_seq<- spmm_forward_fp16_tcu_cuda_hybrid_kernel_seq (:2181-2338)
+ *   trspmm_libra_fp16_tcu_cuda_cc_only   <- spmm_forward_fp16_tcu_cuda_kernel_cc_only  (:1567-1777)
+ *   trspmm_libra_fp16_tcu_nomap          <- spmm_forward_fp16_tcu_cuda_nomap_kernel    (:2101-2180)
+ *   trspmm_libra_fp16_cuda_split         <- spmm_forward_fp16_cuda_kernel_split        (:2651-2766)
+ *   trspmm_libra_fp16_cuda_split_v1      <- spmm_forward_fp16_cuda_kernel_split_v1     (:2767-2884)
+ * Format: TC blocks of 8-row windows (t_window_row[p] = window index, output
+ * rows 8 t_window_row[p] + n; block b = 8 columns t_column[8 b + k] and mask
+ * t_binary[b], bit 8 n + k = row n, column k; values at t_blockNew_offset[b]
+ * in bit order; parts in window order) and residual parts as in the online
+ * format.  Each call builds a plan, runs 10 untimed and `epoches` timed
+ * launches on the null stream, leaves the product (fp32, FP32 accumulation,
+ * rows [0, mOri), row stride dimN) in output_matrix and returns the mean ms
+ * per launch, or -1 with a message for trspmm_last_error().  The seq / hybrid
+ * variants, which differ in the reference only in how they schedule the same
+ * kernels, compute the same product in one launch; cc_only runs every
+ * nonzero on the vector ALU (no matrix instructions); nomap is the TC route
+ * alone, split / split_v1 the residual lists alone.  t_atomic / c_atomic,
+ * partsize and swizzle are accepted and not needed (every output row has one
+ * owner here).  B must hold (largest column + 1) rows of dimN values. */
+float trspmm_libra_fp16_tcu_cuda(const int32_t* t_row_offset, const int32_t* t_blockNew_offset,
+                                 const uint16_t* t_value, const int32_t* t_column, const int32_t* t_window_row,
+                                 const int32_t* t_atomic, const int64_t* t_binary, const int32_t* c_row_offset,
+                                 const int32_t* c_row, const int32_t* c_atomic, const int32_t* c_column,
+                                 const uint16_t* c_value, const int32_t* c_row_offset_short,
+                                 const int32_t* c_row_short, const int32_t* c_atomic_short,
+                                 const int32_t* c_column_short, const uint16_t* c_value_short,
+                                 const uint16_t* rhs_matrix, float* output_matrix, int parts_t, int parts_c,
+                                 int partsize_c, int parts_c_short, int dimN, int mOri, int epoches);
+float trspmm_libra_fp16_tcu_cuda_seq(const int32_t* t_row_offset, const int32_t* t_blockNew_offset,
+                                     const uint16_t* t_value, const int32_t* t_column, const int32_t* t_window_row,
+                                     const int32_t* t_atomic, const int64_t* t_binary, const int32_t* c_row_offset,
+                                     const int32_t* c_row, const int32_t* c_atomic, const int32_t* c_column,
+                                     const uint16_t* c_value, const int32_t* c_row_offset_short,
+                                     const int32_t* c_row_short, const int32_t* c_atomic_short,
+                                     const int32_t* c_column_short, const uint16_t* c_value_short,
+                                     const uint16_t* rhs_matrix, float* output_matrix, int parts_t, int parts_c,
+                                     int partsize_c, int parts_c_short, int dimN, int mOri, int epoches);
+float trspmm_libra_fp16_tcu_cuda_hybrid(const int32_t* t_row_offset, const int32_t* t_blockNew_offset,
+                                        const uint16_t* t_value, const int32_t* t_column, const int32_t* t_window_row,
+                                        const int32_t* t_atomic, const int64_t* t_binary, const int32_t* c_row_offset,
+                                        const int32_t* c_row, const int32_t* c_atomic, const int32_t* c_column,
+                                        const uint16_t* c_value, const int32_t* c_row_offset_short,
+                                        const int32_t* c_row_short, const int32_t* c_atomic_short,
+                                        const int32_t* c_column_short, const uint16_t* c_value_short,
+                                        const uint16_t* rhs_matrix, float* output_matrix, int parts_t, int parts_c,
+                                        int partsize_c, int parts_c_short, int dimN, int mOri, int epoches);
+float trspmm_libra_fp16_tcu_cuda_hybrid_seq(const int32_t* t_row_offset, const int32_t* t_blockNew_offset,
+                                            const uint16_t* t_value, const int32_t* t_column,
+                                            const int32_t* t_window_row, const int32_t* t_atomic,
+                                            const int64_t* t_binary, const int32_t* c_row_offset, const int32_t* c_row,
+                                            const int32_t* c_atomic, const int32_t* c_column, const uint16_t* c_value,
+                                            const int32_t* c_row_offset_short, const int32_t* c_row_short,
+                                            const int32_t* c_atomic_short, const int32_t* c_column_short,
+                                            const uint16_t* c_value_short, const uint16_t* rhs_matrix,
+                                            float* output_matrix, int parts_t, int parts_c, int partsize_c,
+                                            int parts_c_short, int dimN, int mOri, int epoches);
+float trspmm_libra_fp16_tcu_cuda_cc_only(const int32_t* t_row_offset, const int32_t* t_blockNew_offset,
+                                         const uint16_t* t_value, const int32_t* t_column,
+                                         const int32_t* t_window_row, const int32_t* t_atomic,
+                                         const int64_t* t_binary, const int32_t* c_row_offset, const int32_t* c_row,
+                                         const int32_t* c_atomic, const int32_t* c_column, const uint16_t* c_value,
+                                         const int32_t* c_row_offset_short, const int32_t* c_row_short,
+                                         const int32_t* c_atomic_short, const int32_t* c_column_short,
+                                         const uint16_t* c_value_short, const uint16_t* rhs_matrix,
+                                         float* output_matrix, int parts_t, int parts_c, int partsize_c,
+                                         int parts_c_short, int dimN, int mOri, int epoches);
+float trspmm_libra_fp16_tcu_nomap(const int32_t* t_row_offset, const int32_t* t_blockNew_offset,
+                                  const uint16_t* t_value, const int32_t* t_column, const int32_t* t_window_row,
+                                  const int32_t* t_atomic, const int64_t* t_binary, const uint16_t* rhs_matrix,
+                                  float* output_matrix, int parts_t, int dimN, int mOri, int epoches);
+float trspmm_libra_fp16_cuda_split(const int32_t* c_row_offset, const int32_t* c_row, const int32_t* c_atomic,
+                                   const int32_t* c_column, const uint16_t* c_value, const int32_t* c_row_offset_short,
+                                   const int32_t* c_row_short, const int32_t* c_atomic_short,
+                                   const int32_t* c_column_short, const uint16_t* c_value_short,
+                                   const uint16_t* rhs_matrix, float* output_matrix, int partsize, int dimN, int mOri,
+                                   int epoches, int parts, int parts_short, int swizzle);
+float trspmm_libra_fp16_cuda_split_v1(const int32_t* c_row_offset, const int32_t* c_row, const int32_t* c_atomic,
+                                      const int32_t* c_column, const uint16_t* c_value,
+                                      const int32_t* c_row_offset_short, const int32_t* c_row_short,
+                                      const int32_t* c_atomic_short, const int32_t* c_column_short,
+                                      const uint16_t* c_value_short, const uint16_t* rhs_matrix, float* output_matrix,
+                                      int partsize, int dimN, int mOri, int epoches, int parts, int parts_short,
+                                      int swizzle);
+
 /* ------------------------------------------------------------------ diagnostics */
 
 /* libstdc++ std::sort order of iota(n) by key descending, computed by the

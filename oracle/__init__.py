@@ -265,3 +265,42 @@ def route_reordered(rowptr, col, val, **kw):
     if 10 * gathered_rows(other) <= 9 * gathered_rows(natural):
         return other, order
     return natural, None
+
+
+# ---------------------------------------------------------------- legacy Libra format
+def decode_libra(t_row_offset, t_blockNew_offset, t_value, t_column, t_window_row, t_binary,
+                 c_row_offset, c_row, c_column, c_value, cs_row_offset, cs_row, cs_column, cs_value):
+    """COO triplets the legacy Libra metadata encodes, as the reference's
+    kernels read it (TR-source/SpMM/mGCNkernel.cu):
+      * TC (spmm_forward_fp16_csr_v2_kernel_tcu, :60-150): part p, blocks
+        t_row_offset[p] .. [p+1]; block b's mask bit 8 n + k (lane l holds bits
+        2 l, 2 l + 1, the m16n8k8 B-fragment rows k = 2 (l % 4) + j of column
+        n = l / 4) is row 8 t_window_row[p] + n (:132), column t_column[8 b + k]
+        (:62, :113-116), value t_value[t_blockNew_offset[b] + popcount of the
+        lower bits] (:71-95);
+      * residual (spmm_forward_fp16_csr_v2_kernel_cuda{,_short}, :977-1043,
+        :1120-1171): part p adds c_value[e] B[c_column[e]] to row c_row[p].
+    Empty TC columns (-1) carry no nonzero."""
+    rows, cols, vals = [], [], []
+    t_row_offset = np.asarray(t_row_offset)
+    for p in range(len(t_window_row)):
+        base = 8 * int(t_window_row[p])
+        for b in range(int(t_row_offset[p]), int(t_row_offset[p + 1])):
+            bits = int(np.asarray(t_binary)[b]) & ((1 << 64) - 1)
+            rank = 0
+            for bit in range(64):
+                if (bits >> bit) & 1:
+                    n, k = bit // 8, bit % 8
+                    c = int(t_column[8 * b + k])
+                    if c >= 0:
+                        rows.append(base + n)
+                        cols.append(c)
+                        vals.append(float(t_value[int(t_blockNew_offset[b]) + rank]))
+                    rank += 1
+    for off, rr, cc, vv in ((c_row_offset, c_row, c_column, c_value), (cs_row_offset, cs_row, cs_column, cs_value)):
+        for p in range(len(rr)):
+            for e in range(int(off[p]), int(off[p + 1])):
+                rows.append(int(rr[p]))
+                cols.append(int(cc[e]))
+                vals.append(float(vv[e]))
+    return np.asarray(rows, np.int64), np.asarray(cols, np.int64), np.asarray(vals, np.float64)

@@ -30,6 +30,9 @@ EXPORTS = [
     "trspmm_match_2to4_hash", "trspmm_match_2to4_bucket", "trspmm_match_2to4_online", "trspmm_match_2to4_online_noalloc",
     "trspmm_minhash", "trspmm_minhash_coefficients", "trspmm_tca_order", "trspmm_plan_set_values",
     "trspmm_bwd_plan_set_values", "trspmm_cache_trim", "trspmm_meta_row_order", "trspmm_bwd_plan_stats",
+    "trspmm_libra_fp16_tcu_cuda", "trspmm_libra_fp16_tcu_cuda_seq", "trspmm_libra_fp16_tcu_cuda_hybrid",
+    "trspmm_libra_fp16_tcu_cuda_hybrid_seq", "trspmm_libra_fp16_tcu_cuda_cc_only", "trspmm_libra_fp16_tcu_nomap",
+    "trspmm_libra_fp16_cuda_split", "trspmm_libra_fp16_cuda_split_v1",
 ]
 
 ENGINE_DEFAULT, ENGINE_VALU, ENGINE_MFMA = 0, 1, 2
