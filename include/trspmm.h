@@ -486,6 +486,41 @@ float trspmm_libra_fp16_cuda_split_v1(const int32_t* c_row_offset, const int32_t
                                       int partsize, int dimN, int mOri, int epoches, int parts, int parts_short,
                                       int swizzle);
 
+/* The tf32 members (fp32 values and operands; TC blocks of 8 rows x 4 columns,
+ * t_column[4 b + k], a 32-bit mask t_binary[b] whose bit 4 n + k is row n,
+ * column k, as spmm_forward_tf32_csr_v2_kernel_tcu reads it, :2886-3030):
+ *   trspmm_libra_tf32_tcu_cuda      <- spmm_forward_tf32_tcu_cuda_kernel      (:3513-3674)
+ *   trspmm_libra_tf32_tcu_cuda_seq  <- spmm_forward_tf32_tcu_cuda_kernel_seq  (:3675-3837)
+ *   trspmm_libra_tf32_cuda_split    <- spmm_forward_tf32_cuda_kernel_split    (:3388-3512)
+ * gfx950 has no tf32 matrix instruction: each multiply runs as three bf16
+ * products, A_hi B_hi + A_hi B_lo + A_lo B_hi (x_hi = bf16(x), x_lo =
+ * bf16(x - x_hi)), summed in fp32 -- finer than tf32's 10-bit mantissa.  The
+ * returned ms covers the three products and their sum. */
+float trspmm_libra_tf32_tcu_cuda(const int32_t* t_row_offset, const int32_t* t_blockNew_offset, const float* t_value,
+                                 const int32_t* t_column, const int32_t* t_window_row, const int32_t* t_atomic,
+                                 const int32_t* t_binary, const int32_t* c_row_offset, const int32_t* c_row,
+                                 const int32_t* c_atomic, const int32_t* c_column, const float* c_value,
+                                 const int32_t* c_row_offset_short, const int32_t* c_row_short,
+                                 const int32_t* c_atomic_short, const int32_t* c_column_short,
+                                 const float* c_value_short, const float* rhs_matrix, float* output_matrix,
+                                 int parts_t, int parts_c, int partsize_c, int parts_c_short, int dimN, int mOri,
+                                 int epoches);
+float trspmm_libra_tf32_tcu_cuda_seq(const int32_t* t_row_offset, const int32_t* t_blockNew_offset,
+                                     const float* t_value, const int32_t* t_column, const int32_t* t_window_row,
+                                     const int32_t* t_atomic, const int32_t* t_binary, const int32_t* c_row_offset,
+                                     const int32_t* c_row, const int32_t* c_atomic, const int32_t* c_column,
+                                     const float* c_value, const int32_t* c_row_offset_short,
+                                     const int32_t* c_row_short, const int32_t* c_atomic_short,
+                                     const int32_t* c_column_short, const float* c_value_short,
+                                     const float* rhs_matrix, float* output_matrix, int parts_t, int parts_c,
+                                     int partsize_c, int parts_c_short, int dimN, int mOri, int epoches);
+float trspmm_libra_tf32_cuda_split(const int32_t* c_row_offset, const int32_t* c_row, const int32_t* c_atomic,
+                                   const int32_t* c_column, const float* c_value, const int32_t* c_row_offset_short,
+                                   const int32_t* c_row_short, const int32_t* c_atomic_short,
+                                   const int32_t* c_column_short, const float* c_value_short, const float* rhs_matrix,
+                                   float* output_matrix, int partsize, int dimN, int mOri, int epoches, int parts,
+                                   int parts_short, int swizzle);
+
 /* ------------------------------------------------------------------ diagnostics */
 
 /* libstdc++ std::sort order of iota(n) by key descending, computed by the

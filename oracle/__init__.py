@@ -269,7 +269,7 @@ def route_reordered(rowptr, col, val, **kw):
 
 # ---------------------------------------------------------------- legacy Libra format
 def decode_libra(t_row_offset, t_blockNew_offset, t_value, t_column, t_window_row, t_binary,
-                 c_row_offset, c_row, c_column, c_value, cs_row_offset, cs_row, cs_column, cs_value):
+                 c_row_offset, c_row, c_column, c_value, cs_row_offset, cs_row, cs_column, cs_value, tf32=False):
     """COO triplets the legacy Libra metadata encodes, as the reference's
     kernels read it (TR-source/SpMM/mGCNkernel.cu):
       * TC (spmm_forward_fp16_csr_v2_kernel_tcu, :60-150): part p, blocks
@@ -280,18 +280,23 @@ def decode_libra(t_row_offset, t_blockNew_offset, t_value, t_column, t_window_ro
         lower bits] (:71-95);
       * residual (spmm_forward_fp16_csr_v2_kernel_cuda{,_short}, :977-1043,
         :1120-1171): part p adds c_value[e] B[c_column[e]] to row c_row[p].
+    tf32=True reads the tf32 members' blocks (spmm_forward_tf32_csr_v2_kernel_tcu,
+    :2886-3030): 4 columns t_column[4 b + k] and a 32-bit mask whose bit
+    l = 4 n + k is lane l's m16n8k4 B-fragment element (row k = l % 4 of
+    column n = l / 4, :2937-2949), i.e. row n, column k.
     Empty TC columns (-1) carry no nonzero."""
     rows, cols, vals = [], [], []
     t_row_offset = np.asarray(t_row_offset)
     for p in range(len(t_window_row)):
         base = 8 * int(t_window_row[p])
         for b in range(int(t_row_offset[p]), int(t_row_offset[p + 1])):
-            bits = int(np.asarray(t_binary)[b]) & ((1 << 64) - 1)
+            width = 4 if tf32 else 8
+            bits = int(np.asarray(t_binary)[b]) & ((1 << (8 * width)) - 1)
             rank = 0
-            for bit in range(64):
+            for bit in range(8 * width):
                 if (bits >> bit) & 1:
-                    n, k = bit // 8, bit % 8
-                    c = int(t_column[8 * b + k])
+                    n, k = bit // width, bit % width
+                    c = int(t_column[width * b + k])
                     if c >= 0:
                         rows.append(base + n)
                         cols.append(c)

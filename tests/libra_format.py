@@ -14,7 +14,9 @@ import numpy as np
 import torch
 
 
-def encode(csr, density=4, partsize=32, short=3):
+def encode(csr, density=4, partsize=32, short=3, tf32=False):
+    """tf32: the tf32 members' format (blocks of 4 columns, int32 masks whose bit
+    4 n + k is row n, column k; fp32 values)."""
     rowptr, col, val = csr.rowptr.astype(np.int64), csr.col.astype(np.int64), csr.val.astype(np.float32)
     m = rowptr.size - 1
     W = (m + 7) // 8
@@ -29,23 +31,25 @@ def encode(csr, density=4, partsize=32, short=3):
         tcc = sorted(c for c, n in cnt.items() if n >= density)
         if not tcc:
             continue
-        for i in range(0, len(tcc), 8):
-            chunk = tcc[i: i + 8]
+        width = 4 if tf32 else 8
+        for i in range(0, len(tcc), width):
+            chunk = tcc[i: i + width]
             pos = {c: k for k, c in enumerate(chunk)}
             bits = 0
             entries = []
             for r in range(r0, r1):
                 for e in range(rowptr[r], rowptr[r + 1]):
                     if col[e] in pos:
-                        b = 8 * (r - r0) + pos[col[e]]
+                        b = width * (r - r0) + pos[col[e]]
                         bits |= 1 << b
                         entries.append((b, val[e]))
                         tc_set.add((r, int(col[e])))
             entries.sort()
             t_value.extend(v for _, v in entries)
             t_block.append(len(t_value))
-            t_column.extend(chunk + [-1] * (8 - len(chunk)))
-            t_binary.append(bits - (1 << 64) if bits >= (1 << 63) else bits)
+            t_column.extend(chunk + [-1] * (width - len(chunk)))
+            top = 8 * width
+            t_binary.append(bits - (1 << top) if bits >= (1 << (top - 1)) else bits)
         t_row_offset.append(len(t_binary))
         t_window_row.append(w)
     longp = ([0], [], [], [])               # offsets, rows, columns, values
@@ -73,11 +77,12 @@ def encode(csr, density=4, partsize=32, short=3):
             if rest:
                 put(shortp if len(rest) <= short else longp, r, rest)
     i32 = lambda x: torch.tensor(x, dtype=torch.int32)     # noqa: E731
-    f16 = lambda x: torch.tensor(np.asarray(x, np.float32)).half()   # noqa: E731
+    f16 = ((lambda x: torch.tensor(np.asarray(x, np.float32))) if tf32       # noqa: E731
+           else (lambda x: torch.tensor(np.asarray(x, np.float32)).half()))
     meta = {
         "t_row_offset": i32(t_row_offset), "t_blockNew_offset": i32(t_block), "t_value": f16(t_value),
         "t_column": i32(t_column), "t_window_row": i32(t_window_row), "t_atomic": i32([0] * len(t_window_row)),
-        "t_binary": torch.tensor(t_binary, dtype=torch.int64),
+        "t_binary": torch.tensor(t_binary, dtype=torch.int32 if tf32 else torch.int64),
         "c_row_offset": i32(longp[0]), "c_row": i32(longp[1]), "c_atomic": i32([0] * len(longp[1])),
         "c_column": i32(longp[2]), "c_value": f16(longp[3]),
         "c_row_offset_short": i32(shortp[0]), "c_row_short": i32(shortp[1]),
@@ -94,4 +99,4 @@ def decode(meta):
                                g("t_column"), g("t_window_row"), g("t_binary"),
                                g("c_row_offset"), g("c_row"), g("c_column"), meta["c_value"].float().numpy(),
                                g("c_row_offset_short"), g("c_row_short"), g("c_column_short"),
-                               meta["c_value_short"].float().numpy())
+                               meta["c_value_short"].float().numpy(), tf32=meta["t_binary"].dtype == torch.int32)

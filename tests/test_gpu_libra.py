@@ -1,6 +1,7 @@
 """The legacy Libra kernel family (LibraSpMM / trspmm_libra_fp16_*) on the GPU.
 
-Each host function of TR-source/SpMM/mGCNkernel.cu this build replaces runs on
+Each host function of TR-source/SpMM/mGCNkernel.cu this build replaces (fp16
+and tf32 members) runs on
 legacy-format metadata written by tests/libra_format.py and is checked
 against the FP64 product of what oracle.decode_libra reads from that metadata
 (values as fp16): the full products (tcu_cuda, _seq, _hybrid, _hybrid_seq on
@@ -31,7 +32,7 @@ NAMES = ["t_row_offset", "t_blockNew_offset", "t_value", "t_column", "t_window_r
          "c_row_offset_short", "c_row_short", "c_atomic_short", "c_column_short", "c_value_short"]
 
 
-def _ref(meta, B, mOri, which="all"):
+def _ref(meta, B, mOri, which="all", tf32=False):
     m = dict(meta)
     if which == "tc":
         for k in ("c_row", "c_row_short"):
@@ -44,7 +45,8 @@ def _ref(meta, B, mOri, which="all"):
     r, c, v = oracle.decode_libra(gt("t_row_offset"), gt("t_blockNew_offset"), m["t_value"].float().numpy(),
                                   gt("t_column"), gt("t_window_row"), gt("t_binary"), gt("c_row_offset"), gt("c_row"),
                                   gt("c_column"), m["c_value"].float().numpy(), gt("c_row_offset_short"),
-                                  gt("c_row_short"), gt("c_column_short"), m["c_value_short"].float().numpy())
+                                  gt("c_row_short"), gt("c_column_short"), m["c_value_short"].float().numpy(),
+                                  tf32=tf32)
     keep = r < mOri
     return oracle.coo_spmm64(r[keep], c[keep], v[keep], B.double().numpy(), mOri)
 
@@ -118,3 +120,36 @@ def test_malformed_offsets_raise(cuda):
         L.spmm_forward_fp16_tcu_cuda_kernel(*[meta[n].to(cuda) for n in NAMES], B.to(cuda), out[:10],
                                             counts["parts_t"], counts["parts_c"], 32, counts["parts_c_short"], 64,
                                             csr.m, 1)
+
+
+TF32 = ["spmm_forward_tf32_tcu_cuda_kernel", "spmm_forward_tf32_tcu_cuda_kernel_seq"]
+
+
+@pytest.mark.parametrize("fn", TF32)
+@pytest.mark.parametrize("case", sorted(GRAPHS))
+@pytest.mark.parametrize("dimN", [64, 40])
+def test_tf32_products(cuda, fn, case, dimN):
+    """fp32 values and operand, 8 x 4 TC blocks: three bf16 products summed, against
+    the FP64 product of the fp32 metadata (finer than tf32: 3e-5 relative)."""
+    csr = GRAPHS[case]()
+    meta, counts = encode(csr, density=2, tf32=True)
+    B = torch.randn(csr.n, dimN, generator=torch.Generator().manual_seed(dimN + 5))
+    out = torch.full((csr.m, dimN), float("nan"), device=cuda)
+    print(f"[libra-tf32] {fn} {case} dimN={dimN} {counts}", flush=True)
+    ms = getattr(L, fn)(*[meta[n].to(cuda) for n in NAMES], B.to(cuda), out, counts["parts_t"], counts["parts_c"], 32,
+                        counts["parts_c_short"], dimN, csr.m, 2)
+    assert ms > 0
+    got = out.double().cpu().numpy()
+    assert np.isfinite(got).all()
+    assert _rel(got, _ref(meta, B, csr.m, tf32=True)) < 3e-5
+
+
+@pytest.mark.parametrize("case", ["er512", "rmat10"])
+def test_tf32_split_is_the_residual_route(cuda, case):
+    csr = GRAPHS[case]()
+    meta, counts = encode(csr, density=2, tf32=True)
+    B = torch.randn(csr.n, 64, generator=torch.Generator().manual_seed(9))
+    out = torch.empty((csr.m, 64), device=cuda)
+    L.spmm_forward_tf32_cuda_kernel_split(*[meta[n].to(cuda) for n in NAMES[7:]], B.to(cuda), out, 32, 64, csr.m, 2,
+                                          counts["parts_c"], counts["parts_c_short"], False)
+    assert _rel(out.double().cpu().numpy(), _ref(meta, B, csr.m, "res", tf32=True)) < 3e-5

@@ -2,11 +2,13 @@
 
 decode_libra restates how TR-source/SpMM/mGCNkernel.cu's kernels read the
 metadata.  Pinned by:
-  * the m16n8k8 fragment arithmetic it relies on: lane l reads mask bits
+  * the fragment arithmetic it relies on: m16n8k8, lane l reads mask bits
     2 l + j as B-fragment row k = 2 (l % 4) + j, column n = l / 4, i.e. bit
-    8 n + k (checked for all 32 lanes);
-  * a hand-built block (three bits, one of them the sign bit of the int64
-    mask) decoded to its rows, columns and values in bit order;
+    8 n + k; m16n8k4 (tf32), bit l as row k = l % 4, column n = l / 4, i.e.
+    bit 4 n + k (all 32 lanes);
+  * hand-built blocks (three bits, one of them the sign bit of the mask) of
+    both widths (8 columns / int64, the tf32 members' 4 columns / int32)
+    decoded to their rows, columns and values in bit order;
   * round trips through the test encoder (tests/libra_format.py), which is
     written from the format description independently: the decoded triplets
     equal the CSR's (values rounded to fp16), on graphs with and without TC
@@ -30,6 +32,22 @@ def test_fragment_bit_layout():
             assert 2 * lane + j == 8 * n + k
 
 
+def test_fragment_bit_layout_tf32():
+    """m16n8k4 (tf32): lane l holds B-fragment row k = l % 4 of column n = l / 4,
+    read as mask bit l (mGCNkernel.cu:2937-2949), i.e. bit 4 n + k."""
+    for lane in range(32):
+        k, n = lane % 4, lane // 4
+        assert lane == 4 * n + k
+
+
+def test_hand_block_tf32():
+    bits = (1 << 2) | (1 << 13) | (1 << 31)            # (row 0, col 2), (row 3, col 1), (row 7, col 3)
+    cols = np.array([11, 12, 13, 14], np.int32)
+    r, c, v = oracle.decode_libra([0, 1], [0, 3], np.array([1.5, 2.5, -3.5]), cols, [1], np.array([bits - (1 << 32)]),
+                                  [0], [], [], [], [0], [], [], [], tf32=True)
+    assert r.tolist() == [8, 11, 15] and c.tolist() == [13, 12, 14] and v.tolist() == [1.5, 2.5, -3.5]
+
+
 def test_hand_block():
     bits = (1 << 1) | (1 << 24) | (1 << 63)
     cols = np.array([5, 9, -1, -1, -1, -1, -1, 42], np.int32)
@@ -46,12 +64,14 @@ GRAPHS = {"rmat10": lambda: g.rmat_gcn(10, seed=1), "er512": CASES["er512"], "de
 
 @pytest.mark.parametrize("case", sorted(GRAPHS))
 @pytest.mark.parametrize("density", [2, 4, 9])
-def test_round_trip(case, density):
+@pytest.mark.parametrize("tf32", [False, True])
+def test_round_trip(case, density, tf32):
     csr = GRAPHS[case]()
-    meta, counts = encode(csr, density=density)
+    meta, counts = encode(csr, density=density, tf32=tf32)
     r, c, v = decode(meta)
     rows = np.repeat(np.arange(csr.rowptr.size - 1), np.diff(csr.rowptr))
-    want = sorted(zip(rows.tolist(), csr.col.tolist(), torch.from_numpy(csr.val).half().float().tolist()))
+    vals = torch.from_numpy(csr.val).float() if tf32 else torch.from_numpy(csr.val).half().float()
+    want = sorted(zip(rows.tolist(), csr.col.tolist(), vals.tolist()))
     got = sorted(zip(r.tolist(), c.tolist(), v.tolist()))
     assert got == want
     assert counts["parts_t"] == meta["t_window_row"].numel()

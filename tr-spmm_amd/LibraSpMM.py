@@ -35,6 +35,15 @@ accumulate in fp32 (the reference's kernels accumulate in fp16); the seq and
 hybrid variants, which schedule the same kernels differently in the
 reference, run as one launch here; cc_only uses no matrix instructions.
 Errors raise RuntimeError (the reference prints CUDA errors and goes on).
+
+The tf32 members take fp32 values and rhs_matrix, and TC blocks of 8 rows x 4
+columns (t_column[4 b + k], int32 masks whose bit 4 n + k is row n, column k):
+
+  spmm_forward_tf32_tcu_cuda_kernel(...), spmm_forward_tf32_tcu_cuda_kernel_seq(...)   (:3513-3837)
+  spmm_forward_tf32_cuda_kernel_split(...)                                            (:3388-3512)
+
+They run as three bf16 products (hi/lo split of values and operand) summed in
+fp32, finer than tf32, because gfx950 has no tf32 matrix instruction.
 """
 from __future__ import annotations
 
@@ -56,6 +65,9 @@ _SIGS = {
     "trspmm_libra_fp16_tcu_nomap": [_P] * 9 + [_I] * 4,
     "trspmm_libra_fp16_cuda_split": [_P] * 12 + [_I] * 7,
     "trspmm_libra_fp16_cuda_split_v1": [_P] * 12 + [_I] * 7,
+    "trspmm_libra_tf32_tcu_cuda": _FULL,
+    "trspmm_libra_tf32_tcu_cuda_seq": _FULL,
+    "trspmm_libra_tf32_cuda_split": [_P] * 12 + [_I] * 7,
 }
 
 
@@ -105,10 +117,16 @@ _RES = [("c_row_offset", torch.int32), ("c_row", torch.int32), ("c_atomic", torc
 _RES_S = [(n + "_short", dt) for n, dt in _RES]
 
 
-def _full(name, tensors, rhs_matrix, output_matrix, parts_t, parts_c, partsize_c, parts_c_short, dimN, mOri, epoches):
-    spec = _TC + _RES + _RES_S
+_TC32 = [(n, torch.float32 if n == "t_value" else (torch.int32 if n == "t_binary" else dt)) for n, dt in _TC]
+_RES32 = [(n, torch.float32 if n.startswith("c_value") else dt) for n, dt in _RES]
+_RES32_S = [(n, torch.float32 if n.startswith("c_value") else dt) for n, dt in _RES_S]
+
+
+def _full(name, tensors, rhs_matrix, output_matrix, parts_t, parts_c, partsize_c, parts_c_short, dimN, mOri, epoches,
+          tf32=False):
+    spec = _TC32 + _RES32 + _RES32_S if tf32 else _TC + _RES + _RES_S
     ts = [_dev(t, dt, n) for t, (n, dt) in zip(tensors, spec)]
-    rhs = _dev(rhs_matrix, torch.float16, "rhs_matrix")
+    rhs = _dev(rhs_matrix, torch.float32 if tf32 else torch.float16, "rhs_matrix")
     out = _out(output_matrix, mOri, dimN)
     args = [_p(t) for t in ts] + [_p(rhs), _p(out)] + [int(parts_t), int(parts_c), int(partsize_c),
                                                        int(parts_c_short), int(dimN), int(mOri), int(epoches)]
@@ -171,9 +189,10 @@ def spmm_forward_fp16_tcu_cuda_nomap_kernel(t_row_offset, t_blockNew_offset, t_v
     return _call("trspmm_libra_fp16_tcu_nomap", args, (ts, rhs))
 
 
-def _split(name, tensors, rhs_matrix, output_matrix, partsize, dimN, mOri, epoches, parts, parts_short, swizzle):
-    ts = [_dev(t, dt, n) for t, (n, dt) in zip(tensors, _RES + _RES_S)]
-    rhs = _dev(rhs_matrix, torch.float16, "rhs_matrix")
+def _split(name, tensors, rhs_matrix, output_matrix, partsize, dimN, mOri, epoches, parts, parts_short, swizzle,
+           tf32=False):
+    ts = [_dev(t, dt, n) for t, (n, dt) in zip(tensors, _RES32 + _RES32_S if tf32 else _RES + _RES_S)]
+    rhs = _dev(rhs_matrix, torch.float32 if tf32 else torch.float16, "rhs_matrix")
     out = _out(output_matrix, mOri, dimN)
     args = [_p(t) for t in ts] + [_p(rhs), _p(out), int(partsize), int(dimN), int(mOri), int(epoches), int(parts),
                                   int(parts_short), int(bool(swizzle))]
@@ -196,6 +215,33 @@ def spmm_forward_fp16_cuda_kernel_split_v1(c_row_offset, c_row, c_atomic, c_colu
                   (c_row_offset, c_row, c_atomic, c_column, c_value, c_row_offset_short, c_row_short, c_atomic_short,
                    c_column_short, c_value_short), rhs_matrix, output_matrix, partsize, dimN, mOri, epoches, parts,
                   parts_short, swizzle)
+
+
+def spmm_forward_tf32_tcu_cuda_kernel(t_row_offset, t_blockNew_offset, t_value, t_column, t_window_row, t_atomic,
+                                      t_binary, c_row_offset, c_row, c_atomic, c_column, c_value,
+                                      c_row_offset_short, c_row_short, c_atomic_short, c_column_short, c_value_short,
+                                      rhs_matrix, output_matrix, parts_t, parts_c, partsize_c, parts_c_short, dimN,
+                                      mOri, epoches):
+    return _full("trspmm_libra_tf32_tcu_cuda", locals_list(locals()), rhs_matrix, output_matrix, parts_t, parts_c,
+                 partsize_c, parts_c_short, dimN, mOri, epoches, tf32=True)
+
+
+def spmm_forward_tf32_tcu_cuda_kernel_seq(t_row_offset, t_blockNew_offset, t_value, t_column, t_window_row, t_atomic,
+                                          t_binary, c_row_offset, c_row, c_atomic, c_column, c_value,
+                                          c_row_offset_short, c_row_short, c_atomic_short, c_column_short,
+                                          c_value_short, rhs_matrix, output_matrix, parts_t, parts_c, partsize_c,
+                                          parts_c_short, dimN, mOri, epoches):
+    return _full("trspmm_libra_tf32_tcu_cuda_seq", locals_list(locals()), rhs_matrix, output_matrix, parts_t, parts_c,
+                 partsize_c, parts_c_short, dimN, mOri, epoches, tf32=True)
+
+
+def spmm_forward_tf32_cuda_kernel_split(c_row_offset, c_row, c_atomic, c_column, c_value, c_row_offset_short,
+                                        c_row_short, c_atomic_short, c_column_short, c_value_short, rhs_matrix,
+                                        output_matrix, partsize, dimN, mOri, epoches, parts, parts_short, swizzle):
+    return _split("trspmm_libra_tf32_cuda_split",
+                  (c_row_offset, c_row, c_atomic, c_column, c_value, c_row_offset_short, c_row_short, c_atomic_short,
+                   c_column_short, c_value_short), rhs_matrix, output_matrix, partsize, dimN, mOri, epoches, parts,
+                  parts_short, swizzle, tf32=True)
 
 
 def locals_list(d):

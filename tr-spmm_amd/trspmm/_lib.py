@@ -32,7 +32,8 @@ EXPORTS = [
     "trspmm_bwd_plan_set_values", "trspmm_cache_trim", "trspmm_meta_row_order", "trspmm_bwd_plan_stats",
     "trspmm_libra_fp16_tcu_cuda", "trspmm_libra_fp16_tcu_cuda_seq", "trspmm_libra_fp16_tcu_cuda_hybrid",
     "trspmm_libra_fp16_tcu_cuda_hybrid_seq", "trspmm_libra_fp16_tcu_cuda_cc_only", "trspmm_libra_fp16_tcu_nomap",
-    "trspmm_libra_fp16_cuda_split", "trspmm_libra_fp16_cuda_split_v1",
+    "trspmm_libra_fp16_cuda_split", "trspmm_libra_fp16_cuda_split_v1", "trspmm_libra_tf32_tcu_cuda",
+    "trspmm_libra_tf32_tcu_cuda_seq", "trspmm_libra_tf32_cuda_split",
 ]
 
 ENGINE_DEFAULT, ENGINE_VALU, ENGINE_MFMA = 0, 1, 2
