@@ -30,8 +30,11 @@
 #include <chrono>
 #include <cstring>   // rocPRIM's texture iterator uses memset without including it
 #include <algorithm>
+#include <array>
 #include <map>
 #include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include <rocprim/rocprim.hpp>
@@ -1427,27 +1430,28 @@ int bit_width(int64_t x) {
     return b < 1 ? 1 : b;
 }
 
-// two streams per device (default priority, created once) on which the smaller
-// size classes of the window decisions run beside the largest one
-int side_streams(hipStream_t (&out)[2]) {
+// three streams per device (default priority, created once) on which the
+// smaller size classes of the window decisions run beside the largest one
+constexpr int kSide = 3;
+int side_streams(hipStream_t (&out)[kSide]) {
     static std::mutex mu;
-    static std::map<int, std::pair<hipStream_t, hipStream_t>> made;
+    static std::map<int, std::array<hipStream_t, kSide>> made;
     int dev = 0;
     TRSPMM_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> g(mu);
     auto it = made.find(dev);
     if (it == made.end()) {
-        hipStream_t a = nullptr, b = nullptr;
-        TRSPMM_HIP(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
-        if (hipStreamCreateWithFlags(&b, hipStreamNonBlocking) != hipSuccess) {
-            (void)hipStreamDestroy(a);
-            set_error("hipStreamCreateWithFlags failed");
-            return TRSPMM_EHIP;
-        }
-        it = made.emplace(dev, std::make_pair(a, b)).first;
+        std::array<hipStream_t, kSide> st{};
+        for (int i = 0; i < kSide; ++i)
+            if (hipStreamCreateWithFlags(&st[i], hipStreamNonBlocking) != hipSuccess) {
+                (void)hipGetLastError();
+                for (int j = 0; j < i; ++j) (void)hipStreamDestroy(st[j]);
+                set_error("hipStreamCreateWithFlags failed");
+                return TRSPMM_EHIP;
+            }
+        it = made.emplace(dev, st).first;
     }
-    out[0] = it->second.first;
-    out[1] = it->second.second;
+    for (int i = 0; i < kSide; ++i) out[i] = it->second[i];
     return TRSPMM_OK;
 }
 
@@ -1455,8 +1459,8 @@ int side_streams(hipStream_t (&out)[2]) {
 // event once its work is done); a scope left before the join (an error) waits
 // for the side streams, so that no scratch is handed back under their work
 struct EventSet {
-    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
-    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t e[kSide + 1] = {};
+    hipStream_t side[kSide] = {};
     bool joined = false;
     int make() {
         for (auto& x : e) TRSPMM_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
@@ -1534,7 +1538,7 @@ __global__ __launch_bounds__(256) void k_count_valid(const int32_t* __restrict__
 
 // ---------------------------------------------------------------- host driver
 static int route_core(const int32_t* rowptr, const int32_t* col, const float* val, int64_t rows, int64_t nnz,
-                      int64_t ncols, const trspmm_route_cfg& cfg, hipStream_t s, Meta& M) {
+                      int64_t ncols, const trspmm_route_cfg& cfg, hipStream_t s, Meta& M, bool may_fork = true) {
     const auto t_begin = std::chrono::steady_clock::now();
     StreamArena A(s);      // scratch: declared first, so that every buffer below is back before it waits
     TRSPMM_CHECK(cfg.window > 0 && cfg.window <= 16, "window must be in [1, 16]");
@@ -1678,18 +1682,20 @@ static int route_core(const int32_t* rowptr, const int32_t* col, const float* va
         DevBuf wl;
         if ((rc = wl.alloc(A, 4 * all.size()))) return rc;
         TRSPMM_HIP(hipMemcpyAsync(wl.p, all.data(), 4 * all.size(), hipMemcpyHostToDevice, s));
-        // the classes are independent: the two smaller ones run on side streams
-        // while the largest (the longest windows) runs on s
-        hipStream_t side[2] = {s, s};
+        // the classes are independent: the three smaller ones run on side streams
+        // while the largest (windows sorted in global memory) runs on s.  (On
+        // R-MAT scale 18 with reordered rows classes 2 and 3 each hold a hub
+        // window of several ms: one after the other on s they took 10.6 ms.)
+        hipStream_t side[kSide] = {s, s, s};
         EventSet ev;
-        const bool fork = (at[1] > at[0]) + (at[2] > at[1]) + (at[4] > at[2]) > 1;
+        const bool fork = may_fork && (at[1] > at[0]) + (at[2] > at[1]) + (at[3] > at[2]) + (at[4] > at[3]) > 1;
         if (fork) {
             if ((rc = side_streams(side)) || (rc = ev.make())) return rc;
-            ev.side[0] = side[0];
-            ev.side[1] = side[1];
             TRSPMM_HIP(hipEventRecord(ev.e[0], s));
-            TRSPMM_HIP(hipStreamWaitEvent(side[0], ev.e[0], 0));
-            TRSPMM_HIP(hipStreamWaitEvent(side[1], ev.e[0], 0));
+            for (int i = 0; i < kSide; ++i) {
+                ev.side[i] = side[i];
+                TRSPMM_HIP(hipStreamWaitEvent(side[i], ev.e[0], 0));
+            }
         }
         for (int c = 3; c >= 0; --c) {
             const size_t n = at[c + 1] - at[c];
@@ -1704,16 +1710,16 @@ static int route_core(const int32_t* rowptr, const int32_t* col, const float* va
                                    window, cfg.density, cfg.tc_min_util, cfg.skip_tc, cfg.unit_min_nnz,
                                    wl.as<int32_t>() + at[c], caps[c]);
             else
-                hipLaunchKernelGGL(k_window_decide_wg<16>, dim3((unsigned)n), dim3(16 * kWave), lds, s, R, O, W,
-                                   window, cfg.density, cfg.tc_min_util, cfg.skip_tc, cfg.unit_min_nnz,
-                                   wl.as<int32_t>() + at[c], caps[c]);
+                hipLaunchKernelGGL(k_window_decide_wg<16>, dim3((unsigned)n), dim3(16 * kWave), lds,
+                                   c == 2 ? side[2] : s, R, O, W, window, cfg.density, cfg.tc_min_util, cfg.skip_tc,
+                                   cfg.unit_min_nnz, wl.as<int32_t>() + at[c], caps[c]);
             TRSPMM_HIP(hipGetLastError());
         }
         if (fork) {
-            TRSPMM_HIP(hipEventRecord(ev.e[1], side[0]));
-            TRSPMM_HIP(hipEventRecord(ev.e[2], side[1]));
-            TRSPMM_HIP(hipStreamWaitEvent(s, ev.e[1], 0));
-            TRSPMM_HIP(hipStreamWaitEvent(s, ev.e[2], 0));
+            for (int i = 0; i < kSide; ++i) {
+                TRSPMM_HIP(hipEventRecord(ev.e[1 + i], side[i]));
+                TRSPMM_HIP(hipStreamWaitEvent(s, ev.e[1 + i], 0));
+            }
             ev.joined = true;
         }
         TRSPMM_HIP(hipStreamSynchronize(s));    // the window list is released on return from this block
@@ -1910,22 +1916,89 @@ int gathered_rows(const Meta& M, hipStream_t s, StreamArena& A, int64_t* out) {
 
 }  // namespace
 
+// a second stream per device, on which the input-order routing runs beside a
+// reordered one
+static int aux_stream(hipStream_t* out) {
+    static std::mutex mu;
+    static std::map<int, hipStream_t> made;
+    int dev = 0;
+    TRSPMM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    auto it = made.find(dev);
+    if (it == made.end()) {
+        hipStream_t a = nullptr;
+        TRSPMM_HIP(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
+        it = made.emplace(dev, a).first;
+    }
+    *out = it->second;
+    return TRSPMM_OK;
+}
+
+// the input-order routing of a reordered route, on a host thread of its own
+// (route_core waits for the device between its stages) and stream s2 after the
+// event `ready` of the caller's stream, its size classes one after the other on
+// s2 (the side streams serve the reordered routing); joined by the destructor
+// on every path
+struct NaturalRoute {
+    std::thread t;
+    int rc = TRSPMM_OK;
+    std::string err;
+    void start(const int32_t* rowptr, const int32_t* col, const float* val, int64_t rows, int64_t nnz,
+               int64_t ncols, const trspmm_route_cfg& cfg, hipStream_t s2, hipEvent_t ready, int dev, Meta& M) {
+        t = std::thread([=, &M] {
+            if (hipSetDevice(dev) != hipSuccess || hipStreamWaitEvent(s2, ready, 0) != hipSuccess) {
+                (void)hipGetLastError();
+                rc = TRSPMM_EHIP;
+                err = "input-order routing: hipSetDevice / hipStreamWaitEvent failed";
+                return;
+            }
+            rc = route_core(rowptr, col, val, rows, nnz, ncols, cfg, s2, M, false);
+            if (rc == TRSPMM_OK && hipStreamSynchronize(s2) != hipSuccess) {
+                (void)hipGetLastError();
+                rc = TRSPMM_EHIP;
+                set_error("input-order routing: hipStreamSynchronize failed");
+            }
+            if (rc != TRSPMM_OK) err = last_error();
+        });
+    }
+    void join() {
+        if (t.joinable()) t.join();
+    }
+    ~NaturalRoute() { join(); }
+};
+
 int route_device(const int32_t* rowptr, const int32_t* col, const float* val, int64_t rows, int64_t nnz,
                  int64_t ncols, const trspmm_route_cfg& cfg, hipStream_t s, Meta& M) {
     const auto t_begin = std::chrono::steady_clock::now();
     const bool reorder = (cfg.flags & TRSPMM_ROUTE_REORDER) != 0;
     TRSPMM_CHECK(!reorder || (cfg.mode != TRSPMM_MODE_COMPAT && cfg.window == 16),
                  "row reordering needs EXACT or TUNED mode and window 16");
-    int rc = route_core(rowptr, col, val, rows, nnz, ncols, cfg, s, M);
-    if (rc || !reorder || rows < 2 || nnz == 0) return rc;
+    if (!reorder || rows < 2 || nnz == 0) return route_core(rowptr, col, val, rows, nnz, ncols, cfg, s, M);
 
-    // the rows in reordered form, routed the same way; the reordered routing is
-    // kept when it gathers at least a tenth fewer B rows.  The count does not
-    // see what the input order's windows share through the caches beyond one
-    // window (communities and bands spanning windows): on the three-route mix
-    // (2^20 rows, K = 256) 1.9 % fewer rows ran 11 % slower, on R-MAT scale 18
-    // 14 % fewer rows ran 10 % faster.
+    // Both row orders are routed, at the same time: the input order on a second
+    // stream from a second host thread, the reordered rows here.  The reordered
+    // routing is kept when it gathers at least a tenth fewer B rows.  The count
+    // does not see what the input order's windows share through the caches
+    // beyond one window (communities and bands spanning windows): on the
+    // three-route mix (2^20 rows, K = 256) 1.9 % fewer rows ran 11 % slower, on
+    // R-MAT scale 18 14 % fewer rows ran 10 % faster.
+    int rc;
+    hipStream_t s2 = nullptr;
+    int dev = 0;
+    if ((rc = aux_stream(&s2))) return rc;
+    TRSPMM_HIP(hipGetDevice(&dev));
+    struct Event {
+        hipEvent_t e = nullptr;
+        ~Event() {
+            if (e) (void)hipEventDestroy(e);
+        }
+    } ready;
+    TRSPMM_HIP(hipEventCreateWithFlags(&ready.e, hipEventDisableTiming));
+    TRSPMM_HIP(hipEventRecord(ready.e, s));
     StreamArena A(s);
+    Meta R;
+    NaturalRoute nat;                          // declared after what its thread uses: joined first
+    nat.start(rowptr, col, val, rows, nnz, ncols, cfg, s2, ready.e, dev, M);
     DevBuf keys, keys2, idx, order, cnt, prowptr, pcol, pval, pedge, tmp;
     if ((rc = keys.alloc(A, 8 * rows)) || (rc = keys2.alloc(A, 8 * rows)) || (rc = idx.alloc(A, 4 * rows)) ||
         (rc = order.alloc_cached(4 * rows)) || (rc = cnt.alloc(A, 4 * rows)) || (rc = prowptr.alloc(A, 4 * (rows + 1))) ||
@@ -1947,11 +2020,15 @@ int route_device(const int32_t* rowptr, const int32_t* col, const float* val, in
     hipLaunchKernelGGL(k_perm_rows, dim3(nblk(rows, 4)), dim3(256), 0, s, rowptr, col, val, order.as<int32_t>(),
                        prowptr.as<int32_t>(), rows, pcol.as<int32_t>(), pval.as<float>(), pedge.as<int32_t>());
     TRSPMM_HIP(hipGetLastError());
-    Meta R;
     trspmm_route_cfg c2 = cfg;
     c2.flags &= ~TRSPMM_ROUTE_REORDER;
     if ((rc = route_core(prowptr.as<int32_t>(), pcol.as<int32_t>(), pval.as<float>(), rows, nnz, ncols, c2, s, R)))
         return rc;
+    nat.join();
+    if (nat.rc != TRSPMM_OK) {
+        set_error(nat.err);
+        return nat.rc;
+    }
     int64_t g0 = 0, g1 = 0;
     if ((rc = gathered_rows(M, s, A, &g0)) || (rc = gathered_rows(R, s, A, &g1))) return rc;
     if (10 * g1 <= 9 * g0) {
