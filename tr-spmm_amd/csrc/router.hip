@@ -657,7 +657,8 @@ struct UnitFlags {
 template <int NW>
 __global__ __launch_bounds__(64 * NW) void k_window_decide_wg(RunArrays R, WinOut O, int64_t W, int window, int density,
                                                              float tc_min_util, int skip_tc, int unit_min_nnz,
-                                                             const int32_t* __restrict__ wlist, int lds_cap) {
+                                                             const int32_t* __restrict__ wlist, int lds_cap,
+                                                             int32_t host_units) {
     constexpr int T = kWave * NW;
     extern __shared__ __align__(16) uint32_t dyn_wg[];
     __shared__ int32_t hist[17], base[17];
@@ -872,6 +873,18 @@ __global__ __launch_bounds__(64 * NW) void k_window_decide_wg(RunArrays R, WinOu
     // its bit in the bitmap, from which the word is loaded when hw reaches it.
     // A ballot per 64 units skips those already at or below hw.  Pairs are
     // collected in two registers (pair k in lane k % 64) and stored 64 at a time.
+    if (host_units > 0 && nu > host_units) {
+        // the pairing runs on the host (route_core, pair_on_host): the sorted
+        // keys go to units_g, the counts known so far to O
+        if (in_lds)
+            for (int32_t p = tid; p < nu; p += T) R.units_g[rb + p] = a[p];
+        if (tid == 0) {
+            O.n_tcsub[w] = kept_total;
+            O.n_tcval[w] = tcval;
+            O.n_rest[w] = nrest;
+        }
+        return;
+    }
     if (wid != 0) return;
     int32_t hw = -1, np = 0, left_k = nu;
     auto pairing = [&](auto bits) {
@@ -983,6 +996,88 @@ __global__ __launch_bounds__(64 * NW) void k_window_decide_wg(RunArrays R, WinOu
         O.fallback[w] = (left >= 0 && !keep_left) ? 1 : 0;
         O.n_rest[w] = nrest;
     }
+}
+
+// The pairing of a window decided with host_pairing, on the host: the loop of
+// k_window_decide_wg's step g over the sorted unit keys (unit index in the low
+// 24 bits).  In sorted order, a unit above the high-water mark hw takes the
+// first index above hw that no earlier such unit holds, other than itself, and
+// hw moves there; a unit that finds none stays over, as does, when every unit
+// got a turn, the first free index above hw.  One sequential pass: on the
+// device this loop ran on one wave at about 115 ns per unit (R-MAT scale 18's
+// hub window, 23 355 units: 2.65 ms).
+// (windows of more than kHostPairUnits units, the two largest size classes:
+// their pairing takes up to milliseconds each on the device; on the host the
+// loop takes about 1 ns per unit, and the windows' keys and pairs travel in
+// one copy each way)
+constexpr int32_t kHostPairUnits = 6144;
+
+static void pair_on_host(const uint32_t* keys, int32_t nu, int32_t* pa, int32_t* pb, int32_t& np, int32_t& left_k) {
+    std::vector<uint8_t> active((size_t)nu, 0);
+    int32_t hw = -1, scan = 0, last = -1;
+    np = 0;
+    // first index > hw that is not active and is not `except` (nu if none)
+    auto search = [&](int32_t except) {
+        if (scan < hw + 1) scan = hw + 1;
+        int32_t q = scan;
+        while (q < nu && (active[q] || q == except)) ++q;
+        return q;
+    };
+    for (int32_t i = 0; i < nu; ++i) {
+        const int32_t v = (int32_t)(keys[i] & 0xffffffu);
+        if (v <= hw) continue;                      // already a partner
+        const int32_t p = search(v);
+        if (p >= nu) {                              // every index above hw but v is taken
+            last = v;
+            break;
+        }
+        active[v] = 1;
+        pa[np] = v;
+        pb[np] = p;
+        ++np;
+        hw = p;
+    }
+    left_k = nu;
+    if (last >= 0) left_k = last;
+    else if (2 * np < nu) left_k = search(-1);
+}
+
+// the keys of the windows paired on the host, packed: block b copies the
+// sorted unit keys of window tab[4 b] to keys[tab[4 b + 1] ..]
+__global__ __launch_bounds__(256) void k_pack_units(RunArrays R, WinOut O, const int32_t* __restrict__ tab,
+                                                    uint32_t* __restrict__ keys) {
+    const int64_t w = tab[4 * blockIdx.x];
+    const int32_t off = tab[4 * blockIdx.x + 1];
+    const int32_t rb = R.win_rb[w], nu = (O.n_rest[w] + 1) / 2;
+    for (int32_t k = threadIdx.x; k < nu; k += blockDim.x) keys[off + k] = R.units_g[rb + k];
+}
+
+// the rest of a window paired on the host: block b places the pairs of window
+// tab[4 b] (tab[4 b + 2] of them, packed from tab[4 b + 1]) in pair_a / pair_b
+// and writes its counts; tab[4 b + 3] is the leftover unit (nu if none)
+__global__ __launch_bounds__(256) void k_window_finish(RunArrays R, WinOut O, const int32_t* __restrict__ tab,
+                                                       const int32_t* __restrict__ pa, const int32_t* __restrict__ pb,
+                                                       int unit_min_nnz) {
+    const int64_t w = tab[4 * blockIdx.x];
+    const int32_t off = tab[4 * blockIdx.x + 1], np = tab[4 * blockIdx.x + 2], left_k = tab[4 * blockIdx.x + 3];
+    const int32_t rb = R.win_rb[w];
+    for (int32_t k = threadIdx.x; k < np; k += blockDim.x) {
+        R.pair_a[rb + k] = pa[off + k];
+        R.pair_b[rb + k] = pb[off + k];
+    }
+    if (threadIdx.x != 0) return;
+    const int32_t nrest = O.n_rest[w], nu = (nrest + 1) / 2;
+    const int32_t left = left_k < nu ? left_k : -1;
+    int32_t left_nnz = 0;
+    if (left >= 0) {
+        left_nnz = run_count(R, R.rest[rb + 2 * left]);
+        if (2 * left + 1 < nrest) left_nnz += run_count(R, R.rest[rb + 2 * left + 1]);
+    }
+    const bool keep_left = left >= 0 && left_nnz >= unit_min_nnz;
+    O.n_pairs[w] = np;
+    O.n_groups[w] = np + (keep_left ? 1 : 0);
+    O.leftover[w] = left;
+    O.fallback[w] = (left >= 0 && !keep_left) ? 1 : 0;
 }
 
 __global__ void k_widen(const uint8_t* __restrict__ in, int64_t n, int32_t* __restrict__ out) {
@@ -1708,11 +1803,11 @@ static int route_core(const int32_t* rowptr, const int32_t* col, const float* va
             else if (c == 1)
                 hipLaunchKernelGGL(k_window_decide_wg<4>, dim3((unsigned)n), dim3(4 * kWave), lds, side[1], R, O, W,
                                    window, cfg.density, cfg.tc_min_util, cfg.skip_tc, cfg.unit_min_nnz,
-                                   wl.as<int32_t>() + at[c], caps[c]);
+                                   wl.as<int32_t>() + at[c], caps[c], 0);
             else
                 hipLaunchKernelGGL(k_window_decide_wg<16>, dim3((unsigned)n), dim3(16 * kWave), lds,
                                    c == 2 ? side[2] : s, R, O, W, window, cfg.density, cfg.tc_min_util, cfg.skip_tc,
-                                   cfg.unit_min_nnz, wl.as<int32_t>() + at[c], caps[c]);
+                                   cfg.unit_min_nnz, wl.as<int32_t>() + at[c], caps[c], kHostPairUnits);
             TRSPMM_HIP(hipGetLastError());
         }
         if (fork) {
@@ -1722,7 +1817,62 @@ static int route_core(const int32_t* rowptr, const int32_t* col, const float* va
             }
             ev.joined = true;
         }
-        TRSPMM_HIP(hipStreamSynchronize(s));    // the window list is released on return from this block
+        TRSPMM_HIP(hipStreamSynchronize(s));
+        // windows of classes 2 and 3 with more than kHostPairUnits units: the
+        // pairing on the host, their keys packed into one download and their
+        // pairs into one upload
+        std::vector<int32_t> hostw;
+        for (size_t k = at[2]; k < at[4]; ++k)
+            if ((rb_h[all[k] + 1] - rb_h[all[k]] + 1) / 2 > kHostPairUnits) hostw.push_back(all[k]);
+        std::vector<int32_t> nrest_h;
+        if (!hostw.empty()) {
+            nrest_h.resize((size_t)W);
+            TRSPMM_HIP(hipMemcpyAsync(nrest_h.data(), O.n_rest, 4 * W, hipMemcpyDeviceToHost, s));
+            TRSPMM_HIP(hipStreamSynchronize(s));
+            size_t keep = 0;                           // the windows whose units exceed the threshold
+            for (int32_t w : hostw)
+                if ((nrest_h[w] + 1) / 2 > kHostPairUnits) hostw[keep++] = w;
+            hostw.resize(keep);
+        }
+        const int64_t nbig = (int64_t)hostw.size();
+        if (nbig > 0) {
+            // per window: [0] the window, [1] its first packed unit, [2] pairs, [3] leftover
+            std::vector<int32_t> tab((size_t)(4 * nbig));
+            int64_t total = 0;
+            for (int64_t b = 0; b < nbig; ++b) {
+                tab[4 * b] = hostw[b];
+                tab[4 * b + 1] = (int32_t)total;
+                total += (nrest_h[hostw[b]] + 1) / 2;
+            }
+            TRSPMM_CHECK(total < ((int64_t)1 << 31), "router: too many units in the hub windows");
+            DevBuf tab_d, keys_d, pa_d, pb_d;
+            if ((rc = tab_d.alloc(A, 16 * nbig)) || (rc = keys_d.alloc(A, 4 * total)) || (rc = pa_d.alloc(A, 4 * total)) ||
+                (rc = pb_d.alloc(A, 4 * total)))
+                return rc;
+            TRSPMM_HIP(hipMemcpyAsync(tab_d.p, tab.data(), 16 * nbig, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_pack_units, dim3((unsigned)nbig), dim3(256), 0, s, R, O, tab_d.as<int32_t>(),
+                               keys_d.as<uint32_t>());
+            TRSPMM_HIP(hipGetLastError());
+            std::vector<uint32_t> keys((size_t)total);
+            std::vector<int32_t> pa((size_t)total), pb((size_t)total);
+            TRSPMM_HIP(hipMemcpyAsync(keys.data(), keys_d.p, 4 * total, hipMemcpyDeviceToHost, s));
+            TRSPMM_HIP(hipStreamSynchronize(s));
+            for (int64_t b = 0; b < nbig; ++b) {
+                const int32_t off = tab[4 * b + 1];
+                const int32_t nu = (nrest_h[hostw[b]] + 1) / 2;
+                int32_t np = 0, left_k = nu;
+                if (nu > 0) pair_on_host(keys.data() + off, nu, pa.data() + off, pb.data() + off, np, left_k);
+                tab[4 * b + 2] = np;
+                tab[4 * b + 3] = left_k;
+            }
+            TRSPMM_HIP(hipMemcpyAsync(tab_d.p, tab.data(), 16 * nbig, hipMemcpyHostToDevice, s));
+            TRSPMM_HIP(hipMemcpyAsync(pa_d.p, pa.data(), 4 * total, hipMemcpyHostToDevice, s));
+            TRSPMM_HIP(hipMemcpyAsync(pb_d.p, pb.data(), 4 * total, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_window_finish, dim3((unsigned)nbig), dim3(256), 0, s, R, O, tab_d.as<int32_t>(),
+                               pa_d.as<int32_t>(), pb_d.as<int32_t>(), cfg.unit_min_nnz);
+            TRSPMM_HIP(hipGetLastError());
+            TRSPMM_HIP(hipStreamSynchronize(s));    // the host vectors are in the copies above
+        }
     }
 
     // 4. window prefix sums -> output sizes
