@@ -571,7 +571,9 @@ __global__ __launch_bounds__(64) void k_window_decide(RunArrays R, WinOut O, int
 //   g    the pairing on one wave, in scalar registers, with the active-unit
 //        flags as a bitmap: the 64-flag word that holds the high-water mark
 //        stays in two SGPRs, so a partner search is a masked find-first-zero
-//        and a word load only when the mark passes the word.
+//        and a word load only when the mark passes the word.  Windows of more
+//        than kHostPairUnits units stop before it: route_core pairs them on
+//        the host (pair_on_host) and k_window_finish completes them.
 
 constexpr int kKeyBuckets = 64;      // unit keys (nnz of two runs of a <= 16-row window) are < 33
 
