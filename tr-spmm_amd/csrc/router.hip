@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <array>
 #include <map>
+#include <exception>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -2104,8 +2105,13 @@ struct NaturalRoute {
                 err = "input-order routing: hipSetDevice / hipStreamWaitEvent failed";
                 return;
             }
-            rc = route_core(rowptr, col, val, rows, nnz, ncols, cfg, s2, M, false);
-            if (rc == TRSPMM_OK && hipStreamSynchronize(s2) != hipSuccess) {
+            try {          // an exception must not leave the thread (std::terminate)
+                rc = route_core(rowptr, col, val, rows, nnz, ncols, cfg, s2, M, false);
+            } catch (const std::exception& e) {
+                rc = TRSPMM_ENOMEM;
+                set_error(std::string("input-order routing: ") + e.what());
+            }
+            if (hipStreamSynchronize(s2) != hipSuccess && rc == TRSPMM_OK) {
                 (void)hipGetLastError();
                 rc = TRSPMM_EHIP;
                 set_error("input-order routing: hipStreamSynchronize failed");
