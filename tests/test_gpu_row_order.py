@@ -167,3 +167,18 @@ def test_reordered_routing_needs_the_deep_kernel(cuda):
     E = RoutedSparse(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val), csr.n,
                      m=csr.m, mode="exact")
     assert not E.meta.reordered
+
+
+def test_reordered_routing_reports_bad_input_and_recovers(cuda):
+    """Both row orders are routed at the same time (the input order on a second
+    host thread): a column out of range is reported as an error, not a hang,
+    and the next routing works."""
+    csr = GRAPHS["rmat10"]()
+    col = csr.col.copy()
+    col[len(col) // 2] = csr.n + 5
+    with pytest.raises(TrspmmError):
+        route(torch.from_numpy(csr.rowptr), torch.from_numpy(col), torch.from_numpy(csr.val), csr.n,
+              mode="tuned", reorder=True)
+    meta = route(torch.from_numpy(csr.rowptr), torch.from_numpy(csr.col), torch.from_numpy(csr.val), csr.n,
+                 mode="tuned", reorder=True)
+    assert meta.reordered
