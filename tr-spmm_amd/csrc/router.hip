@@ -1636,7 +1636,7 @@ __global__ __launch_bounds__(256) void k_count_valid(const int32_t* __restrict__
 
 // ---------------------------------------------------------------- host driver
 static int route_core(const int32_t* rowptr, const int32_t* col, const float* val, int64_t rows, int64_t nnz,
-                      int64_t ncols, const trspmm_route_cfg& cfg, hipStream_t s, Meta& M, bool may_fork = true) {
+                      int64_t ncols, const trspmm_route_cfg& cfg, hipStream_t s, Meta& M) {
     const auto t_begin = std::chrono::steady_clock::now();
     StreamArena A(s);      // scratch: declared first, so that every buffer below is back before it waits
     TRSPMM_CHECK(cfg.window > 0 && cfg.window <= 16, "window must be in [1, 16]");
@@ -1786,7 +1786,7 @@ static int route_core(const int32_t* rowptr, const int32_t* col, const float* va
         // window of several ms: one after the other on s they took 10.6 ms.)
         hipStream_t side[kSide] = {s, s, s};
         EventSet ev;
-        const bool fork = may_fork && (at[1] > at[0]) + (at[2] > at[1]) + (at[3] > at[2]) + (at[4] > at[3]) > 1;
+        const bool fork = (at[1] > at[0]) + (at[2] > at[1]) + (at[3] > at[2]) + (at[4] > at[3]) > 1;
         if (fork) {
             if ((rc = side_streams(side)) || (rc = ev.make())) return rc;
             TRSPMM_HIP(hipEventRecord(ev.e[0], s));
@@ -2089,9 +2089,9 @@ static int aux_stream(hipStream_t* out) {
 
 // the input-order routing of a reordered route, on a host thread of its own
 // (route_core waits for the device between its stages) and stream s2 after the
-// event `ready` of the caller's stream, its size classes one after the other on
-// s2 (the side streams serve the reordered routing); joined by the destructor
-// on every path
+// event `ready` of the caller's stream; its smaller size classes share the side
+// streams with the reordered routing (each fork waits for its own events only).
+// Joined by the destructor on every path
 struct NaturalRoute {
     std::thread t;
     int rc = TRSPMM_OK;
@@ -2106,7 +2106,7 @@ struct NaturalRoute {
                 return;
             }
             try {          // an exception must not leave the thread (std::terminate)
-                rc = route_core(rowptr, col, val, rows, nnz, ncols, cfg, s2, M, false);
+                rc = route_core(rowptr, col, val, rows, nnz, ncols, cfg, s2, M);
             } catch (const std::exception& e) {
                 rc = TRSPMM_ENOMEM;
                 set_error(std::string("input-order routing: ") + e.what());
