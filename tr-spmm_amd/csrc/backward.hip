@@ -414,8 +414,10 @@ __global__ void k_invert(const int32_t* __restrict__ perm, int64_t n, int32_t* _
 // The grid is 8 * per blocks and block b takes span (b % 8) * per + b / 8, so
 // XCD x (blocks go to the 8 XCDs round robin) walks one contiguous eighth of
 // dst: entries of nearby rows of A that share a column sit next to each other
-// in src, and their lines then meet in one L2.
-constexpr int GATHER_PER = 4;
+// in src, and their lines then meet in one L2.  (Fused backward on R-MAT scale
+// 18, K = 256, in one process: 1 or 2 entries per lane 434.5-435.3 us, 4 or 8
+// 437.0-437.7 us, 16 443.8-444.9 us.)
+constexpr int GATHER_PER = 2;
 constexpr int64_t kGatherSpan = 256 * GATHER_PER;
 
 __global__ __launch_bounds__(256) void k_gather_f32(const float* __restrict__ src, const int32_t* __restrict__ idx,
